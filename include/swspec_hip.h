@@ -1,0 +1,249 @@
+/*
+ * swspec_hip.h -- C ABI of the MI355X spectrometer core (TaskCoreHIP).
+ *
+ * Drop-in boundary: each entry point replaces one virtual of the reference's
+ * compute-core plugin interface, class TaskCore (src/TaskCore.h:36-89), whose
+ * only complete implementation is TaskCoreIPP (src/IA-32/TaskCoreIPP.cpp).
+ * The reference dispatcher (src/TaskDispatcher.cpp:92-247) drives a core with
+ * prepare -> { run ; join ; combineResults | write } ... -> finalize from one
+ * thread; the same call order applies here.  Plain C types only: pointers,
+ * sizes, integers.  Return codes: 0 = ok, negative = error (text from
+ * swspec_hip_last_error).
+ */
+#ifndef SWSPEC_HIP_H
+#define SWSPEC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWSPEC_HIP_ABI_VERSION 1
+
+/* error codes */
+#define SWSPEC_OK 0
+#define SWSPEC_EINVAL (-1)      /* bad argument or configuration */
+#define SWSPEC_EUNSUPPORTED (-2)/* configuration the core does not implement */
+#define SWSPEC_EDEVICE (-3)     /* HIP runtime error */
+#define SWSPEC_ENOMEM (-4)      /* device allocation refused */
+#define SWSPEC_ESTATE (-5)      /* call out of order (e.g. join without run) */
+
+/* window / format enumerations keep the reference's values (src/Settings.h:45-47) */
+enum swspec_window { SWSPEC_WIN_NONE = 0, SWSPEC_WIN_COSINE, SWSPEC_WIN_COSINE2, SWSPEC_WIN_HAMMING,
+                     SWSPEC_WIN_HANN, SWSPEC_WIN_BLACKMAN };
+enum swspec_format { SWSPEC_FMT_UNKNOWN = -1, SWSPEC_FMT_RAWSIGNED = 0, SWSPEC_FMT_RAWUNSIGNED,
+                     SWSPEC_FMT_MK5B, SWSPEC_FMT_IBOB, SWSPEC_FMT_VDIF, SWSPEC_FMT_VLBA, SWSPEC_FMT_MKIV,
+                     SWSPEC_FMT_MARK5B, SWSPEC_FMT_MAXIM };
+
+/* flags */
+#define SWSPEC_FLAG_NO_KAHAN 0x1u   /* plain FP32 accumulation (accuracy comparisons only) */
+
+/*
+ * Flat copy of the derived settings a core reads (src/Settings.h:59-159,
+ * derived in src/swspectrometer.cpp:161-231).  All fields are the reference's
+ * values after main() has run, e.g. use_channel_file1 is 0-based.
+ */
+typedef struct swspec_hip_cfg {
+    uint32_t struct_size;          /* sizeof(swspec_hip_cfg) */
+    uint32_t abi_version;          /* SWSPEC_HIP_ABI_VERSION */
+    int64_t  fft_points;
+    int32_t  fft_overlap_factor;
+    int32_t  fft_ssb_points;
+    int32_t  window;               /* swspec_window */
+    int32_t  bits_per_sample;
+    int32_t  source_channels;
+    int32_t  channelorder_increasing;
+    int32_t  source_format;        /* swspec_format */
+    int32_t  use_channel_file1;    /* 0-based */
+    int32_t  use_channel_file2;    /* 0-based */
+    int32_t  num_sources;          /* 1 or 2 */
+    int32_t  num_xpols;            /* 0 or 1 (pair 0,1) */
+    int64_t  raw_fullfft_bytes;
+    int64_t  raw_overlap_bytes;
+    int32_t  core_overlapped_ffts; /* K: spectra are scaled by 1/K */
+    int32_t  max_spectra_per_buffer;
+    int32_t  extract_pcal;
+    int32_t  pcal_tonebins;
+    int32_t  pcal_rotatorlen;
+    int32_t  device;               /* HIP device ordinal */
+    double   samplingfreq;
+    double   pcaloffsethz;
+    double   pcalharmonicshz;
+    int64_t  max_rawbuf_bytes;     /* largest raw buffer run() will be given */
+    uint32_t flags;                /* SWSPEC_FLAG_* */
+    uint32_t ffts_per_chunk;       /* 0 = automatic */
+} swspec_hip_cfg;
+
+typedef struct swspec_hip_core swspec_hip_core;
+
+/*
+ * INI-level settings -> derived settings, restating swspectrometer.cpp:67-231
+ * (float arithmetic included).  max_source_buffer_mb < 0 means the key is
+ * absent (platform default platform_max_raw_mb, src/Settings.h:30-36).
+ * Returns 0, SWSPEC_EINVAL for an invalid channel (:148-155) or for
+ * ExtractPCal with PCalOffsetHz == 0 (the reference divides by zero there).
+ */
+typedef struct swspec_hip_ini {
+    int32_t num_cores;
+    double  max_source_buffer_mb;
+    int32_t platform_max_raw_mb;
+    int64_t fft_points;
+    float   fft_integ_seconds;
+    int32_t fft_overlap_factor;
+    int32_t window;
+    float   bandwidth_hz;
+    float   pcal_offset_hz;
+    float   pcal_harmonics_hz;
+    int32_t source_format;
+    int32_t bits_per_sample;
+    int32_t channelorder_increasing;
+    int32_t source_channels;
+    int32_t use_channel_file1;     /* 1-based, as in the INI */
+    int32_t use_channel_file2;
+    int32_t extract_pcal;
+    int32_t num_sources;
+    int32_t do_xpol;
+} swspec_hip_ini;
+
+typedef struct swspec_hip_derived {
+    int32_t averaged_ffts;
+    int32_t fft_overlap_points;
+    int32_t fft_ssb_points;
+    int64_t raw_fullfft_bytes;
+    int64_t raw_overlap_bytes;
+    int64_t rawbuf_size;
+    int32_t max_spectra_per_buffer;
+    int32_t max_specffts_per_buffer;
+    int32_t max_buffers_per_spectrum;
+    int32_t core_overlapped_ffts;
+    int32_t core_averaged_ffts;
+    int32_t extract_pcal;
+    int32_t pcal_tonebins;
+    int32_t pcal_rotatorlen;
+    int32_t pcal_pulses;
+    int32_t pcal_pulses_per_fft;
+    int32_t num_xpols;
+    int32_t use_channel_file1;     /* 0-based */
+    int32_t use_channel_file2;
+    double  samplingfreq;
+    int64_t max_rawbuf_size;
+} swspec_hip_derived;
+
+int swspec_hip_derive(const swspec_hip_ini* ini, swspec_hip_derived* out);
+
+/* Fill a cfg from ini + derived (convenience for hosts without the reference's main). */
+int swspec_hip_cfg_from_ini(const swspec_hip_ini* ini, const swspec_hip_derived* d, swspec_hip_cfg* cfg);
+
+/*
+ * TaskCore::TaskCore(int rank) + TaskCore::prepare(settings)
+ * (src/TaskCore.h:39,47; TaskCoreIPP.cpp:70-145): picks the decoder
+ * (DataUnpackerFactory.h:38-108), builds window, FFT plan, phase-cal rotator
+ * and device buffers.  Unlike the reference (which logs and continues with a
+ * NULL unpacker, :79-83) an unsupported configuration fails here.
+ */
+int swspec_hip_create(const swspec_hip_cfg* cfg, int rank, swspec_hip_core** core);
+
+/*
+ * TaskCore::run (src/TaskCore.h:58; TaskCoreIPP.cpp:155-166), asynchronous.
+ * in[s]      : host raw buffer of source s (in_len[s] bytes, Buffer::getLength)
+ * out_auto[s]: host float array of max(spectra,1) * fft_ssb_points
+ * out_xpol[0]: host float array of 2 * max(spectra,1) * fft_ssb_points (or NULL)
+ * out_pcal[s]: host float array of 2 * max(spectra,1) * pcal_tonebins (or NULL)
+ * Output arrays are written by the time swspec_hip_join returns; input
+ * buffers must stay untouched until then (the dispatcher contract,
+ * TaskDispatcher.cpp:119-139).  Pinned host memory (swspec_hip_host_alloc)
+ * makes the copies asynchronous.
+ */
+int swspec_hip_run(swspec_hip_core* core, const void* const* in, const size_t* in_len,
+                   float* const* out_auto, float* const* out_xpol, float* const* out_pcal);
+
+/*
+ * Several channels of source 0 in one pass: one job per entry of channels[]
+ * (0-based), outputs out_auto[j] / out_pcal[j] per job, no cross-power.
+ * No TaskCore counterpart: the reference runs one core per UseFile1Channel
+ * value over the same file (src/swspectrometer.cpp:126, :162); this does the
+ * same channels in one pass over the raw bytes.
+ */
+int swspec_hip_create_multichannel(const swspec_hip_cfg* cfg, int nchannels, const int* channels, int rank,
+                                   swspec_hip_core** core);
+
+/* Same as swspec_hip_run with device-resident inputs and outputs. */
+int swspec_hip_run_device(swspec_hip_core* core, const void* const* d_in, const size_t* in_len,
+                          float* const* d_out_auto, float* const* d_out_xpol, float* const* d_out_pcal);
+
+/*
+ * TaskCore::join (src/TaskCore.h:65; TaskCoreIPP.cpp:173-183): blocks until
+ * the run completes.  *nspectra = complete + partial spectra (:519-524);
+ * output lengths in bytes as Buffer::setLength would carry them (:527-541).
+ * Returns 0 or a negative error (the reference's join() == -1).
+ */
+int swspec_hip_join(swspec_hip_core* core, int* nspectra, size_t* out_len_auto, size_t* out_len_xpol,
+                    size_t* out_len_pcal);
+
+/*
+ * TaskCore::combineResults (src/TaskCore.h:81; TaskCoreIPP.cpp:311-349):
+ * adds the core's last outputs (first spectrum of each) into the combined
+ * host arrays and clears the core's outputs.  combined_out has num_sources
+ * auto arrays followed by num_xpols cross arrays; lengths set as the
+ * reference does (cross length = fft_ssb_points floats, :333).
+ */
+int swspec_hip_combine(swspec_hip_core* core, float* const* combined_out, float* const* combined_pcal,
+                       size_t* combined_out_len, size_t* combined_pcal_len);
+
+/* TaskCore::resetBuffer (src/TaskCore.h:87; TaskCoreIPP.cpp:297-300): zero floats. */
+void swspec_hip_reset(void* buf, size_t bytes);
+
+/* TaskCore::finalize (src/TaskCore.h:71; TaskCoreIPP.cpp:189-227): release all. */
+int swspec_hip_finalize(swspec_hip_core* core);
+
+/* Last error text of a core (or of the library when core == NULL). */
+const char* swspec_hip_last_error(const swspec_hip_core* core);
+
+/*
+ * Phase-cal tones of spectrum `spectrum` of source `source` from the last run:
+ * amplitude/phase source data in FP64, tone n = bin n*floor(T*spacing/fs) of
+ * the 1/T-scaled forward DFT of the folded vector (PCal.cpp:489-515).
+ * tones: 2*ntones doubles (re, im).  No counterpart call in TaskCore: the
+ * reference writes only the folded vector; this exposes the FP64 extraction.
+ */
+int swspec_hip_pcal_tones(swspec_hip_core* core, int source, int spectrum, double* tones, int ntones);
+
+/* Tones per source the core extracts (0 when phase-cal is off). */
+int swspec_hip_pcal_ntones(const swspec_hip_core* core);
+
+/* Internal calculation time of the last run in seconds (device events),
+ * the figure TaskCoreIPP::finalize reports (:204-206). */
+double swspec_hip_last_run_seconds(const swspec_hip_core* core);
+
+/*
+ * Per-kernel timing with HIP events recorded on the core's stream around each
+ * launch (no TaskCore counterpart; the reference times whole doMaths calls,
+ * TaskCoreIPP.cpp:390-391,516-517).  kernel: SWSPEC_K_* below.  Totals
+ * accumulate over runs while profiling is on; set_profiling(core, 1) clears.
+ */
+#define SWSPEC_K_PASS1 0
+#define SWSPEC_K_PASS2 1
+#define SWSPEC_K_PCAL 2
+#define SWSPEC_K_FINALIZE 3
+int swspec_hip_set_profiling(swspec_hip_core* core, int on);
+int swspec_hip_kernel_stats(const swspec_hip_core* core, int kernel, double* total_ms, int64_t* launches);
+
+/* Pinned host memory for Buffer rings (hipHostMalloc) and its release. */
+void* swspec_hip_host_alloc(size_t bytes);
+void swspec_hip_host_free(void* p);
+
+/* Decoder only (no counterpart call: DataUnpacker::extract_samples,
+ * src/IA-32/DataUnpacker.h:32-38): unpack `count` samples of `channel` from
+ * device raw bytes into device floats.  Used to check bit-exactness. */
+int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d_raw, int channel, int64_t count,
+                             float* d_out);
+
+/* Plan geometry for an FFT length (no device work): N1, N2, B, threads. */
+int swspec_hip_plan_info(int64_t fft_points, int32_t* info, int ninfo);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,244 @@
+"""ctypes view of the CPU oracle (oracle/swspec_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The product (swspec_amd) never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libswspec_oracle.so")
+
+WINDOWS = {"None": 0, "Cosine": 1, "Cosine2": 2, "Hamming": 3, "Hann": 4, "Blackman": 5}
+FORMATS = {"RawSigned": 0, "RawUnsigned": 1, "Mk5B": 2, "iBOB": 3, "VDIF": 4, "VLBA": 5,
+           "MKIV": 6, "Mark5B": 7, "Maxim": 8}
+
+
+class Ini(C.Structure):
+    _fields_ = [
+        ("num_cores", C.c_int), ("max_source_buffer_mb", C.c_double), ("platform_max_raw_mb", C.c_int),
+        ("fft_points", C.c_long), ("fft_integ_seconds", C.c_float), ("fft_overlap_factor", C.c_int),
+        ("window", C.c_int), ("bandwidth_hz", C.c_float), ("pcal_offset_hz", C.c_float),
+        ("pcal_harmonics_hz", C.c_float), ("source_format", C.c_int), ("bits_per_sample", C.c_int),
+        ("channelorder_increasing", C.c_int), ("source_channels", C.c_int),
+        ("use_channel_file1", C.c_int), ("use_channel_file2", C.c_int), ("extract_pcal", C.c_int),
+        ("num_sources", C.c_int), ("do_xpol", C.c_int),
+    ]
+
+
+class Settings(C.Structure):
+    _fields_ = [
+        ("num_cores", C.c_int), ("fft_points", C.c_long), ("fft_integ_seconds", C.c_float),
+        ("fft_overlap_factor", C.c_int), ("window", C.c_int), ("bits_per_sample", C.c_int),
+        ("channelorder_increasing", C.c_int), ("source_channels", C.c_int),
+        ("use_channel_file1", C.c_int), ("use_channel_file2", C.c_int),
+        ("samplingfreq", C.c_float), ("pcaloffsethz", C.c_float), ("pcalharmonicshz", C.c_float),
+        ("source_format", C.c_int), ("extract_pcal", C.c_int), ("num_sources", C.c_int),
+        ("num_xpols", C.c_int), ("max_rawbuf_size", C.c_size_t), ("dt", C.c_float), ("df", C.c_float),
+        ("averaged_ffts", C.c_int), ("fft_overlap_points", C.c_int), ("fft_ssb_points", C.c_int),
+        ("rawbytes_per_channelsample", C.c_double), ("raw_fullfft_bytes", C.c_size_t),
+        ("raw_overlap_bytes", C.c_size_t), ("fft_bytes", C.c_size_t), ("fft_bytes_ssb", C.c_size_t),
+        ("fft_bytes_xpol", C.c_size_t), ("rawbuf_size", C.c_size_t), ("max_spectra_per_buffer", C.c_int),
+        ("max_specffts_per_buffer", C.c_int), ("max_buffers_per_spectrum", C.c_int),
+        ("core_overlapped_ffts", C.c_int), ("core_averaged_ffts", C.c_int), ("pcal_tonebins", C.c_int),
+        ("pcal_rotatorlen", C.c_int), ("pcal_pulses", C.c_int), ("pcal_pulses_per_fft", C.c_int),
+        ("pcal_result_bytes", C.c_size_t),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def build():
+    """Compile the oracle with gcc (a checker, not product code)."""
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.so_derive.argtypes = [C.POINTER(Ini), C.POINTER(Settings)]
+        L.so_gcd.restype = C.c_longlong
+        L.so_gcd.argtypes = [C.c_longlong, C.c_longlong]
+        L.so_pick_unpacker.argtypes = [C.POINTER(Settings)]
+        L.so_unpack.restype = C.c_size_t
+        L.so_unpack.argtypes = [C.POINTER(Settings), P, P, C.c_size_t, C.c_int]
+        L.so_window_table.argtypes = [C.c_int, C.c_long, P]
+        L.so_window_table64.argtypes = [C.c_int, C.c_long, P]
+        L.so_rfft64.argtypes = [P, C.c_long, P]
+        L.so_rfft32.argtypes = [P, C.c_long, P]
+        L.so_cfft64.argtypes = [P, C.c_long]
+        L.so_count_spectra.argtypes = [C.POINTER(Settings), C.c_size_t, C.POINTER(C.c_int)]
+        L.so_domaths.argtypes = [C.POINTER(Settings), C.c_int, C.POINTER(P), C.POINTER(C.c_size_t),
+                                 C.POINTER(P), C.POINTER(P), C.POINTER(P)]
+        L.so_extract_pcal32.argtypes = [C.POINTER(Settings), P, C.c_long, P]
+        L.so_extract_pcal64.argtypes = [C.POINTER(Settings), P, C.c_long, P]
+        L.so_pcal_fold64.argtypes = [P, C.c_long, C.c_double, C.c_double, C.c_int, C.c_long,
+                                     C.c_uint64, C.c_int, P]
+        L.so_pcal_tones64.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_double, P]
+        L.so_pcal_ntones.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.so_pcal_reference64.argtypes = [P, C.c_long, C.c_double, C.c_double, C.c_double, C.c_uint64, P]
+        L.so_mk5b_strip.restype = C.c_long
+        L.so_mk5b_strip.argtypes = [P, C.c_long, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def make_ini(**kw):
+    """Defaults of src/swspectrometer.cpp:67-91 (platform INTEL_IPP: 32 MB)."""
+    d = dict(num_cores=1, max_source_buffer_mb=-1.0, platform_max_raw_mb=32, fft_points=320000,
+             fft_integ_seconds=20.0, fft_overlap_factor=2, window=WINDOWS["Cosine2"], bandwidth_hz=8e6,
+             pcal_offset_hz=10e3, pcal_harmonics_hz=1e6, source_format=FORMATS["RawSigned"],
+             bits_per_sample=8, channelorder_increasing=1, source_channels=1, use_channel_file1=1,
+             use_channel_file2=1, extract_pcal=0, num_sources=1, do_xpol=0)
+    for k, v in kw.items():
+        if k == "window" and isinstance(v, str):
+            v = WINDOWS[v]
+        if k == "source_format" and isinstance(v, str):
+            v = FORMATS[v]
+        d[k] = v
+    ini = Ini()
+    for k, v in d.items():
+        setattr(ini, k, v)
+    return ini
+
+
+def derive(**kw):
+    ini = make_ini(**kw)
+    s = Settings()
+    rc = lib().so_derive(C.byref(ini), C.byref(s))
+    if rc != 0:
+        raise ValueError("so_derive failed rc=%d" % rc)
+    return s
+
+
+def gcd(a, b):
+    return lib().so_gcd(a, b)
+
+
+def unpack(s, raw, count, channel):
+    raw = np.ascontiguousarray(raw, dtype=np.uint8)
+    out = np.zeros(count + 32, dtype=np.float32)
+    n = lib().so_unpack(C.byref(s), _p(raw), _p(out), count, channel)
+    return out[:n]
+
+
+def window(kind, n, fp64=False):
+    if isinstance(kind, str):
+        kind = WINDOWS[kind]
+    if fp64:
+        out = np.empty(n, dtype=np.float64)
+        lib().so_window_table64(kind, n, _p(out))
+    else:
+        out = np.empty(n, dtype=np.float32)
+        lib().so_window_table(kind, n, _p(out))
+    return out
+
+
+def rfft64(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    X = np.empty(2 * (len(x) // 2 + 1), dtype=np.float64)
+    if lib().so_rfft64(_p(x), len(x), _p(X)) != 0:
+        raise ValueError("unsupported length")
+    return X[0::2] + 1j * X[1::2]
+
+
+def rfft32(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    X = np.empty(2 * (len(x) // 2 + 1), dtype=np.float32)
+    if lib().so_rfft32(_p(x), len(x), _p(X)) != 0:
+        raise ValueError("unsupported length")
+    return (X[0::2] + 1j * X[1::2]).astype(np.complex64)
+
+
+def cfft64(z):
+    z = np.ascontiguousarray(z, dtype=np.complex128).copy()
+    if lib().so_cfft64(_p(z), len(z)) != 0:
+        raise ValueError("unsupported length")
+    return z
+
+
+def count_spectra(s, nbytes):
+    n = C.c_int(0)
+    sp = lib().so_count_spectra(C.byref(s), nbytes, C.byref(n))
+    return sp, n.value
+
+
+def domaths(s, raws, precision=64):
+    """doMaths on one buffer per source; returns (nspectra, auto, xpol, pcal)."""
+    nsrc = s.num_sources
+    raws = [np.ascontiguousarray(r, dtype=np.uint8) for r in raws]
+    nbytes = min(len(r) for r in raws)
+    nsp, _ = count_spectra(s, nbytes)
+    nsp_alloc = max(nsp, 1)
+    ssb = s.fft_ssb_points
+    ft = np.float32 if precision == 32 else np.float64
+    auto = [np.zeros(nsp_alloc * ssb, dtype=ft) for _ in range(nsrc)]
+    xpol = [np.zeros(2 * nsp_alloc * ssb, dtype=ft) for _ in range(s.num_xpols)]
+    T = max(s.pcal_tonebins, 1)
+    pcal = [np.zeros(2 * nsp_alloc * T, dtype=ft) for _ in range(nsrc)]
+    P = C.c_void_p
+    srcs = (P * 2)(*[_p(r) for r in raws] + [None] * (2 - nsrc))
+    lens = (C.c_size_t * 2)(*[len(r) for r in raws] + [0] * (2 - nsrc))
+    oa = (P * 2)(*[_p(a) for a in auto] + [None] * (2 - nsrc))
+    ox = (P * 1)(*([_p(xpol[0])] if xpol else [None]))
+    op = (P * 2)(*[_p(a) for a in pcal] + [None] * (2 - nsrc))
+    n = lib().so_domaths(C.byref(s), precision, srcs, lens, oa, ox, op)
+    auto = [a.reshape(nsp_alloc, ssb)[:n] for a in auto]
+    xpol = [(x[0::2] + 1j * x[1::2]).reshape(nsp_alloc, ssb)[:n] for x in xpol]
+    pcal = [(p[0::2] + 1j * p[1::2]).reshape(nsp_alloc, T)[:n] for p in pcal]
+    return n, auto, xpol, pcal
+
+
+def pcal_fold64(x, fs_hz, offset_hz, nbins, rotatorlen, sampleoffset=0, rot32=False, acc=None):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    if acc is None:
+        acc = np.zeros(2 * nbins, dtype=np.float64)
+    lib().so_pcal_fold64(_p(x), len(x), fs_hz, offset_hz, nbins, rotatorlen, sampleoffset,
+                         1 if rot32 else 0, _p(acc))
+    return acc
+
+
+def pcal_tones64(folded, ntones, spacing_hz, fs_hz):
+    f = np.ascontiguousarray(folded)
+    if np.iscomplexobj(f):
+        f = np.ascontiguousarray(np.stack([f.real, f.imag], axis=-1).reshape(-1), dtype=np.float64)
+    nbins = len(f) // 2
+    out = np.zeros(2 * ntones, dtype=np.float64)
+    lib().so_pcal_tones64(_p(f), nbins, ntones, spacing_hz, fs_hz, _p(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def pcal_ntones(bw, spacing, offset):
+    return lib().so_pcal_ntones(bw, spacing, offset)
+
+
+def pcal_reference64(x, bw, spacing, offset, sampleoffset=0):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nt = pcal_ntones(bw, spacing, offset)
+    out = np.zeros(2 * nt, dtype=np.float64)
+    lib().so_pcal_reference64(_p(x), len(x), bw, spacing, offset, sampleoffset, _p(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def mk5b_strip(frames):
+    frames = np.ascontiguousarray(frames, dtype=np.uint8)
+    out = np.zeros(len(frames), dtype=np.uint8)
+    n = lib().so_mk5b_strip(_p(frames), len(frames), _p(out))
+    if n < 0:
+        raise ValueError("Mk5B sync word missing")
+    return out[:n]

@@ -1,0 +1,813 @@
+// core.cpp -- TaskCoreHIP behind the C ABI of include/swspec_hip.h.
+//
+// One core = one HIP stream + one FFT plan + resident accumulators.  run()
+// walks a buffer with the reference's doMaths loop arithmetic
+// (src/IA-32/TaskCoreIPP.cpp:394-524): FFT k of a spectrum starts
+// raw_overlap_bytes after FFT k-1, a spectrum of K = core_overlapped_ffts
+// FFTs is followed by a hop of (raw_fullfft_bytes - raw_overlap_bytes), a
+// trailing incomplete spectrum is emitted unscaled.  The FFTs of a spectrum
+// are processed in chunks of G; each chunk is pass 1 (+ phase-cal) + pass 2
+// over all jobs of the core.
+#include "../../include/swspec_hip.h"
+#include "engine.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace swspec;
+
+namespace {
+
+thread_local std::string g_lib_error;
+
+struct SpecWork {
+    long long start;      // byte offset of FFT 0 of the spectrum
+    int nfft;
+    bool complete;
+    std::vector<unsigned char> pcal;  // [fft][job] gate (TaskCoreIPP.cpp:414-418)
+};
+
+}  // namespace
+
+struct swspec_hip_core {
+    swspec_hip_cfg cfg{};
+    int rank = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Plan plan;
+    PcalGeom pg;
+    float2* d_rot = nullptr;
+    int nsrc = 1, nx = 0, njobs = 1;
+    int job_src[kMaxJobs] = {0};
+    Unpack up[kMaxJobs];
+    int G = 1;
+    float2* d_Y = nullptr;
+    float* d_acc = nullptr;       // njobs * 2*(M+1)
+    float* d_xacc = nullptr;      // 4*(M+1)
+    double* d_pcal_part = nullptr;
+    double* d_pcal_acc = nullptr;
+    int ntones = 0, tone_step = 0;
+    // host-mode staging
+    uint8_t* d_in[2] = {nullptr, nullptr};
+    size_t d_in_cap = 0;
+    float* d_out = nullptr;       // staging outputs (auto, xpol, pcal)
+    size_t d_out_cap = 0;
+    double* d_tones = nullptr;
+    size_t d_tones_cap = 0;
+    std::vector<double> h_tones;
+    // last run
+    float* h_auto[kMaxJobs] = {nullptr};
+    float* h_xpol = nullptr;
+    float* h_pcal[kMaxJobs] = {nullptr};
+    float* o_auto[kMaxJobs] = {nullptr};   // where the last run wrote (device)
+    float* o_xpol = nullptr;
+    float* o_pcal[kMaxJobs] = {nullptr};
+    bool host_mode = false;
+    bool pending = false;
+    int last_nspectra = 0;
+    double last_seconds = 0.0;
+    std::string err;
+    // per-kernel profiling (HIP events around launches on `stream`)
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<int> ev_kind;        // kernel id of event pair i
+    double k_ms[4] = {0, 0, 0, 0};
+    long long k_n[4] = {0, 0, 0, 0};
+};
+
+// Records an event pair around the launches issued inside its scope.
+struct ProfScope {
+    swspec_hip_core* c;
+    int idx = -1;
+    ProfScope(swspec_hip_core* core, int kind) : c(core)
+    {
+        if (!c->profiling) return;
+        const size_t need = 2 * (c->ev_used + 1);
+        while (c->ev_pool.size() < need) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            c->ev_pool.push_back(e);
+        }
+        idx = (int)c->ev_used++;
+        c->ev_kind.resize(c->ev_used);
+        c->ev_kind[idx] = kind;
+        (void)hipEventRecord(c->ev_pool[2 * idx], c->stream);
+    }
+    ~ProfScope()
+    {
+        if (idx >= 0) (void)hipEventRecord(c->ev_pool[2 * idx + 1], c->stream);
+    }
+};
+
+static void prof_collect(swspec_hip_core* c)
+{
+    for (size_t i = 0; i < c->ev_used; i++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev_pool[2 * i], c->ev_pool[2 * i + 1]) == hipSuccess) {
+            c->k_ms[c->ev_kind[i]] += ms;
+            c->k_n[c->ev_kind[i]] += 1;
+        }
+    }
+    c->ev_used = 0;
+}
+
+static int fail(swspec_hip_core* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg; else g_lib_error = msg;
+    return code;
+}
+
+static int hipfail(swspec_hip_core* c, hipError_t e, const char* what)
+{
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(c, e == hipErrorOutOfMemory ? SWSPEC_ENOMEM : SWSPEC_EDEVICE, m);
+}
+
+#define HIPCHK(c, call)                                              \
+    do {                                                             \
+        hipError_t _e = (call);                                      \
+        if (_e != hipSuccess) return hipfail((c), _e, #call);        \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// settings derivation (src/swspectrometer.cpp:67-231)
+// ---------------------------------------------------------------------------
+
+static long long gcd_ll(long long a, long long b)   // src/Helpers.cpp:150-162
+{
+    for (;;) {
+        a = a % b;
+        if (a == 0) return b;
+        b = b % a;
+        if (b == 0) return a;
+    }
+}
+
+extern "C" int swspec_hip_derive(const swspec_hip_ini* ini, swspec_hip_derived* d)
+{
+    if (!ini || !d) return fail(nullptr, SWSPEC_EINVAL, "null argument");
+    std::memset(d, 0, sizeof(*d));
+    const int cores = ini->num_cores > 0 ? ini->num_cores : 1;
+    size_t maxraw;
+    if (ini->max_source_buffer_mb >= 0) {
+        maxraw = (size_t)ini->max_source_buffer_mb;
+        maxraw *= 1024 * 1024 / 2;    // MByte, double-buffered (:104-106)
+    } else {
+        maxraw = (size_t)ini->platform_max_raw_mb * 1024 * 1024;
+    }
+    if (ini->use_channel_file1 < 1 || ini->use_channel_file1 > ini->source_channels ||
+        ini->use_channel_file2 < 1 || ini->use_channel_file2 > ini->source_channels)
+        return fail(nullptr, SWSPEC_EINVAL, "invalid channel number");
+    if (ini->fft_points <= 0 || ini->fft_overlap_factor <= 0)
+        return fail(nullptr, SWSPEC_EINVAL, "invalid FFT points or overlap");
+    d->use_channel_file1 = ini->use_channel_file1 - 1;
+    d->use_channel_file2 = ini->use_channel_file2 - 1;
+    d->num_xpols = (ini->do_xpol && ini->num_sources == 2) ? 1 : 0;
+    const float fs = (float)((double)ini->bandwidth_hz * 2.0);
+    d->samplingfreq = fs;
+    const size_t N = (size_t)ini->fft_points;
+    {
+        const float prod = fs * ini->fft_integ_seconds;   // float arithmetic (:168)
+        d->averaged_ffts = (int)(prod / (float)N);
+    }
+    d->fft_overlap_points = (int)(N / (size_t)ini->fft_overlap_factor);
+    d->fft_ssb_points = (int)(N / 2 + 1);
+    const double rbpcs = (ini->bits_per_sample * ini->source_channels) / 8.0;
+    d->raw_fullfft_bytes = (int)((double)N * rbpcs);
+    d->raw_overlap_bytes = (int)((double)d->fft_overlap_points * rbpcs);
+    d->extract_pcal = ini->extract_pcal;
+    if (d->extract_pcal) {
+        const float off = ini->pcal_offset_hz, harm = ini->pcal_harmonics_hz;
+        if ((long long)off == 0) return fail(nullptr, SWSPEC_EINVAL, "PCalOffsetHz 0 with ExtractPCal");
+        d->pcal_tonebins = (int)(fs / harm);
+        d->pcal_rotatorlen = (int)(fs / (float)gcd_ll((long long)fs, (long long)off));
+        d->pcal_pulses = d->pcal_tonebins ? d->pcal_rotatorlen / d->pcal_tonebins : 0;
+        d->pcal_pulses_per_fft = (int)(N / (size_t)d->pcal_rotatorlen);
+        if ((N % (size_t)d->pcal_rotatorlen) != 0) d->extract_pcal = 0;   // :185-189
+    }
+    maxraw /= (size_t)cores;
+    size_t tent = (size_t)d->raw_fullfft_bytes * (size_t)d->averaged_ffts;
+    if (tent > maxraw) {
+        d->max_spectra_per_buffer = 0;
+        d->max_specffts_per_buffer = 1;
+        d->max_buffers_per_spectrum = d->averaged_ffts;
+        tent = (size_t)d->raw_fullfft_bytes;
+        while (tent <= maxraw && (d->max_buffers_per_spectrum % 2) == 0) {
+            tent *= 2;
+            d->max_buffers_per_spectrum /= 2;
+            d->max_specffts_per_buffer *= 2;
+        }
+    } else {
+        d->max_spectra_per_buffer = 1;
+        d->max_specffts_per_buffer = d->averaged_ffts;
+        d->max_buffers_per_spectrum = 0;
+        while (2 * tent <= maxraw) {
+            tent *= 2;
+            d->max_spectra_per_buffer *= 2;
+        }
+    }
+    d->rawbuf_size = (int64_t)tent;
+    d->max_rawbuf_size = (int64_t)maxraw;
+    d->core_averaged_ffts = d->max_specffts_per_buffer;
+    d->core_overlapped_ffts =
+        ini->fft_overlap_factor * d->max_specffts_per_buffer - (ini->fft_overlap_factor - 1);
+    return SWSPEC_OK;
+}
+
+extern "C" int swspec_hip_cfg_from_ini(const swspec_hip_ini* ini, const swspec_hip_derived* d,
+                                       swspec_hip_cfg* c)
+{
+    if (!ini || !d || !c) return fail(nullptr, SWSPEC_EINVAL, "null argument");
+    std::memset(c, 0, sizeof(*c));
+    c->struct_size = sizeof(*c);
+    c->abi_version = SWSPEC_HIP_ABI_VERSION;
+    c->fft_points = ini->fft_points;
+    c->fft_overlap_factor = ini->fft_overlap_factor;
+    c->fft_ssb_points = d->fft_ssb_points;
+    c->window = ini->window;
+    c->bits_per_sample = ini->bits_per_sample;
+    c->source_channels = ini->source_channels;
+    c->channelorder_increasing = ini->channelorder_increasing;
+    c->source_format = ini->source_format;
+    c->use_channel_file1 = d->use_channel_file1;
+    c->use_channel_file2 = d->use_channel_file2;
+    c->num_sources = ini->num_sources;
+    c->num_xpols = d->num_xpols;
+    c->raw_fullfft_bytes = d->raw_fullfft_bytes;
+    c->raw_overlap_bytes = d->raw_overlap_bytes;
+    c->core_overlapped_ffts = d->core_overlapped_ffts;
+    c->max_spectra_per_buffer = d->max_spectra_per_buffer;
+    c->extract_pcal = d->extract_pcal;
+    c->pcal_tonebins = d->pcal_tonebins;
+    c->pcal_rotatorlen = d->pcal_rotatorlen;
+    c->samplingfreq = d->samplingfreq;
+    c->pcaloffsethz = ini->pcal_offset_hz;
+    c->pcalharmonicshz = ini->pcal_harmonics_hz;
+    c->max_rawbuf_bytes = d->rawbuf_size;
+    return SWSPEC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// decoder selection (src/IA-32/DataUnpackerFactory.h:38-108)
+// ---------------------------------------------------------------------------
+
+static int pick_unpack(const swspec_hip_cfg* c, int channel, Unpack& up, std::string& why)
+{
+    up = Unpack();
+    if (c->bits_per_sample != 2) { why = "only 2-bit samples are implemented"; return -1; }
+    if (channel < 0 || channel >= c->source_channels) { why = "channel out of range"; return -1; }
+    const bool raw = c->source_format == SWSPEC_FMT_RAWSIGNED || c->source_format == SWSPEC_FMT_RAWUNSIGNED;
+    if (raw && (c->source_channels % 4) == 0) {
+        // TwoBitUnpacker (DataUnpackers.cpp:191-244)
+        const int step = c->source_channels / 4;
+        up.kind = kUnpackMultiChannel;
+        up.step = step;
+        if (c->channelorder_increasing) {
+            up.cbyte = channel / 4;
+            up.shift = 2 * (channel % 4);
+        } else {
+            up.cbyte = (step - 1) - channel / 4;
+            up.shift = 6 - 2 * (channel % 4);
+        }
+        return 0;
+    }
+    if (raw && c->source_channels == 1) {
+        up.kind = kUnpackSingleChannel;     // TwoBitSinglechannelUnpacker (:261-305)
+        return 0;
+    }
+    if (c->source_format == SWSPEC_FMT_MK5B && (c->source_channels == 4 || c->source_channels == 8)) {
+        // Mk5BUnpacker 4/8-channel branches (:358-407): increasing order always
+        up.kind = kUnpackMultiChannel;
+        up.step = c->source_channels / 4;
+        up.cbyte = channel / 4;
+        up.shift = 2 * (channel % 4);
+        return 0;
+    }
+    why = "decoder not implemented for this format/channel count";
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
+// create / destroy
+// ---------------------------------------------------------------------------
+
+static int core_alloc(swspec_hip_core* c)
+{
+    const Plan& p = c->plan;
+    const size_t M1 = (size_t)p.M + 1;
+    HIPCHK(c, hipMalloc(&c->d_Y, (size_t)c->njobs * c->G * (size_t)p.M * sizeof(float2)));
+    HIPCHK(c, hipMalloc(&c->d_acc, (size_t)c->njobs * 2 * M1 * sizeof(float)));
+    HIPCHK(c, (hipError_t)launch_zero(c->d_acc, (size_t)c->njobs * 2 * M1 * sizeof(float), c->stream));
+    if (c->nx) {
+        HIPCHK(c, hipMalloc(&c->d_xacc, 4 * M1 * sizeof(float)));
+        HIPCHK(c, (hipError_t)launch_zero(c->d_xacc, 4 * M1 * sizeof(float), c->stream));
+    }
+    if (c->cfg.extract_pcal) {
+        const size_t part = (size_t)c->njobs * c->G * c->pg.nblk * c->pg.T * 2;
+        HIPCHK(c, hipMalloc(&c->d_pcal_part, part * sizeof(double)));
+        HIPCHK(c, hipMalloc(&c->d_pcal_acc, (size_t)c->njobs * c->pg.T * 2 * sizeof(double)));
+        HIPCHK(c, (hipError_t)launch_zero(c->d_pcal_acc, (size_t)c->njobs * c->pg.T * 2 * sizeof(double),
+                                          c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SWSPEC_OK;
+}
+
+static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, const int* job_src,
+                      const int* job_ch)
+{
+    if (cfg->struct_size != sizeof(swspec_hip_cfg) || cfg->abi_version != SWSPEC_HIP_ABI_VERSION)
+        return fail(c, SWSPEC_EINVAL, "swspec_hip_cfg size/version mismatch");
+    c->cfg = *cfg;
+    if (cfg->fft_ssb_points != cfg->fft_points / 2 + 1) return fail(c, SWSPEC_EINVAL, "fft_ssb_points mismatch");
+    if (cfg->raw_overlap_bytes <= 0 || cfg->raw_fullfft_bytes < cfg->raw_overlap_bytes)
+        return fail(c, SWSPEC_EINVAL, "raw byte counts invalid");
+    if (cfg->core_overlapped_ffts < 1) return fail(c, SWSPEC_EINVAL, "core_overlapped_ffts < 1");
+    if (cfg->window < 0 || cfg->window > 5) return fail(c, SWSPEC_EINVAL, "unknown window");
+    c->njobs = njobs;
+    c->nsrc = cfg->num_sources;
+    c->nx = cfg->num_xpols;
+    for (int j = 0; j < njobs; j++) {
+        std::string why;
+        c->job_src[j] = job_src[j];
+        if (pick_unpack(cfg, job_ch[j], c->up[j], why) != 0) return fail(c, SWSPEC_EUNSUPPORTED, why);
+    }
+    // bytes consumed per FFT must match the decoder layout
+    {
+        const long long per = (c->up[0].kind == kUnpackSingleChannel) ? cfg->fft_points / 4
+                                                                       : cfg->fft_points * c->up[0].step;
+        if (per != cfg->raw_fullfft_bytes) return fail(c, SWSPEC_EINVAL, "raw_fullfft_bytes does not match format");
+        if (c->up[0].kind == kUnpackSingleChannel && (cfg->raw_overlap_bytes * 4) % 2 != 0)
+            return fail(c, SWSPEC_EINVAL, "overlap not byte aligned");
+    }
+    if (hipSetDevice(cfg->device) != hipSuccess) return fail(c, SWSPEC_EDEVICE, "hipSetDevice failed");
+    HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreate(&c->ev0));
+    HIPCHK(c, hipEventCreate(&c->ev1));
+    std::string e = plan_create(c->plan, cfg->fft_points, cfg->window, c->stream);
+    if (!e.empty()) return fail(c, SWSPEC_EUNSUPPORTED, e);
+
+    const long long M = c->plan.M;
+    if (cfg->ffts_per_chunk > 0) {
+        c->G = (int)std::min<uint32_t>(cfg->ffts_per_chunk, 64);
+    } else {
+        // keep the chunk's intermediate near 256 MB (MALL-sized), 1..32 FFTs
+        const double per = (double)njobs * M * sizeof(float2);
+        c->G = (int)std::max(1.0, std::min(32.0, std::floor(256.0e6 / per)));
+    }
+    c->G = std::max(1, std::min(c->G, cfg->core_overlapped_ffts));
+
+    if (cfg->extract_pcal) {
+        const int T = cfg->pcal_tonebins, L = cfg->pcal_rotatorlen;
+        if (T < 1 || L < T || (L % T) != 0 || (cfg->fft_points % L) != 0 || T > 1024)
+            return fail(c, SWSPEC_EINVAL, "phase-cal geometry invalid");
+        c->pg.T = T;
+        c->pg.L = L;
+        int th = (256 / T) * T;
+        if (th < T) th = T;
+        while (th > L) th -= T;
+        c->pg.threads = th;
+        // ~32 samples per thread per block, a multiple of T
+        long long chunk = (long long)th * 32;
+        chunk = (chunk / T) * T;
+        c->pg.chunk = (int)chunk;
+        c->pg.nblk = (int)((cfg->fft_points + chunk - 1) / chunk);
+        std::vector<float2> rot(L);
+        const double dphi = 2 * M_PI * (-cfg->pcaloffsethz / cfg->samplingfreq);   // :130-135
+        for (int i = 0; i < L; i++) {
+            const double arg = dphi * (double)i;
+            rot[i] = make_float2((float)cos(arg), (float)sin(arg));
+        }
+        HIPCHK(c, hipMalloc(&c->d_rot, L * sizeof(float2)));
+        HIPCHK(c, hipMemcpy(c->d_rot, rot.data(), L * sizeof(float2), hipMemcpyHostToDevice));
+        c->pg.d_rot = c->d_rot;
+        const double bw = cfg->samplingfreq / 2;
+        c->ntones = (int)std::floor((bw - cfg->pcaloffsethz) / cfg->pcalharmonicshz) + 1;   // PCal.cpp:328
+        c->ntones = std::max(0, std::min(c->ntones, T));
+        c->tone_step = (int)std::floor(T * (cfg->pcalharmonicshz / cfg->samplingfreq));     // PCal.cpp:506
+    }
+    return core_alloc(c);
+}
+
+extern "C" int swspec_hip_create(const swspec_hip_cfg* cfg, int rank, swspec_hip_core** out)
+{
+    if (!cfg || !out) return fail(nullptr, SWSPEC_EINVAL, "null argument");
+    *out = nullptr;
+    if (cfg->num_sources < 1 || cfg->num_sources > 2) return fail(nullptr, SWSPEC_EINVAL, "num_sources must be 1 or 2");
+    if (cfg->num_xpols && cfg->num_sources != 2) return fail(nullptr, SWSPEC_EINVAL, "cross-power needs 2 sources");
+    swspec_hip_core* c = new swspec_hip_core();
+    c->rank = rank;
+    int src[2] = {0, 1};
+    int ch[2] = {cfg->use_channel_file1, cfg->use_channel_file2};
+    int rc = core_setup(c, cfg, cfg->num_sources, src, ch);
+    if (rc != SWSPEC_OK) {
+        g_lib_error = c->err;
+        swspec_hip_finalize(c);
+        return rc;
+    }
+    *out = c;
+    return SWSPEC_OK;
+}
+
+// Several channels of source 0 in one pass (no TaskCore counterpart: the
+// reference runs one core per UseFile1Channel value over the same file).
+extern "C" int swspec_hip_create_multichannel(const swspec_hip_cfg* cfg, int nch, const int* channels, int rank,
+                                              swspec_hip_core** out)
+{
+    if (!cfg || !out || !channels) return fail(nullptr, SWSPEC_EINVAL, "null argument");
+    *out = nullptr;
+    if (nch < 1 || nch > kMaxJobs) return fail(nullptr, SWSPEC_EINVAL, "1..8 channels");
+    if (cfg->num_xpols) return fail(nullptr, SWSPEC_EINVAL, "no cross-power in multichannel mode");
+    swspec_hip_cfg c1 = *cfg;
+    c1.num_sources = 1;
+    swspec_hip_core* c = new swspec_hip_core();
+    c->rank = rank;
+    int src[kMaxJobs] = {0};
+    int rc = core_setup(c, &c1, nch, src, channels);
+    if (rc != SWSPEC_OK) {
+        g_lib_error = c->err;
+        swspec_hip_finalize(c);
+        return rc;
+    }
+    *out = c;
+    return SWSPEC_OK;
+}
+
+extern "C" int swspec_hip_finalize(swspec_hip_core* c)
+{
+    if (!c) return fail(nullptr, SWSPEC_EINVAL, "null core");
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    plan_destroy(c->plan);
+    for (void* p : {(void*)c->d_rot, (void*)c->d_Y, (void*)c->d_acc, (void*)c->d_xacc, (void*)c->d_pcal_part,
+                    (void*)c->d_pcal_acc, (void*)c->d_in[0], (void*)c->d_in[1], (void*)c->d_out, (void*)c->d_tones})
+        if (p) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SWSPEC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// run
+// ---------------------------------------------------------------------------
+
+// The doMaths loop arithmetic (TaskCoreIPP.cpp:394-524).
+static std::vector<SpecWork> schedule(const swspec_hip_core* c, size_t len)
+{
+    const swspec_hip_cfg& g = c->cfg;
+    std::vector<SpecWork> out;
+    size_t remain = len;
+    long long off = 0;
+    int acc = 0, curr = 0;
+    SpecWork cur{0, 0, false, {}};
+    const size_t full = (size_t)g.raw_fullfft_bytes, ovl = (size_t)g.raw_overlap_bytes;
+    while (remain >= full) {
+        if (cur.nfft == 0) cur.start = off;
+        for (int j = 0; j < c->njobs; j++) {
+            // curr_ffts counts once per source FFT (:415, :435); multichannel
+            // jobs of one source share the source's count
+            const int srcidx = c->job_src[j];
+            const int cnt = curr + srcidx;
+            cur.pcal.push_back(g.extract_pcal && (cnt % g.fft_overlap_factor) == 0);
+        }
+        curr += c->nsrc;
+        cur.nfft++;
+        off += (long long)ovl;
+        remain -= ovl;
+        if (++acc >= g.core_overlapped_ffts) {
+            cur.complete = true;
+            out.push_back(cur);
+            cur = SpecWork{0, 0, false, {}};
+            acc = 0;
+            off += (long long)(full - ovl);
+            remain -= (full - ovl);
+        }
+    }
+    if (acc > 0) out.push_back(cur);   // partial, unscaled (:519-524)
+    return out;
+}
+
+static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len, float* const* d_auto,
+                      float* d_xpol, float* const* d_pcal, double* d_tones)
+{
+    const Plan& p = c->plan;
+    const swspec_hip_cfg& g = c->cfg;
+    std::vector<SpecWork> sched = schedule(c, len);
+    const size_t M1 = (size_t)p.M + 1;
+    const int ssb = g.fft_ssb_points;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (size_t sp = 0; sp < sched.size(); sp++) {
+        const SpecWork& w = sched[sp];
+        for (int f0 = 0; f0 < w.nfft; f0 += c->G) {
+            const int nf = std::min(c->G, w.nfft - f0);
+            JobDev jobs[kMaxJobs];
+            bool any_pcal = false;
+            for (int j = 0; j < c->njobs; j++) {
+                JobDev& J = jobs[j];
+                J.raw = d_in[c->job_src[j]] + w.start + (long long)f0 * g.raw_overlap_bytes;
+                J.fft_stride = g.raw_overlap_bytes;
+                J.up = c->up[j];
+                J.Y = c->d_Y + (size_t)j * c->G * (size_t)p.M;
+                J.acc = c->d_acc + (size_t)j * 2 * M1;
+                if (g.extract_pcal) {
+                    J.pcal_part = c->d_pcal_part + (size_t)j * c->G * c->pg.nblk * c->pg.T * 2;
+                    J.pcal_acc = c->d_pcal_acc + (size_t)j * c->pg.T * 2;
+                    unsigned long long m = 0;
+                    for (int f = 0; f < nf; f++)
+                        if (w.pcal[(size_t)(f0 + f) * c->njobs + j]) m |= 1ull << f;
+                    J.pcal_mask = m;
+                    any_pcal |= (m != 0);
+                }
+            }
+            {
+                ProfScope ps(c, SWSPEC_K_PASS1);
+                HIPCHK(c, launch_pass1(p, jobs, c->njobs, nf, c->stream));
+            }
+            if (any_pcal) {
+                ProfScope ps(c, SWSPEC_K_PCAL);
+                HIPCHK(c, launch_pcal(c->pg, g.fft_points, jobs, c->njobs, nf, c->stream));
+            }
+            XpolDev xp;
+            xp.acc = c->d_xacc;
+            {
+                ProfScope ps(c, SWSPEC_K_PASS2);
+                HIPCHK(c, launch_pass2(p, jobs, c->njobs, c->nx ? &xp : nullptr, nf,
+                                       (g.flags & SWSPEC_FLAG_NO_KAHAN) ? 0 : 1, c->stream));
+            }
+        }
+        ProfScope psf(c, SWSPEC_K_FINALIZE);
+        // TaskCoreIPP.cpp:491-498 scale 1/K for complete spectra; partial unscaled
+        const double scale = w.complete ? 1.0 / (double)g.core_overlapped_ffts : 1.0;
+        for (int j = 0; j < c->njobs; j++) {
+            float* acc = c->d_acc + (size_t)j * 2 * M1;
+            HIPCHK(c, launch_finalize(p, acc, d_auto[j] + sp * (size_t)ssb, scale, c->stream));
+            HIPCHK(c, launch_zero(acc, 2 * M1 * sizeof(float), c->stream));
+        }
+        if (c->nx) {
+            HIPCHK(c, launch_finalize_xpol(p, c->d_xacc, d_xpol + sp * 2 * (size_t)ssb, scale, c->stream));
+            HIPCHK(c, launch_zero(c->d_xacc, 4 * M1 * sizeof(float), c->stream));
+        }
+        if (g.extract_pcal) {
+            for (int j = 0; j < c->njobs; j++) {
+                float* po = d_pcal && d_pcal[j] ? d_pcal[j] + sp * 2 * (size_t)c->pg.T : nullptr;
+                double* to = d_tones ? d_tones + ((size_t)j * sched.size() + sp) * 2 * c->ntones : nullptr;
+                HIPCHK(c, launch_pcal_emit(c->pg, c->d_pcal_acc + (size_t)j * c->pg.T * 2, po, to, c->ntones,
+                                           c->tone_step, c->stream));
+            }
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->last_nspectra = (int)sched.size();
+    return SWSPEC_OK;
+}
+
+static int ensure_tones(swspec_hip_core* c, size_t nspectra)
+{
+    if (!c->cfg.extract_pcal || c->ntones == 0) return SWSPEC_OK;
+    const size_t need = (size_t)c->njobs * std::max<size_t>(nspectra, 1) * 2 * c->ntones;
+    if (need > c->d_tones_cap) {
+        if (c->d_tones) HIPCHK(c, hipFree(c->d_tones));
+        c->d_tones = nullptr;
+        HIPCHK(c, hipMalloc(&c->d_tones, need * sizeof(double)));
+        c->d_tones_cap = need;
+    }
+    return SWSPEC_OK;
+}
+
+static size_t min_len(const swspec_hip_core* c, const size_t* in_len)
+{
+    size_t len = in_len[0];
+    for (int s = 1; s < c->nsrc; s++) len = std::min(len, in_len[s]);
+    return len;
+}
+
+extern "C" int swspec_hip_run_device(swspec_hip_core* c, const void* const* d_in, const size_t* in_len,
+                                     float* const* d_out_auto, float* const* d_out_xpol, float* const* d_out_pcal)
+{
+    if (!c || !d_in || !in_len || !d_out_auto) return fail(c, SWSPEC_EINVAL, "null argument");
+    if (c->pending) return fail(c, SWSPEC_ESTATE, "run while a run is pending");
+    if (c->nx && (!d_out_xpol || !d_out_xpol[0])) return fail(c, SWSPEC_EINVAL, "cross-power output missing");
+    const size_t len = min_len(c, in_len);
+    int nfft = 0;
+    size_t nsp = schedule(c, len).size();
+    (void)nfft;
+    int rc = ensure_tones(c, nsp);
+    if (rc) return rc;
+    const uint8_t* ins[2] = {(const uint8_t*)d_in[0], c->nsrc > 1 ? (const uint8_t*)d_in[1] : nullptr};
+    float* autos[kMaxJobs];
+    float* pcals[kMaxJobs];
+    for (int j = 0; j < c->njobs; j++) {
+        autos[j] = d_out_auto[j];
+        pcals[j] = d_out_pcal ? d_out_pcal[j] : nullptr;
+    }
+    c->host_mode = false;
+    rc = run_common(c, ins, len, autos, c->nx ? d_out_xpol[0] : nullptr, pcals, c->d_tones);
+    if (rc) return rc;
+    c->pending = true;
+    return SWSPEC_OK;
+}
+
+extern "C" int swspec_hip_run(swspec_hip_core* c, const void* const* in, const size_t* in_len,
+                              float* const* out_auto, float* const* out_xpol, float* const* out_pcal)
+{
+    if (!c || !in || !in_len || !out_auto) return fail(c, SWSPEC_EINVAL, "null argument");
+    if (c->pending) return fail(c, SWSPEC_ESTATE, "run while a run is pending");
+    if (c->nx && (!out_xpol || !out_xpol[0])) return fail(c, SWSPEC_EINVAL, "cross-power output missing");
+    const size_t len = min_len(c, in_len);
+    const size_t nsp = schedule(c, len).size();
+    const size_t nalloc = std::max<size_t>(nsp, 1);
+    const size_t ssb = (size_t)c->cfg.fft_ssb_points;
+    const size_t T = c->cfg.extract_pcal ? (size_t)c->pg.T : 0;
+    // staging
+    size_t inmax = 0;
+    for (int s = 0; s < c->nsrc; s++) inmax = std::max(inmax, in_len[s]);
+    if (inmax > c->d_in_cap) {
+        for (int s = 0; s < 2; s++) if (c->d_in[s]) { HIPCHK(c, hipFree(c->d_in[s])); c->d_in[s] = nullptr; }
+        for (int s = 0; s < c->nsrc; s++) HIPCHK(c, hipMalloc(&c->d_in[s], inmax));
+        c->d_in_cap = inmax;
+    }
+    const size_t per_auto = nalloc * ssb, per_x = 2 * nalloc * ssb, per_p = 2 * nalloc * T;
+    const size_t need = c->njobs * per_auto + (c->nx ? per_x : 0) + c->njobs * per_p;
+    if (need > c->d_out_cap) {
+        if (c->d_out) HIPCHK(c, hipFree(c->d_out));
+        c->d_out = nullptr;
+        HIPCHK(c, hipMalloc(&c->d_out, need * sizeof(float)));
+        c->d_out_cap = need;
+    }
+    int rc = ensure_tones(c, nsp);
+    if (rc) return rc;
+    for (int s = 0; s < c->nsrc; s++)
+        HIPCHK(c, hipMemcpyAsync(c->d_in[s], in[s], in_len[s], hipMemcpyHostToDevice, c->stream));
+    float* autos[kMaxJobs];
+    float* pcals[kMaxJobs];
+    float* base = c->d_out;
+    for (int j = 0; j < c->njobs; j++) { autos[j] = base; base += per_auto; }
+    float* xp = nullptr;
+    if (c->nx) { xp = base; base += per_x; }
+    for (int j = 0; j < c->njobs; j++) { pcals[j] = T ? base : nullptr; base += per_p; }
+    const uint8_t* ins[2] = {c->d_in[0], c->d_in[1]};
+    rc = run_common(c, ins, len, autos, xp, pcals, c->d_tones);
+    if (rc) return rc;
+    // results back to the caller's host arrays
+    for (int j = 0; j < c->njobs; j++) {
+        if (nsp) HIPCHK(c, hipMemcpyAsync(out_auto[j], autos[j], nsp * ssb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        c->h_auto[j] = out_auto[j];
+        c->o_auto[j] = autos[j];
+        if (T && out_pcal && out_pcal[j]) {
+            if (nsp) HIPCHK(c, hipMemcpyAsync(out_pcal[j], pcals[j], nsp * 2 * T * sizeof(float),
+                                              hipMemcpyDeviceToHost, c->stream));
+            c->h_pcal[j] = out_pcal[j];
+        } else {
+            c->h_pcal[j] = nullptr;
+        }
+    }
+    if (c->nx) {
+        if (nsp) HIPCHK(c, hipMemcpyAsync(out_xpol[0], xp, nsp * 2 * ssb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        c->h_xpol = out_xpol[0];
+    }
+    c->host_mode = true;
+    c->pending = true;
+    return SWSPEC_OK;
+}
+
+extern "C" int swspec_hip_join(swspec_hip_core* c, int* nspectra, size_t* la, size_t* lx, size_t* lp)
+{
+    if (!c) return fail(nullptr, SWSPEC_EINVAL, "null core");
+    if (!c->pending) return fail(c, SWSPEC_ESTATE, "join without run");
+    c->pending = false;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->last_seconds = ms * 1e-3;
+    if (c->profiling) prof_collect(c);
+    const int n = c->last_nspectra;
+    if (c->cfg.extract_pcal && c->ntones > 0 && n > 0) {
+        c->h_tones.resize((size_t)c->njobs * n * 2 * c->ntones);
+        HIPCHK(c, hipMemcpy(c->h_tones.data(), c->d_tones, c->h_tones.size() * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        c->h_tones.clear();
+    }
+    if (nspectra) *nspectra = n;
+    if (la) *la = (size_t)n * c->cfg.fft_ssb_points * sizeof(float);
+    if (lx) *lx = c->nx ? (size_t)n * 2 * c->cfg.fft_ssb_points * sizeof(float) : 0;
+    if (lp) *lp = c->cfg.extract_pcal ? (size_t)n * c->cfg.pcal_tonebins * 2 * sizeof(float) : 0;
+    return SWSPEC_OK;
+}
+
+extern "C" int swspec_hip_combine(swspec_hip_core* c, float* const* out, float* const* pcal, size_t* out_len,
+                                  size_t* pcal_len)
+{
+    // TaskCoreIPP::combineResults (:311-349) on the host arrays of the last run
+    if (!c || !out) return fail(c, SWSPEC_EINVAL, "null argument");
+    if (!c->host_mode) return fail(c, SWSPEC_ESTATE, "combine needs a host-mode run");
+    const size_t ssb = (size_t)c->cfg.fft_ssb_points;
+    for (int s = 0; s < c->njobs; s++) {
+        if (!out[s] || !c->h_auto[s]) continue;
+        for (size_t k = 0; k < ssb; k++) out[s][k] += c->h_auto[s][k];
+        if (out_len) out_len[s] = sizeof(float) * ssb;
+        std::memset(c->h_auto[s], 0, sizeof(float) * ssb);
+    }
+    for (int x = 0; x < c->nx; x++) {
+        float* dst = out[c->njobs + x];
+        if (!dst || !c->h_xpol) continue;
+        for (size_t k = 0; k < 2 * ssb; k++) dst[k] += c->h_xpol[k];
+        if (out_len) out_len[c->njobs + x] = sizeof(float) * ssb;   // Q5: float-sized length (:333)
+        std::memset(c->h_xpol, 0, sizeof(float) * 2 * ssb);
+    }
+    if (c->cfg.extract_pcal && pcal) {
+        const size_t T = (size_t)c->cfg.pcal_tonebins;
+        for (int s = 0; s < c->njobs; s++) {
+            if (!pcal[s] || !c->h_pcal[s]) continue;
+            for (size_t k = 0; k < 2 * T; k++) pcal[s][k] += c->h_pcal[s][k];
+            if (pcal_len) pcal_len[s] = sizeof(float) * 2 * T;
+            std::memset(c->h_pcal[s], 0, sizeof(float) * 2 * T);
+        }
+    }
+    return SWSPEC_OK;
+}
+
+extern "C" void swspec_hip_reset(void* buf, size_t bytes)
+{
+    if (buf) std::memset(buf, 0, (bytes / sizeof(float)) * sizeof(float));
+}
+
+extern "C" const char* swspec_hip_last_error(const swspec_hip_core* c)
+{
+    return c ? c->err.c_str() : g_lib_error.c_str();
+}
+
+extern "C" int swspec_hip_pcal_tones(swspec_hip_core* c, int source, int spectrum, double* tones, int ntones)
+{
+    if (!c || !tones) return fail(c, SWSPEC_EINVAL, "null argument");
+    if (!c->cfg.extract_pcal) return fail(c, SWSPEC_ESTATE, "phase-cal not enabled");
+    if (source < 0 || source >= c->njobs || spectrum < 0 || spectrum >= c->last_nspectra)
+        return fail(c, SWSPEC_EINVAL, "source/spectrum out of range");
+    if (ntones != c->ntones) return fail(c, SWSPEC_EINVAL, "ntones mismatch");
+    const double* src = c->h_tones.data() + ((size_t)source * c->last_nspectra + spectrum) * 2 * c->ntones;
+    std::memcpy(tones, src, sizeof(double) * 2 * c->ntones);
+    return SWSPEC_OK;
+}
+
+extern "C" int swspec_hip_pcal_ntones(const swspec_hip_core* c) { return c ? c->ntones : -1; }
+
+extern "C" double swspec_hip_last_run_seconds(const swspec_hip_core* c) { return c ? c->last_seconds : 0.0; }
+
+extern "C" int swspec_hip_set_profiling(swspec_hip_core* c, int on)
+{
+    if (!c) return fail(nullptr, SWSPEC_EINVAL, "null core");
+    if (c->pending) return fail(c, SWSPEC_ESTATE, "set_profiling while a run is pending");
+    c->profiling = on != 0;
+    c->ev_used = 0;
+    for (int k = 0; k < 4; k++) { c->k_ms[k] = 0.0; c->k_n[k] = 0; }
+    return SWSPEC_OK;
+}
+
+extern "C" int swspec_hip_kernel_stats(const swspec_hip_core* c, int kernel, double* total_ms, int64_t* launches)
+{
+    if (!c || kernel < 0 || kernel > 3) return fail(nullptr, SWSPEC_EINVAL, "bad argument");
+    if (total_ms) *total_ms = c->k_ms[kernel];
+    if (launches) *launches = c->k_n[kernel];
+    return SWSPEC_OK;
+}
+
+extern "C" void* swspec_hip_host_alloc(size_t bytes)
+{
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+    return p;
+}
+
+extern "C" void swspec_hip_host_free(void* p)
+{
+    if (p) (void)hipHostFree(p);
+}
+
+extern "C" int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d_raw, int channel, int64_t count,
+                                        float* d_out)
+{
+    if (!cfg || !d_raw || !d_out || count < 0) return fail(nullptr, SWSPEC_EINVAL, "null argument");
+    Unpack up;
+    std::string why;
+    if (pick_unpack(cfg, channel, up, why) != 0) return fail(nullptr, SWSPEC_EUNSUPPORTED, why);
+    hipError_t e = launch_unpack((const uint8_t*)d_raw, up, count, d_out, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(nullptr, SWSPEC_EDEVICE, hipGetErrorString(e));
+    return SWSPEC_OK;
+}
+
+extern "C" int swspec_hip_plan_info(int64_t fft_points, int32_t* info, int ninfo)
+{
+    Plan p;
+    std::string e = plan_geometry(p, fft_points);
+    if (!e.empty()) return fail(nullptr, SWSPEC_EUNSUPPORTED, e);
+    int32_t v[8] = {p.N1, p.N2, p.B, p.p1_threads, p.sp2.T * 2, p.npairs, p.p1_maxpts, p.p2_maxpts};
+    for (int i = 0; i < ninfo && i < 8; i++) info[i] = v[i];
+    return SWSPEC_OK;
+}

@@ -1,0 +1,113 @@
+// engine.h -- internal C++ interface of the MI355X spectrometer engine.
+//
+// The engine processes "jobs": one job is one channel of one raw byte stream
+// (the unit that TaskCoreIPP::doMaths handles per source,
+// src/IA-32/TaskCoreIPP.cpp:401-449).  Jobs that share an FFT length share one
+// plan and run in the same kernel launches; cross-power pairs (:457-487) are
+// job pairs processed by the same row-pass workgroups.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace swspec {
+
+constexpr int kMaxStages = 16;
+constexpr int kMaxJobs = 8;       // jobs per engine launch (kernel-argument tables)
+
+// 2-bit decoder kinds (src/IA-32/DataUnpackerFactory.h:38-108)
+enum UnpackKind : int {
+    kUnpackNone = 0,
+    kUnpackMultiChannel = 1,   // TwoBitUnpacker / Mk5BUnpacker: one byte group per time sample
+    kUnpackSingleChannel = 2,  // TwoBitSinglechannelUnpacker: 4 samples per byte, oldest in MSB
+};
+
+struct Unpack {
+    int kind = kUnpackNone;
+    int step = 1;      // bytes per time sample (multi-channel)
+    int cbyte = 0;     // byte offset of the channel inside a time sample
+    int shift = 0;     // bit shift of the 2-bit field
+};
+
+// Stockham sub-FFT executed in LDS by one workgroup slice.
+struct SubPlan {
+    int L = 0;               // points
+    int nst = 0;             // stages
+    int radix[kMaxStages];
+    int ns[kMaxStages];      // product of the radices before the stage
+    int twoff[kMaxStages];   // offset (in float2) of the stage's twiddles
+    int T = 0;               // threads per FFT
+    int twlen = 0;           // total twiddles
+};
+
+// Two-pass four-step real FFT plan for N real points (M = N/2 complex).
+struct Plan {
+    long long N = 0, M = 0;
+    int N1 = 0, N2 = 0;      // M = N1 * N2; pass 1 = N1-point columns, pass 2 = N2-point rows
+    int B = 0;               // columns per pass-1 workgroup
+    int p1_threads = 0, p2_threads = 0;
+    int p1_maxpts = 0, p2_maxpts = 0;
+    int npairs = 0;          // pass-2 workgroups per FFT (row pairs)
+    int twshift = 0;         // four-step twiddle split W_M^e = A[e>>s] * B[e & (2^s-1)]
+    SubPlan sp1, sp2;
+    int window = 0;
+    // device tables
+    float2* d_stw1 = nullptr;   // stage twiddles for sub-FFT N1
+    float2* d_stw2 = nullptr;   // stage twiddles for sub-FFT N2
+    float2* d_twA = nullptr;
+    float2* d_twB = nullptr;
+    float2* d_rA = nullptr;     // W_N^{k1}, k1 < N1
+    float2* d_rB = nullptr;     // W_N^{N1 k2}, k2 < N2
+    float* d_window = nullptr;  // N window values
+};
+
+// Build a plan (host tables in FP64, rounded to FP32 once).  Returns an
+// error string (empty on success).
+std::string plan_create(Plan& p, long long N, int window, hipStream_t stream);
+void plan_destroy(Plan& p);
+// Factorisation only (no device work): used by tests and the tuner.
+std::string plan_geometry(Plan& p, long long N);
+
+// One job as the kernels see it.
+struct JobDev {
+    const uint8_t* raw = nullptr;  // device pointer to the first byte of FFT 0 of the chunk
+    long long fft_stride = 0;      // bytes between consecutive FFT starts in the chunk
+    Unpack up;
+    float2* Y = nullptr;           // pass-1 output [G][N1][N2]
+    float* acc = nullptr;          // accumulator: (sum, comp) pairs, permuted layout, M+1 bins
+    double* pcal_part = nullptr;   // pcal partial sums [G][nblk][T] double2
+    double* pcal_acc = nullptr;    // pcal accumulator [T] double2
+    unsigned long long pcal_mask = 0;  // bit f: FFT f of the chunk gets phase-cal
+};
+
+struct XpolDev {
+    float* acc = nullptr;          // (re, im, comp_re, comp_im) per bin, permuted layout
+};
+
+struct PcalGeom {
+    int T = 0;        // tone bins
+    int L = 0;        // rotator length
+    int nblk = 0;     // blocks per FFT
+    int chunk = 0;    // samples per block
+    int threads = 0;
+    const float2* d_rot = nullptr;  // FP32 rotator (FP64-computed), L entries
+};
+
+// Launchers (engine.hip).
+hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, hipStream_t s);
+hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const XpolDev* xp, int nfft,
+                        int kahan, hipStream_t s);
+hipError_t launch_finalize(const Plan& p, const float* acc, float* out, double scale, hipStream_t s);
+hipError_t launch_finalize_xpol(const Plan& p, const float* acc, float* out, double scale, hipStream_t s);
+hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s);
+hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int njobs, int nfft,
+                       hipStream_t s);
+hipError_t launch_pcal_emit(const PcalGeom& g, double* acc, float* out, double* tones, int ntones,
+                            int tone_step, hipStream_t s);
+hipError_t launch_unpack(const uint8_t* raw, Unpack up, long long count, float* out, hipStream_t s);
+
+}  // namespace swspec

@@ -1,0 +1,875 @@
+// engine.hip -- CDNA4 (gfx950) kernels of the spectrometer core.
+//
+// Hot path of TaskCoreIPP::doMaths (src/IA-32/TaskCoreIPP.cpp:394-511) as
+// three fused kernels per chunk of G consecutive FFTs:
+//   pass 1  : 2-bit unpack + window + N1-point column FFTs (LDS Stockham) +
+//             four-step twiddle; writes Y[k1][n2] (row-contiguous).
+//   pcal    : unwindowed unpack x rotator, folded into tone bins in FP64
+//             (extract_PCal, :633-693); per-block partials, fixed-order reduce.
+//   pass 2  : N2-point row FFTs for the row pair (k1, N1-k1), real-to-complex
+//             split, |X|^2 (and X*conj(Y)) summed over the chunk in registers,
+//             then one Kahan-compensated update of the resident accumulators.
+// The real N-point transform is computed as an M = N/2 complex FFT of
+// z[m] = x[2m] + i x[2m+1] followed by the split
+//   X[k] = E[k] + W_N^k O[k],  E = (Z[k] + conj Z[M-k])/2,  O = (Z[k] - conj Z[M-k])/(2i).
+// Accumulators are kept in the permuted order of pass 2 (bin k = k1 + N1 k2
+// stored at k1*N2 + k2, Nyquist at M); finalize writes natural order.
+#include "engine.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+
+namespace swspec {
+
+// ---------------------------------------------------------------------------
+// complex helpers and radix-R forward DFTs (W_R = e^{-2 pi i / R})
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <int R> struct Dft;
+
+template <> struct Dft<2> {
+    __device__ __forceinline__ static void run(float2* v)
+    {
+        float2 a = v[0], b = v[1];
+        v[0] = make_float2(a.x + b.x, a.y + b.y);
+        v[1] = make_float2(a.x - b.x, a.y - b.y);
+    }
+};
+
+template <> struct Dft<3> {
+    __device__ __forceinline__ static void run(float2* v)
+    {
+        const float s3 = 0.866025403784438647f;
+        float2 t = make_float2(v[1].x + v[2].x, v[1].y + v[2].y);
+        float2 d = make_float2((v[1].x - v[2].x) * s3, (v[1].y - v[2].y) * s3);
+        float2 a = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+        v[0] = make_float2(v[0].x + t.x, v[0].y + t.y);
+        v[1] = make_float2(a.x + d.y, a.y - d.x);
+        v[2] = make_float2(a.x - d.y, a.y + d.x);
+    }
+};
+
+template <> struct Dft<4> {
+    __device__ __forceinline__ static void run(float2* v)
+    {
+        float2 s0 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+        float2 d0 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+        float2 s1 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+        float2 d1 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
+        v[0] = make_float2(s0.x + s1.x, s0.y + s1.y);
+        v[2] = make_float2(s0.x - s1.x, s0.y - s1.y);
+        v[1] = make_float2(d0.x + d1.y, d0.y - d1.x);   // d0 - i d1
+        v[3] = make_float2(d0.x - d1.y, d0.y + d1.x);   // d0 + i d1
+    }
+};
+
+template <> struct Dft<5> {
+    __device__ __forceinline__ static void run(float2* v)
+    {
+        const float c1 = 0.309016994374947424f, c2 = -0.809016994374947424f;
+        const float s1 = 0.951056516295153572f, s2 = 0.587785252292473129f;
+        float2 t1 = make_float2(v[1].x + v[4].x, v[1].y + v[4].y);
+        float2 t2 = make_float2(v[2].x + v[3].x, v[2].y + v[3].y);
+        float2 t3 = make_float2(v[1].x - v[4].x, v[1].y - v[4].y);
+        float2 t4 = make_float2(v[2].x - v[3].x, v[2].y - v[3].y);
+        float2 a1 = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+        float2 a2 = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+        float2 b1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+        float2 b2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+        v[0] = make_float2(v[0].x + t1.x + t2.x, v[0].y + t1.y + t2.y);
+        v[1] = make_float2(a1.x + b1.y, a1.y - b1.x);
+        v[4] = make_float2(a1.x - b1.y, a1.y + b1.x);
+        v[2] = make_float2(a2.x + b2.y, a2.y - b2.x);
+        v[3] = make_float2(a2.x - b2.y, a2.y + b2.x);
+    }
+};
+
+template <> struct Dft<8> {
+    __device__ __forceinline__ static void run(float2* v)
+    {
+        const float h = 0.707106781186547524f;
+        float2 a[4], b[4];
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            a[n] = make_float2(v[n].x + v[n + 4].x, v[n].y + v[n + 4].y);
+            b[n] = make_float2(v[n].x - v[n + 4].x, v[n].y - v[n + 4].y);
+        }
+        // b[n] *= W_8^n
+        b[1] = make_float2(h * (b[1].x + b[1].y), h * (b[1].y - b[1].x));
+        b[2] = make_float2(b[2].y, -b[2].x);
+        b[3] = make_float2(h * (b[3].y - b[3].x), -h * (b[3].x + b[3].y));
+        Dft<4>::run(a);
+        Dft<4>::run(b);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[2 * k] = a[k];
+            v[2 * k + 1] = b[k];
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Stockham stage in LDS.  Element e of the FFT lives at lds[ebase + e*estr].
+// Butterfly q (q < L/R) reads q + r*L/R, writes (q - q%ns)*R + q%ns + r*ns.
+// Every thread of the workgroup must call it (it contains barriers).
+// ---------------------------------------------------------------------------
+
+template <int R, int NB>
+__device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int ns,
+                                          const float2* __restrict__ tw, int ebase, int estr,
+                                          int q0, int T)
+{
+    const int nb = L / R;
+    float2 v[NB][R];
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        const int q = q0 + i * T;
+        if (q < nb) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[i][r] = lds[ebase + (q + r * nb) * estr];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        const int q = q0 + i * T;
+        if (q < nb) {
+            const int k = q % ns;
+            if (ns > 1) {
+                const float2* w = tw + k * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], w[r - 1]);
+            }
+            Dft<R>::run(v[i]);
+            const int o = (q - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[ebase + (o + r * ns) * estr] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int MAXPTS>
+__device__ __forceinline__ void fft_lds(float2* lds, const SubPlan& sp, const float2* __restrict__ stw,
+                                        int ebase, int estr, int q0, int T)
+{
+    for (int s = 0; s < sp.nst; s++) {
+        const float2* tw = stw + sp.twoff[s];
+        switch (sp.radix[s]) {
+        case 2: stage_lds<2, MAXPTS / 2>(lds, sp.L, sp.ns[s], tw, ebase, estr, q0, T); break;
+        case 3: stage_lds<3, MAXPTS / 3>(lds, sp.L, sp.ns[s], tw, ebase, estr, q0, T); break;
+        case 4: stage_lds<4, MAXPTS / 4>(lds, sp.L, sp.ns[s], tw, ebase, estr, q0, T); break;
+        case 5: stage_lds<5, MAXPTS / 5>(lds, sp.L, sp.ns[s], tw, ebase, estr, q0, T); break;
+        default: stage_lds<8, MAXPTS / 8>(lds, sp.L, sp.ns[s], tw, ebase, estr, q0, T); break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2-bit decode: field s -> {+1, -1, +3.3359, -3.3359}[s]
+// (src/IA-32/DataUnpackers.cpp:212, 274, 362: bit 0 = sign, bit 1 = magnitude)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float map2bit(unsigned s)
+{
+    const float mag = (s & 2u) ? 3.3359f : 1.0f;
+    return (s & 1u) ? -mag : mag;
+}
+
+__device__ __forceinline__ unsigned field2(const uint8_t* __restrict__ raw, const Unpack& up, long long j)
+{
+    if (up.kind == kUnpackSingleChannel) {
+        // TwoBitSinglechannelUnpacker layout: 4 samples per byte, oldest in the MSB
+        return (raw[j >> 2] >> (6 - 2 * (int)(j & 3))) & 3u;
+    }
+    // TwoBitUnpacker / Mk5BUnpacker: one byte group of `step` bytes per time sample
+    return (raw[j * up.step + up.cbyte] >> up.shift) & 3u;
+}
+
+// ---------------------------------------------------------------------------
+// pass 1: unpack + window + column FFTs + four-step twiddle
+// ---------------------------------------------------------------------------
+
+struct P1Args {
+    int N1, N2;
+    long long M;
+    int B;
+    int twshift;
+    SubPlan sp;
+    const float2* stw;
+    const float2* twA;
+    const float2* twB;
+    const float* window;
+    JobDev jobs[kMaxJobs];
+};
+
+template <int MAXPTS>
+__global__ void __launch_bounds__(1024) k_pass1(P1Args a)
+{
+    extern __shared__ float2 lds[];
+    const JobDev J = a.jobs[blockIdx.z];
+    const int f = blockIdx.y;
+    const int B = a.B, N1 = a.N1, N2 = a.N2;
+    const int c0 = blockIdx.x * B;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const uint8_t* __restrict__ raw = J.raw + (long long)f * J.fft_stride;
+    const float* __restrict__ win = a.window;
+    const Unpack up = J.up;
+
+    for (int idx = tid; idx < N1 * B; idx += nth) {
+        const int e = idx / B, b = idx - e * B;
+        const long long j = 2 * ((long long)N2 * e + c0 + b);
+        const float x0 = map2bit(field2(raw, up, j)) * win[j];
+        const float x1 = map2bit(field2(raw, up, j + 1)) * win[j + 1];
+        lds[e * B + b] = make_float2(x0, x1);
+    }
+    __syncthreads();
+
+    fft_lds<MAXPTS>(lds, a.sp, a.stw, tid % B, B, tid / B, nth / B);
+
+    float2* __restrict__ Y = J.Y + (long long)f * a.M;
+    const unsigned mask = (1u << a.twshift) - 1u;
+    for (int idx = tid; idx < N1 * B; idx += nth) {
+        const int k1 = idx / B, b = idx - k1 * B;
+        const int n2 = c0 + b;
+        const long long e = ((long long)n2 * k1) % a.M;
+        const float2 w = cmul(a.twA[e >> a.twshift], a.twB[e & mask]);
+        Y[(long long)k1 * N2 + n2] = cmul(lds[k1 * B + b], w);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pass 2: row FFTs, r2c split, detection, resident accumulation
+// ---------------------------------------------------------------------------
+
+struct P2Args {
+    int N1, N2;
+    long long M;
+    int nfft;
+    int kahan;
+    SubPlan sp;
+    const float2* stw;
+    const float2* rA;
+    const float2* rB;
+    JobDev jobs[kMaxJobs];
+    XpolDev xp[kMaxJobs / 2];
+};
+
+__device__ __forceinline__ float2 r2c_bin(float2 Z, float2 Zm, float2 w)
+{
+    const float2 E = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+    const float2 O = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
+    const float2 wO = cmul(w, O);
+    return make_float2(E.x + wO.x, E.y + wO.y);
+}
+
+__device__ __forceinline__ void kahan_add(float* __restrict__ p, float v)
+{
+    float s = p[0], c = p[1];
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    p[0] = t;
+    p[1] = c;
+}
+
+template <int MAXPTS, int NSRC, int MAXBINS>
+__global__ void __launch_bounds__(1024) k_pass2(P2Args a)
+{
+    extern __shared__ float2 lds[];
+    const int p = blockIdx.x, grp = blockIdx.y;
+    const int N1 = a.N1, N2 = a.N2;
+    const int rowA = p;
+    const int rowB = (p == 0) ? 0 : N1 - p;
+    const bool self = (rowA == rowB);
+    const int tid = threadIdx.x, nth = blockDim.x;
+    constexpr int NSL = 2 * NSRC;
+    const int T = nth / NSL;
+    const int slot = tid / T, q0 = tid - slot * T;
+    float2* Yp[NSRC];
+    float* accp[NSRC];
+#pragma unroll
+    for (int s = 0; s < NSRC; s++) {
+        Yp[s] = a.jobs[grp * NSRC + s].Y;
+        accp[s] = a.jobs[grp * NSRC + s].acc;
+    }
+    const int nbins = (self ? 1 : 2) * N2;
+
+    float accP[NSRC][MAXBINS];
+    float2 accX[MAXBINS];
+    float nyqP[NSRC];
+    float nyqX = 0.f;
+#pragma unroll
+    for (int i = 0; i < MAXBINS; i++) {
+#pragma unroll
+        for (int s = 0; s < NSRC; s++) accP[s][i] = 0.f;
+        accX[i] = make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int s = 0; s < NSRC; s++) nyqP[s] = 0.f;
+
+    for (int f = 0; f < a.nfft; f++) {
+        for (int idx = tid; idx < NSL * N2; idx += nth) {
+            const int sl = idx / N2, e = idx - sl * N2;
+            const int row = (sl & 1) ? rowB : rowA;
+            const float2* Ys = (NSRC == 2 && (sl >> 1)) ? Yp[NSRC - 1] : Yp[0];
+            lds[sl * N2 + e] = Ys[(long long)f * a.M + (long long)row * N2 + e];
+        }
+        __syncthreads();
+        fft_lds<MAXPTS>(lds, a.sp, a.stw, slot * N2, 1, q0, T);   // blockDim == NSL * T
+
+#pragma unroll
+        for (int i = 0; i < MAXBINS; i++) {
+            const int idx = tid + i * nth;
+            if (idx < nbins) {
+                const int which = idx / N2, k2 = idx - which * N2;
+                const int row = which ? rowB : rowA;
+                int msl, mk2;
+                if (row == 0) { msl = which; mk2 = (k2 == 0) ? 0 : N2 - k2; }
+                else { msl = self ? which : 1 - which; mk2 = N2 - 1 - k2; }
+                const float2 w = cmul(a.rA[row], a.rB[k2]);
+                float2 X[NSRC];
+#pragma unroll
+                for (int s = 0; s < NSRC; s++) {
+                    const float2 Z = lds[(2 * s + which) * N2 + k2];
+                    const float2 Zm = lds[(2 * s + msl) * N2 + mk2];
+                    X[s] = r2c_bin(Z, Zm, w);
+                    accP[s][i] += X[s].x * X[s].x + X[s].y * X[s].y;
+                }
+                if (NSRC == 2) {
+                    // X0 * conj(X1)
+                    accX[i].x += X[0].x * X[NSRC - 1].x + X[0].y * X[NSRC - 1].y;
+                    accX[i].y += X[0].y * X[NSRC - 1].x - X[0].x * X[NSRC - 1].y;
+                }
+                if (row == 0 && k2 == 0) {
+                    float xn[NSRC];
+#pragma unroll
+                    for (int s = 0; s < NSRC; s++) {
+                        const float2 Z = lds[(2 * s) * N2];
+                        xn[s] = Z.x - Z.y;   // X[M] = Re Z0 - Im Z0
+                        nyqP[s] += xn[s] * xn[s];
+                    }
+                    if (NSRC == 2) nyqX += xn[0] * xn[NSRC - 1];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < MAXBINS; i++) {
+        const int idx = tid + i * nth;
+        if (idx < nbins) {
+            const int which = idx / N2, k2 = idx - which * N2;
+            const int row = which ? rowB : rowA;
+            const long long o = (long long)row * N2 + k2;
+#pragma unroll
+            for (int s = 0; s < NSRC; s++) {
+                float* acc = accp[s] + 2 * o;
+                if (a.kahan) kahan_add(acc, accP[s][i]);
+                else acc[0] += accP[s][i];
+            }
+            if (NSRC == 2) {
+                float* xa = a.xp[grp].acc + 4 * o;
+                if (a.kahan) {
+                    float s0 = xa[0], s1 = xa[1], c0 = xa[2], c1 = xa[3];
+                    float y0 = accX[i].x - c0, y1 = accX[i].y - c1;
+                    float t0 = s0 + y0, t1 = s1 + y1;
+                    xa[2] = (t0 - s0) - y0; xa[3] = (t1 - s1) - y1;
+                    xa[0] = t0; xa[1] = t1;
+                } else {
+                    xa[0] += accX[i].x;
+                    xa[1] += accX[i].y;
+                }
+            }
+            if (row == 0 && k2 == 0) {
+#pragma unroll
+                for (int s = 0; s < NSRC; s++) {
+                    float* acc = accp[s] + 2 * a.M;
+                    if (a.kahan) kahan_add(acc, nyqP[s]);
+                    else acc[0] += nyqP[s];
+                }
+                if (NSRC == 2) {
+                    float* xa = a.xp[grp].acc + 4 * a.M;
+                    if (a.kahan) {
+                        float s0 = xa[0], c0 = xa[2];
+                        float y0 = nyqX - c0, t0 = s0 + y0;
+                        xa[2] = (t0 - s0) - y0; xa[0] = t0;
+                    } else {
+                        xa[0] += nyqX;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// finalize: permuted (sum, comp) accumulators -> natural-order float spectrum,
+// scaled in FP64, accumulators cleared for the next spectrum.
+// ---------------------------------------------------------------------------
+
+__global__ void k_finalize(float* __restrict__ acc, float* __restrict__ out, long long M, int N1, int N2,
+                           double scale)
+{
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k <= M;
+         k += (long long)gridDim.x * blockDim.x) {
+        const long long idx = (k == M) ? M : (k % N1) * (long long)N2 + k / N1;
+        const double v = (double)acc[2 * idx] - (double)acc[2 * idx + 1];
+        out[k] = (float)(v * scale);
+    }
+}
+
+__global__ void k_finalize_xpol(float* __restrict__ acc, float* __restrict__ out, long long M, int N1, int N2,
+                                double scale)
+{
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k <= M;
+         k += (long long)gridDim.x * blockDim.x) {
+        const long long idx = (k == M) ? M : (k % N1) * (long long)N2 + k / N1;
+        const double re = (double)acc[4 * idx] - (double)acc[4 * idx + 2];
+        const double im = (double)acc[4 * idx + 1] - (double)acc[4 * idx + 3];
+        out[2 * k] = (float)(re * scale);
+        // DC and Nyquist cross products are real (TaskCoreIPP.cpp:482-485)
+        out[2 * k + 1] = (k == 0 || k == M) ? 0.0f : (float)(im * scale);
+    }
+}
+
+__global__ void k_zero(uint32_t* __restrict__ p, size_t nwords)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// phase-cal: fold x[j] * rot[j mod L] into bin j mod T, FP64 accumulation
+// (TaskCoreIPP.cpp:633-693 folds per FFT starting at rotator index 0)
+// ---------------------------------------------------------------------------
+
+struct PcArgs {
+    PcalGeom g;
+    long long N;
+    JobDev jobs[kMaxJobs];
+};
+
+__global__ void __launch_bounds__(1024) k_pcal(PcArgs a)
+{
+    __shared__ double2 red[1024];
+    const JobDev J = a.jobs[blockIdx.z];
+    const int f = blockIdx.y, blk = blockIdx.x;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int T = a.g.T, L = a.g.L;
+    double ar = 0.0, ai = 0.0;
+    if ((J.pcal_mask >> f) & 1ull) {
+        const uint8_t* __restrict__ raw = J.raw + (long long)f * J.fft_stride;
+        const Unpack up = J.up;
+        const long long s0 = (long long)blk * a.g.chunk;
+        const long long s1 = min(s0 + (long long)a.g.chunk, a.N);
+        int ri = (int)((s0 + tid) % L);
+        for (long long j = s0 + tid; j < s1; j += nth) {
+            const float x = map2bit(field2(raw, up, j));
+            const float2 r = a.g.d_rot[ri];
+            ar += (double)x * (double)r.x;
+            ai += (double)x * (double)r.y;
+            ri += nth;
+            if (ri >= L) ri -= L;
+        }
+    }
+    red[tid] = make_double2(ar, ai);
+    __syncthreads();
+    if (tid < T) {
+        double sr = 0.0, si = 0.0;
+        for (int k = tid; k < nth; k += T) { sr += red[k].x; si += red[k].y; }
+        double* part = J.pcal_part + 2 * (((long long)f * a.g.nblk + blk) * T + tid);
+        part[0] = sr;
+        part[1] = si;
+    }
+}
+
+struct PcRedArgs {
+    int T, nblk, nfft;
+    JobDev jobs[kMaxJobs];
+};
+
+__global__ void k_pcal_reduce(PcRedArgs a)
+{
+    const JobDev J = a.jobs[blockIdx.x];
+    const int t = threadIdx.x;
+    if (t >= a.T) return;
+    double sr = 0.0, si = 0.0;
+    for (int f = 0; f < a.nfft; f++) {
+        if (!((J.pcal_mask >> f) & 1ull)) continue;
+        for (int b = 0; b < a.nblk; b++) {
+            const double* part = J.pcal_part + 2 * (((long long)f * a.nblk + b) * a.T + t);
+            sr += part[0];
+            si += part[1];
+        }
+    }
+    J.pcal_acc[2 * t] += sr;
+    J.pcal_acc[2 * t + 1] += si;
+}
+
+// Folded vector out (FP32, as the reference's pcal sink carries) and tones in
+// FP64: tone n = (1/T) sum_b acc[b] e^{-2 pi i (n*step) b / T} (PCal.cpp:489-515).
+__global__ void k_pcal_emit(double* __restrict__ acc, float* __restrict__ out, double* __restrict__ tones,
+                            int T, int ntones, int step)
+{
+    __shared__ double2 s[1024];
+    const int t = threadIdx.x;
+    if (t < T) s[t] = make_double2(acc[2 * t], acc[2 * t + 1]);
+    __syncthreads();
+    if (t < T && out) {
+        out[2 * t] = (float)s[t].x;
+        out[2 * t + 1] = (float)s[t].y;
+    }
+    if (t < ntones && tones) {
+        const long long bin = (long long)t * step;
+        double re = 0.0, im = 0.0;
+        if (bin < T) {
+            for (int b = 0; b < T; b++) {
+                const long long e = (bin * b) % T;
+                double sn, cs;
+                sincospi(-2.0 * (double)e / (double)T, &sn, &cs);
+                re += s[b].x * cs - s[b].y * sn;
+                im += s[b].x * sn + s[b].y * cs;
+            }
+        }
+        tones[2 * t] = re / T;
+        tones[2 * t + 1] = im / T;
+    }
+    __syncthreads();
+    if (t < T) {
+        acc[2 * t] = 0.0;
+        acc[2 * t + 1] = 0.0;
+    }
+}
+
+__global__ void k_unpack(const uint8_t* __restrict__ raw, Unpack up, long long count, float* __restrict__ out)
+{
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < count;
+         j += (long long)gridDim.x * blockDim.x)
+        out[j] = map2bit(field2(raw, up, j));
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+static size_t p1_lds(const Plan& p) { return (size_t)p.N1 * p.B * sizeof(float2); }
+static size_t p2_lds(const Plan& p, int nsrc) { return (size_t)2 * nsrc * p.N2 * sizeof(float2); }
+
+hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, hipStream_t s)
+{
+    if (njobs < 1 || njobs > kMaxJobs || nfft < 1) return hipErrorInvalidValue;
+    P1Args a;
+    a.N1 = p.N1; a.N2 = p.N2; a.M = p.M; a.B = p.B; a.twshift = p.twshift;
+    a.sp = p.sp1; a.stw = p.d_stw1; a.twA = p.d_twA; a.twB = p.d_twB; a.window = p.d_window;
+    for (int j = 0; j < njobs; j++) a.jobs[j] = jobs[j];
+    dim3 grid(p.N2 / p.B, nfft, njobs);
+    const size_t lds = p1_lds(p);
+    switch (p.p1_maxpts) {
+    case 16: hipLaunchKernelGGL(k_pass1<16>, grid, dim3(p.p1_threads), lds, s, a); break;
+    case 32: hipLaunchKernelGGL(k_pass1<32>, grid, dim3(p.p1_threads), lds, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const XpolDev* xp, int nfft,
+                        int kahan, hipStream_t s)
+{
+    const int nsrc = xp ? 2 : 1;
+    if (njobs < 1 || njobs > kMaxJobs || (njobs % nsrc) != 0 || nfft < 1) return hipErrorInvalidValue;
+    P2Args a;
+    a.N1 = p.N1; a.N2 = p.N2; a.M = p.M; a.nfft = nfft; a.kahan = kahan;
+    a.sp = p.sp2; a.stw = p.d_stw2; a.rA = p.d_rA; a.rB = p.d_rB;
+    for (int j = 0; j < njobs; j++) a.jobs[j] = jobs[j];
+    if (xp) for (int g = 0; g < njobs / 2; g++) a.xp[g] = xp[g];
+    const int threads = nsrc * 2 * p.sp2.T;
+    dim3 grid(p.npairs, njobs / nsrc);
+    const size_t lds = p2_lds(p, nsrc);
+    const int bins = (2 * p.N2 + threads - 1) / threads;
+    if (bins > 16) return hipErrorInvalidValue;
+    if (nsrc == 1) {
+        switch (p.p2_maxpts) {
+        case 16: hipLaunchKernelGGL((k_pass2<16, 1, 16>), grid, dim3(threads), lds, s, a); break;
+        case 32: hipLaunchKernelGGL((k_pass2<32, 1, 16>), grid, dim3(threads), lds, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (p.p2_maxpts) {
+        case 16: hipLaunchKernelGGL((k_pass2<16, 2, 16>), grid, dim3(threads), lds, s, a); break;
+        case 32: hipLaunchKernelGGL((k_pass2<32, 2, 16>), grid, dim3(threads), lds, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const Plan& p, const float* acc, float* out, double scale, hipStream_t s)
+{
+    const long long n = p.M + 1;
+    const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, s, const_cast<float*>(acc), out, p.M, p.N1,
+                       p.N2, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize_xpol(const Plan& p, const float* acc, float* out, double scale, hipStream_t s)
+{
+    const long long n = p.M + 1;
+    const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_finalize_xpol, dim3(blocks), dim3(256), 0, s, const_cast<float*>(acc), out, p.M,
+                       p.N1, p.N2, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s)
+{
+    if (bytes % 4) return hipErrorInvalidValue;
+    const size_t n = bytes / 4;
+    if (n == 0) return hipSuccess;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, s, (uint32_t*)ptr, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int njobs, int nfft, hipStream_t s)
+{
+    if (njobs < 1 || njobs > kMaxJobs || nfft < 1 || nfft > 64) return hipErrorInvalidValue;
+    PcArgs a;
+    a.g = g; a.N = N;
+    for (int j = 0; j < njobs; j++) a.jobs[j] = jobs[j];
+    hipLaunchKernelGGL(k_pcal, dim3(g.nblk, nfft, njobs), dim3(g.threads), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    PcRedArgs r;
+    r.T = g.T; r.nblk = g.nblk; r.nfft = nfft;
+    for (int j = 0; j < njobs; j++) r.jobs[j] = jobs[j];
+    const int th = ((g.T + 63) / 64) * 64;
+    hipLaunchKernelGGL(k_pcal_reduce, dim3(njobs), dim3(th), 0, s, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_pcal_emit(const PcalGeom& g, double* acc, float* out, double* tones, int ntones,
+                            int tone_step, hipStream_t s)
+{
+    const int th = ((std::max(g.T, ntones) + 63) / 64) * 64;
+    if (th > 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pcal_emit, dim3(1), dim3(th), 0, s, acc, out, tones, g.T, ntones, tone_step);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack(const uint8_t* raw, Unpack up, long long count, float* out, hipStream_t s)
+{
+    const int blocks = (int)std::min<long long>((count + 255) / 256, 16384);
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unpack, dim3(blocks), dim3(256), 0, s, raw, up, count, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// planning (host): factorisation, thread shapes, FP64-derived tables
+// ---------------------------------------------------------------------------
+
+static bool smooth235(long long n)
+{
+    if (n < 1) return false;
+    for (int p : {2, 3, 5})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+// Radix list for an L-point sub-FFT: 5s, 3s, then 8s, a 4 or 2.
+static bool subplan_radices(int L, SubPlan& sp)
+{
+    int n = L, ns = 1;
+    sp.L = L;
+    sp.nst = 0;
+    int twl = 0;
+    auto push = [&](int R) {
+        sp.radix[sp.nst] = R;
+        sp.ns[sp.nst] = ns;
+        sp.twoff[sp.nst] = twl;
+        twl += ns * (R - 1);
+        ns *= R;
+        sp.nst++;
+    };
+    while (n % 5 == 0) { if (sp.nst >= kMaxStages) return false; push(5); n /= 5; }
+    while (n % 3 == 0) { if (sp.nst >= kMaxStages) return false; push(3); n /= 3; }
+    while (n % 8 == 0) { if (sp.nst >= kMaxStages) return false; push(8); n /= 8; }
+    if (n % 4 == 0) { push(4); n /= 4; }
+    if (n % 2 == 0) { push(2); n /= 2; }
+    sp.twlen = twl;
+    return n == 1 && sp.nst > 0;
+}
+
+// Smallest thread count per FFT such that each stage's butterflies per thread
+// fit the register budget MAXPTS/R.
+static int subplan_threads(const SubPlan& sp, int maxpts)
+{
+    int T = 1;
+    for (int s = 0; s < sp.nst; s++) {
+        const int R = sp.radix[s];
+        const int nb = sp.L / R;
+        const int per = maxpts / R;
+        T = std::max(T, (nb + per - 1) / per);
+    }
+    return T;
+}
+
+std::string plan_geometry(Plan& p, long long N)
+{
+    if (N < 16 || (N % 2) != 0) return "FFT length must be even and >= 16";
+    const long long M = N / 2;
+    if (!smooth235(M)) return "FFT length must have prime factors 2, 3, 5 only";
+    if (M > (1LL << 30)) return "FFT length too large";
+    // N1 <= N2 closest to sqrt(M)
+    long long best1 = 0;
+    for (long long d = 1; d * d <= M; d++) {
+        if (M % d) continue;
+        if (!smooth235(d)) continue;
+        best1 = d;
+    }
+    if (best1 < 2) return "no usable factorisation";
+    p.N = N;
+    p.M = M;
+    p.N1 = (int)best1;
+    p.N2 = (int)(M / best1);
+    if (p.N2 > 16384) return "FFT length too large for two passes";
+    if (!subplan_radices(p.N1, p.sp1) || !subplan_radices(p.N2, p.sp2)) return "radix decomposition failed";
+
+    // pass 1: B columns of N1 points, LDS budget 80 KB, <= 1024 threads
+    p.p1_maxpts = 16;
+    int T1 = subplan_threads(p.sp1, 16);
+    if (T1 > 512) { p.p1_maxpts = 32; T1 = subplan_threads(p.sp1, 32); }
+    if (T1 > 1024) return "column FFT too large";
+    p.sp1.T = T1;
+    int B = 1;
+    for (int c : {16, 8, 4, 2, 1}) {
+        if ((p.N2 % c) == 0 && (size_t)c * p.N1 * sizeof(float2) <= 80 * 1024 && c * T1 <= 1024) {
+            B = c;
+            break;
+        }
+    }
+    p.B = B;
+    p.p1_threads = B * T1;
+
+    // pass 2: 2 rows (x2 for cross-power pairs) of N2 points, <= 16 bins/thread
+    p.p2_maxpts = 16;
+    int T2 = subplan_threads(p.sp2, 16);
+    while (2 * p.N2 > 16 * 2 * T2) T2++;
+    if (4 * T2 > 1024) { p.p2_maxpts = 32; T2 = subplan_threads(p.sp2, 32); while (2 * p.N2 > 16 * 2 * T2) T2++; }
+    if (4 * T2 > 1024) return "row FFT too large";
+    p.sp2.T = T2;
+    p.p2_threads = 2 * T2;
+    p.npairs = p.N1 / 2 + 1;
+    p.twshift = 10;
+    return "";
+}
+
+static void stage_twiddles(const SubPlan& sp, std::vector<float2>& out)
+{
+    out.assign(std::max(sp.twlen, 1), make_float2(1.f, 0.f));
+    for (int s = 0; s < sp.nst; s++) {
+        const int R = sp.radix[s], ns = sp.ns[s];
+        const long long D = (long long)ns * R;
+        for (int k = 0; k < ns; k++)
+            for (int r = 1; r < R; r++) {
+                const long long e = ((long long)k * r) % D;
+                const double a = -2.0 * M_PI * (double)e / (double)D;
+                out[sp.twoff[s] + k * (R - 1) + (r - 1)] = make_float2((float)cos(a), (float)sin(a));
+            }
+    }
+}
+
+static float2 wroot(long long e, long long D)
+{
+    e %= D;
+    const double a = -2.0 * M_PI * (double)e / (double)D;
+    return make_float2((float)cos(a), (float)sin(a));
+}
+
+// Window values as the reference computes them (TaskCoreIPP.cpp:569-605);
+// the IPP windows follow Intel's published symmetric formulas.
+static double window_value(int type, long long i, long long n)
+{
+    const double n1 = (double)(n - 1);
+    switch (type) {
+    case 0: return 1.0;
+    case 1: return cos(M_PI * ((double)(float)i - 0.5 * (double)(float)n + 0.5) / (double)(float)n);
+    case 2: {
+        const double w = cos(M_PI * ((double)(float)i - 0.5 * (double)(float)n + 0.5) / (double)(float)n);
+        return w * w;
+    }
+    case 3: return 0.54 - 0.46 * cos(2.0 * M_PI * (double)i / n1);
+    case 4: return 0.5 - 0.5 * cos(2.0 * M_PI * (double)i / n1);
+    case 5: {
+        const double al = 0.5, x = 2.0 * M_PI * (double)i / n1;
+        return (al + 1.0) / 2.0 - 0.5 * cos(x) - (al / 2.0) * cos(2.0 * x);
+    }
+    default: return 1.0;
+    }
+}
+
+template <typename T>
+static hipError_t upload(T** d, const std::vector<T>& h, hipStream_t s)
+{
+    hipError_t e = hipMalloc((void**)d, h.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+std::string plan_create(Plan& p, long long N, int window, hipStream_t stream)
+{
+    std::string err = plan_geometry(p, N);
+    if (!err.empty()) return err;
+    p.window = window;
+    std::vector<float2> t;
+    stage_twiddles(p.sp1, t);
+    if (upload(&p.d_stw1, t, stream) != hipSuccess) return "device allocation failed (stage twiddles)";
+    stage_twiddles(p.sp2, t);
+    if (upload(&p.d_stw2, t, stream) != hipSuccess) return "device allocation failed (stage twiddles)";
+    {
+        const long long nb = 1LL << p.twshift;
+        const long long na = (p.M + nb - 1) / nb + 1;
+        std::vector<float2> A(na), Bt(nb);
+        for (long long i = 0; i < na; i++) A[i] = wroot(i * nb, p.M);
+        for (long long i = 0; i < nb; i++) Bt[i] = wroot(i, p.M);
+        if (upload(&p.d_twA, A, stream) != hipSuccess || upload(&p.d_twB, Bt, stream) != hipSuccess)
+            return "device allocation failed (four-step twiddles)";
+    }
+    {
+        std::vector<float2> rA(p.N1), rB(p.N2);
+        for (int k = 0; k < p.N1; k++) rA[k] = wroot(k, p.N);
+        for (int k = 0; k < p.N2; k++) rB[k] = wroot((long long)p.N1 * k, p.N);
+        if (upload(&p.d_rA, rA, stream) != hipSuccess || upload(&p.d_rB, rB, stream) != hipSuccess)
+            return "device allocation failed (split twiddles)";
+    }
+    {
+        std::vector<float> w(p.N);
+        for (long long i = 0; i < p.N; i++) w[i] = (float)window_value(window, i, p.N);
+        if (upload(&p.d_window, w, stream) != hipSuccess) return "device allocation failed (window)";
+    }
+    return "";
+}
+
+void plan_destroy(Plan& p)
+{
+    for (void* q : {(void*)p.d_stw1, (void*)p.d_stw2, (void*)p.d_twA, (void*)p.d_twB, (void*)p.d_rA,
+                    (void*)p.d_rB, (void*)p.d_window})
+        if (q) (void)hipFree(q);
+    p.d_stw1 = p.d_stw2 = p.d_twA = p.d_twB = p.d_rA = p.d_rB = nullptr;
+    p.d_window = nullptr;
+}
+
+}  // namespace swspec

@@ -1,0 +1,199 @@
+"""GPU parity tests: TaskCoreHIP (HIP kernels via the C ABI) against the oracle.
+
+Tolerances (floating point, stated per the north star):
+  spectra / cross-spectra : normwise relative error <= 1e-5 vs the FP64 oracle
+                            and vs the FP32 restatement of the reference core
+  decoded samples         : bit-exact
+  phase-cal folded vector : relative error <= 1e-9 vs the FP64 fold with the
+                            reference's FP32 rotator (FP64 accumulation on both)
+"""
+import numpy as np
+import pytest
+
+from gpu_util import oracle_settings, relerr, run_device, say
+from swspec_amd import synth
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+# ---------------------------------------------------------------- decoders
+
+@pytest.mark.parametrize("fmt,nch,incr", [("RawSigned", 1, 1), ("RawSigned", 4, 1), ("RawSigned", 4, 0),
+                                          ("RawSigned", 8, 1), ("RawSigned", 8, 0), ("RawUnsigned", 12, 1),
+                                          ("Mk5B", 4, 1), ("Mk5B", 8, 0)])
+def test_unpack_bitexact(hip, oracle, fmt, nch, incr):
+    import torch
+    say("unpack", fmt, nch, incr)
+    count = 8192
+    raw = np.random.default_rng(nch * 10 + incr).integers(0, 256, count * max(nch // 4, 1), dtype=np.uint8)
+    if nch == 1:
+        raw = raw[: count // 4]
+    s = hip.derive_settings(FFTpoints=4096, FFTIntegrationTimeSec=1.0, FFToverlapFactor=1, BandwidthHz=8e6,
+                            BitsPerSample=2, SourceChannels=nch, SourceFormat=fmt, ChannelOrderIncreasing=bool(incr))
+    os_ = oracle_settings(oracle, s)
+    d_raw = torch.from_numpy(raw).cuda()
+    for ch in range(nch):
+        d_out = torch.empty(count, dtype=torch.float32, device="cuda")
+        hip.unpack_device(s, d_raw, ch, count, d_out)
+        ref = oracle.unpack(os_, raw, count, ch)[:count]
+        assert np.array_equal(d_out.cpu().numpy(), ref), (fmt, nch, incr, ch)
+
+
+# ---------------------------------------------------------------- spectra
+
+SIZES = [
+    (4096, "Hann", 5), (20000, "Cosine2", 3), (32000, "Hann", 4), (65536, "Hamming", 3),
+    (320000, "Blackman", 2), (1 << 20, "Cosine", 2), (1000000, "None", 2), (3200000, "Cosine2", 2),
+]
+
+
+@pytest.mark.parametrize("n,win,nf", SIZES)
+def test_spectrum_vs_oracle(hip, oracle, n, win, nf):
+    say("spectrum", n, win, nf)
+    fs = 32e6
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=1, WindowType=win,
+                            BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=1, MaxSourceBufferMB=4096)
+    assert s.core_overlapped_ffts == nf
+    raw = synth.raw_1ch(n + nf, n * nf, tones=[(0.2, 4.8e6, 0.3)], fs=fs)
+    m, auto, _, _, core = run_device(s, [raw])
+    os_ = oracle_settings(oracle, s)
+    m64, a64, _, _ = oracle.domaths(os_, [raw], 64)
+    m32, a32, _, _ = oracle.domaths(os_, [raw], 32)
+    assert m == m64 == m32 == 1
+    e64 = relerr(auto[0], a64[0])
+    e32 = relerr(auto[0], a32[0])
+    say("  err vs fp64 %.3e  vs fp32 restatement %.3e" % (e64, e32))
+    assert e64 <= TOL and e32 <= TOL
+    core.finalize()
+
+
+def test_multichannel_mk5b_and_pcal(hip, oracle):
+    """cfg2 geometry at reduced length: 4-channel Mk5B payload, 3.2M points,
+    Cosine2, phase-cal comb; all four channels in one pass."""
+    n, nf, fs = 3200000, 2, 32e6
+    say("cfg2 small", n, nf)
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=1,
+                            WindowType="Cosine2", BandwidthHz=16e6, BitsPerSample=2, SourceChannels=4,
+                            SourceFormat="Mk5B", ExtractPCal=True, MaxSourceBufferMB=4096)
+    assert s.extract_pcal == 1 and s.pcal_tonebins == 32 and s.core_overlapped_ffts == nf
+    rng = np.random.default_rng(42)
+    tones = [synth.pcal_comb(fs, fs / 2, amp=0.141, rng=rng) for _ in range(4)]
+    raw = synth.raw_multi(7, 4, n * nf, tones_per_ch=tones, fs=fs)
+    m, auto, _, pc, core = run_device(s, [raw], channels=[0, 1, 2, 3])
+    assert m == 1
+    for ch in range(4):
+        os_ = oracle_settings(oracle, hip.derive_settings(**dict(s.keys, UseFile1Channel=ch + 1)))
+        _, a64, _, p64 = oracle.domaths(os_, [raw], 64)
+        e = relerr(auto[ch], a64[0])
+        # FP64 fold with the FP32 rotator, as the kernel does
+        x = oracle.unpack(os_, raw, n * nf, ch).astype(np.float64)
+        acc = np.zeros(64)
+        for f in range(nf):
+            acc = oracle.pcal_fold64(x[f * n:(f + 1) * n], fs, 10e3, 32, 3200, rot32=True, acc=acc)
+        pref = acc[0::2] + 1j * acc[1::2]
+        ep = relerr(pc[ch][0], pref)
+        ep64 = relerr(pc[ch][0], p64[0][0])
+        tones_gpu = core.pcal_tones(ch, 0)
+        tones_ref = oracle.pcal_tones64(acc, len(tones_gpu), 1e6, fs)
+        et = relerr(tones_gpu, tones_ref)
+        say("  ch%d spectrum %.3e pcal %.3e (vs fp64 rotator %.3e) tones %.3e" % (ch, e, ep, ep64, et))
+        assert e <= TOL
+        assert ep <= 1e-6 and ep64 <= 1e-5 and et <= 1e-6
+    core.finalize()
+
+
+def test_xpol_vs_oracle(hip, oracle):
+    n, nf, fs = 1 << 16, 4, 64e6
+    say("xpol", n, nf)
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=1, WindowType="Hann",
+                            BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=1, DoCrossPolarization=True,
+                            MaxSourceBufferMB=4096, num_sources=2)
+    assert s.num_xpols == 1
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(n * nf)
+    b = 0.2 * a + np.sqrt(1 - 0.04) * rng.standard_normal(n * nf)
+    ra, rb = synth.pack_1ch(synth.quantize(a)), synth.pack_1ch(synth.quantize(b))
+    m, auto, xp, _, core = run_device(s, [ra, rb])
+    os_ = oracle_settings(oracle, s)
+    _, a64, x64, _ = oracle.domaths(os_, [ra, rb], 64)
+    _, a32, x32, _ = oracle.domaths(os_, [ra, rb], 32)
+    assert m == 1
+    ex = relerr(xp[0], x64[0])
+    say("  auto %.3e %.3e xpol %.3e (fp32 restatement %.3e)" % (relerr(auto[0], a64[0]), relerr(auto[1], a64[1]),
+                                                               ex, relerr(xp[0], x32[0])))
+    assert relerr(auto[0], a64[0]) <= TOL and relerr(auto[1], a64[1]) <= TOL
+    assert ex <= TOL and relerr(xp[0], x32[0]) <= TOL
+    assert xp[0][0, 0].imag == 0 and xp[0][0, -1].imag == 0
+    core.finalize()
+
+
+def test_schedule_overlap_partial_multispectrum(hip, oracle):
+    """Overlap factor 2, several spectra per buffer, trailing partial spectrum."""
+    n, fs = 6400, 32e6     # N % rotatorlen (3200) == 0 keeps phase-cal on
+    say("schedule overlap")
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * 3 / fs, FFToverlapFactor=2, WindowType="Cosine",
+                            BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=1, MaxSourceBufferMB=4096,
+                            ExtractPCal=True, PCalOffsetHz=10e3)
+    K = s.core_overlapped_ffts
+    assert K == 5
+    raw = synth.raw_1ch(1, n * 3 * 3 + n // 2 * 3, fs=fs)   # 3 full spectra + partial
+    m, auto, _, pc, core = run_device(s, [raw], ffts_per_chunk=2)
+    os_ = oracle_settings(oracle, s)
+    m64, a64, _, p64 = oracle.domaths(os_, [raw], 64)
+    assert m == m64 == 4
+    assert relerr(auto[0], a64[0]) <= TOL
+    say("  pcal err %.3e" % relerr(pc[0], p64[0]))
+    assert relerr(pc[0], p64[0]) <= 1e-5
+    core.finalize()
+
+
+def test_host_mode_matches_device_mode(hip, oracle):
+    n, nf, fs = 32000, 6, 32e6
+    say("host mode")
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=1, WindowType="Hann",
+                            BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=1, MaxSourceBufferMB=4096)
+    raw = synth.raw_1ch(9, n * nf * 2, fs=fs)   # two spectra in one buffer
+    m, auto, _, _, core = run_device(s, [raw])
+    core.finalize()
+    c2 = hip.TaskCoreHIP(1)
+    c2.prepare(s)
+    inb = hip.Buffer(len(raw), pinned=True)
+    inb.getData()[:] = raw
+    inb.setLength(len(raw))
+    out = hip.Buffer(2 * s.fft_bytes_ssb)
+    c2.run([inb], [out], [], [])
+    k = c2.join()
+    assert k == m == 2
+    assert out.getLength() == 2 * s.fft_bytes_ssb
+    host = out.getData().view(np.float32).reshape(2, -1)
+    assert np.array_equal(host, auto[0])
+    # combineResults adds the first spectrum into a combined buffer (TaskCoreIPP.cpp:311-349)
+    comb = hip.Buffer(s.fft_bytes_ssb)
+    c2.resetBuffer(comb)
+    c2.combineResults([comb], [])
+    assert np.array_equal(comb.getData().view(np.float32), auto[0][0])
+    c2.finalize()
+
+
+def test_parseval_full_cfg2_length(hip):
+    """Size-independent property at the full 3.2M-point length with many FFTs:
+    sum_k c_k P[k] = N * mean_f sum_n (w x)^2 (c = 1 at DC/Nyquist, 2 otherwise)."""
+    import torch
+    n, nf, fs = 3200000, 12, 32e6
+    say("parseval 3.2M x", nf)
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=1,
+                            WindowType="Cosine2", BandwidthHz=16e6, BitsPerSample=2, SourceChannels=4,
+                            SourceFormat="Mk5B", MaxSourceBufferMB=4096)
+    raw = np.random.default_rng(77).integers(0, 256, n * nf, dtype=np.uint8)
+    m, auto, _, _, core = run_device(s, [raw], channels=[2])
+    P = auto[0][0].astype(np.float64)
+    lhs = P[0] + P[-1] + 2 * P[1:-1].sum()
+    lev = torch.tensor(synth.LEVELS, dtype=torch.float64, device="cuda")
+    x = lev[((torch.from_numpy(raw).cuda().long() >> 4) & 3)].reshape(nf, n)
+    i = torch.arange(n, dtype=torch.float64, device="cuda")
+    w = torch.sin(np.pi * (2 * i + 1) / (2 * n)) ** 2
+    rhs = n * float(((x * w) ** 2).sum(dim=1).mean())
+    say("  parseval rel %.3e" % (abs(lhs - rhs) / rhs))
+    assert abs(lhs - rhs) <= 1e-5 * rhs
+    core.finalize()
