@@ -85,7 +85,7 @@ def flops_per_sample(plan):
     return {"pass1": 1.0 + 2.5 * l1, "pass2": 2.0 + 2.5 * l2, "pcal": 4.0, "finalize": 0.0}
 
 
-def cpu_baseline(raw_host, threads, ffts_per_thread=4):
+def cpu_baseline(raw_host, threads, ffts_per_thread=12):
     """FP32 restatement of TaskCoreIPP::doMaths (oracle/) on `threads` threads,
     each a core object working on its own 3.2M-point FFTs of one channel."""
     from oracle import oracle as O
@@ -195,7 +195,7 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log("cpu baseline on %d threads ..." % threads)
-        raw_host = raw[: N * 4 * ((threads + NCH - 1) // NCH) + N].cpu().numpy()
+        raw_host = raw[: N * 12 * ((threads + NCH - 1) // NCH) + N].cpu().numpy()
         cpu = cpu_baseline(raw_host, threads)
 
     out = {

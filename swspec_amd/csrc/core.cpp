@@ -310,7 +310,7 @@ static int core_alloc(swspec_hip_core* c)
         HIPCHK(c, (hipError_t)launch_zero(c->d_xacc, 4 * M1 * sizeof(float), c->stream));
     }
     if (c->cfg.extract_pcal) {
-        const size_t part = (size_t)c->njobs * c->G * c->pg.nblk * c->pg.T * 2;
+        const size_t part = (size_t)c->njobs * c->G * c->pg.nblk_max * c->pg.T * 2;
         HIPCHK(c, hipMalloc(&c->d_pcal_part, part * sizeof(double)));
         HIPCHK(c, hipMalloc(&c->d_pcal_acc, (size_t)c->njobs * c->pg.T * 2 * sizeof(double)));
         HIPCHK(c, (hipError_t)launch_zero(c->d_pcal_acc, (size_t)c->njobs * c->pg.T * 2 * sizeof(double),
@@ -370,15 +370,7 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
             return fail(c, SWSPEC_EINVAL, "phase-cal geometry invalid");
         c->pg.T = T;
         c->pg.L = L;
-        int th = (256 / T) * T;
-        if (th < T) th = T;
-        while (th > L) th -= T;
-        c->pg.threads = th;
-        // ~32 samples per thread per block, a multiple of T
-        long long chunk = (long long)th * 32;
-        chunk = (chunk / T) * T;
-        c->pg.chunk = (int)chunk;
-        c->pg.nblk = (int)((cfg->fft_points + chunk - 1) / chunk);
+        pcal_geometry(c->pg, cfg->fft_points);
         std::vector<float2> rot(L);
         const double dphi = 2 * M_PI * (-cfg->pcaloffsethz / cfg->samplingfreq);   // :130-135
         for (int i = 0; i < L; i++) {
@@ -518,7 +510,7 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
                 J.Y = c->d_Y + (size_t)j * c->G * (size_t)p.M;
                 J.acc = c->d_acc + (size_t)j * 2 * M1;
                 if (g.extract_pcal) {
-                    J.pcal_part = c->d_pcal_part + (size_t)j * c->G * c->pg.nblk * c->pg.T * 2;
+                    J.pcal_part = c->d_pcal_part + (size_t)j * c->G * c->pg.nblk_max * c->pg.T * 2;
                     J.pcal_acc = c->d_pcal_acc + (size_t)j * c->pg.T * 2;
                     unsigned long long m = 0;
                     for (int f = 0; f < nf; f++)

@@ -39,6 +39,7 @@ struct SubPlan {
     int nst = 0;             // stages
     int radix[kMaxStages];
     int ns[kMaxStages];      // product of the radices before the stage
+    unsigned mag[kMaxStages];// ceil(2^32 / ns) for q % ns by multiply-high (ns > 1)
     int twoff[kMaxStages];   // offset (in float2) of the stage's twiddles
     int T = 0;               // threads per FFT
     int twlen = 0;           // total twiddles
@@ -48,7 +49,8 @@ struct SubPlan {
 struct Plan {
     long long N = 0, M = 0;
     int N1 = 0, N2 = 0;      // M = N1 * N2; pass 1 = N1-point columns, pass 2 = N2-point rows
-    int B = 0;               // columns per pass-1 workgroup
+    int B = 0;               // columns per pass-1 workgroup (power of two)
+    int lgB = 0;             // log2(B)
     int p1_threads = 0, p2_threads = 0;
     int p1_maxpts = 0, p2_maxpts = 0;
     int npairs = 0;          // pass-2 workgroups per FFT (row pairs)
@@ -91,11 +93,21 @@ struct XpolDev {
 struct PcalGeom {
     int T = 0;        // tone bins
     int L = 0;        // rotator length
-    int nblk = 0;     // blocks per FFT
-    int chunk = 0;    // samples per block
-    int threads = 0;
+    int nblk = 0;     // blocks per FFT (generic kernel)
+    int chunk = 0;    // samples per block (generic kernel)
+    int threads = 0;  // generic kernel
+    int fast = 0;     // 1: position-owning integer-count kernel (k_pcal_fast) usable
+    int nblk_fast = 0;
+    int threads_fast = 0;
+    int ppb = 0;      // rotator periods per block (fast kernel)
+    int gpt = 0;      // groups of 4 positions per thread (fast kernel)
+    int nblk_max = 0; // partial-sum slots per FFT to allocate
     const float2* d_rot = nullptr;  // FP32 rotator (FP64-computed), L entries
 };
+
+// Choose the phase-cal launch geometry for N-point FFTs (fills threads, nblk,
+// chunk, fast, ppb, gpt from T and L).
+void pcal_geometry(PcalGeom& g, long long N);
 
 // Launchers (engine.hip).
 hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, hipStream_t s);

@@ -18,9 +18,10 @@
 
 #include <hip/hip_runtime.h>
 
-#include <cmath>
-#include <cstring>
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
 
 namespace swspec {
 
@@ -122,8 +123,16 @@ template <> struct Dft<8> {
 // Every thread of the workgroup must call it (it contains barriers).
 // ---------------------------------------------------------------------------
 
+// q % ns via a multiply-high with mag = ceil(2^32/ns): exact while q*ns < 2^32
+// (q, ns < 16384 here); ns == 1 is handled by the caller's k = 0.
+__device__ __forceinline__ int mod_magic(int q, int ns, unsigned mag)
+{
+    const int d = (int)__umulhi((unsigned)q, mag);
+    return q - d * ns;
+}
+
 template <int R, int NB>
-__device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int ns,
+__device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int ns, unsigned mag,
                                           const float2* __restrict__ tw, int ebase, int estr,
                                           int q0, int T)
 {
@@ -142,7 +151,7 @@ __device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int n
     for (int i = 0; i < NB; i++) {
         const int q = q0 + i * T;
         if (q < nb) {
-            const int k = q % ns;
+            const int k = (ns > 1) ? mod_magic(q, ns, mag) : 0;
             if (ns > 1) {
                 const float2* w = tw + k * (R - 1);
 #pragma unroll
@@ -163,12 +172,14 @@ __device__ __forceinline__ void fft_lds(float2* lds, const SubPlan& sp, const fl
 {
     for (int s = 0; s < sp.nst; s++) {
         const float2* tw = stw + sp.twoff[s];
+        const int ns = sp.ns[s];
+        const unsigned mag = sp.mag[s];
         switch (sp.radix[s]) {
-        case 2: stage_lds<2, MAXPTS / 2>(lds, sp.L, sp.ns[s], tw, ebase, estr, q0, T); break;
-        case 3: stage_lds<3, MAXPTS / 3>(lds, sp.L, sp.ns[s], tw, ebase, estr, q0, T); break;
-        case 4: stage_lds<4, MAXPTS / 4>(lds, sp.L, sp.ns[s], tw, ebase, estr, q0, T); break;
-        case 5: stage_lds<5, MAXPTS / 5>(lds, sp.L, sp.ns[s], tw, ebase, estr, q0, T); break;
-        default: stage_lds<8, MAXPTS / 8>(lds, sp.L, sp.ns[s], tw, ebase, estr, q0, T); break;
+        case 2: stage_lds<2, MAXPTS / 2>(lds, sp.L, ns, mag, tw, ebase, estr, q0, T); break;
+        case 3: stage_lds<3, MAXPTS / 3>(lds, sp.L, ns, mag, tw, ebase, estr, q0, T); break;
+        case 4: stage_lds<4, MAXPTS / 4>(lds, sp.L, ns, mag, tw, ebase, estr, q0, T); break;
+        case 5: stage_lds<5, MAXPTS / 5>(lds, sp.L, ns, mag, tw, ebase, estr, q0, T); break;
+        default: stage_lds<8, MAXPTS / 8>(lds, sp.L, ns, mag, tw, ebase, estr, q0, T); break;
         }
     }
 }
@@ -198,10 +209,30 @@ __device__ __forceinline__ unsigned field2(const uint8_t* __restrict__ raw, cons
 // pass 1: unpack + window + column FFTs + four-step twiddle
 // ---------------------------------------------------------------------------
 
+// Both samples of complex point n (real samples 2n, 2n+1) from one load where
+// the layout allows it.
+__device__ __forceinline__ void fields_pair(const uint8_t* __restrict__ raw, const Unpack& up, long long n,
+                                            bool even, unsigned& f0, unsigned& f1)
+{
+    if (up.kind == kUnpackSingleChannel) {
+        const unsigned b = raw[n >> 1];
+        const int sh = 6 - 4 * (int)(n & 1);
+        f0 = (b >> sh) & 3u;
+        f1 = (b >> (sh - 2)) & 3u;
+    } else if (up.step == 1 && even) {
+        const unsigned w = *(const uint16_t*)(raw + 2 * n);   // time samples 2n, 2n+1
+        f0 = (w >> up.shift) & 3u;
+        f1 = (w >> (8 + up.shift)) & 3u;
+    } else {
+        f0 = (raw[2 * n * up.step + up.cbyte] >> up.shift) & 3u;
+        f1 = (raw[(2 * n + 1) * up.step + up.cbyte] >> up.shift) & 3u;
+    }
+}
+
 struct P1Args {
     int N1, N2;
     long long M;
-    int B;
+    int B, lgB;
     int twshift;
     SubPlan sp;
     const float2* stw;
@@ -212,37 +243,40 @@ struct P1Args {
 };
 
 template <int MAXPTS>
-__global__ void __launch_bounds__(1024) k_pass1(P1Args a)
+__global__ void __launch_bounds__(512) k_pass1(P1Args a)
 {
     extern __shared__ float2 lds[];
     const JobDev J = a.jobs[blockIdx.z];
     const int f = blockIdx.y;
-    const int B = a.B, N1 = a.N1, N2 = a.N2;
+    const int B = a.B, lgB = a.lgB, N1 = a.N1, N2 = a.N2;
     const int c0 = blockIdx.x * B;
     const int tid = threadIdx.x, nth = blockDim.x;
     const uint8_t* __restrict__ raw = J.raw + (long long)f * J.fft_stride;
     const float* __restrict__ win = a.window;
     const Unpack up = J.up;
+    const bool even = ((uintptr_t)raw & 1) == 0;
 
+    // lds[e*B + b] = z[N2*e + c0 + b] windowed; idx == e*B + b
     for (int idx = tid; idx < N1 * B; idx += nth) {
-        const int e = idx / B, b = idx - e * B;
-        const long long j = 2 * ((long long)N2 * e + c0 + b);
-        const float x0 = map2bit(field2(raw, up, j)) * win[j];
-        const float x1 = map2bit(field2(raw, up, j + 1)) * win[j + 1];
-        lds[e * B + b] = make_float2(x0, x1);
+        const int e = idx >> lgB, b = idx & (B - 1);
+        const long long n = (long long)N2 * e + c0 + b;
+        unsigned f0, f1;
+        fields_pair(raw, up, n, even, f0, f1);
+        const float2 wv = *(const float2*)(win + 2 * n);
+        lds[idx] = make_float2(map2bit(f0) * wv.x, map2bit(f1) * wv.y);
     }
     __syncthreads();
 
-    fft_lds<MAXPTS>(lds, a.sp, a.stw, tid % B, B, tid / B, nth / B);
+    fft_lds<MAXPTS>(lds, a.sp, a.stw, tid & (B - 1), B, tid >> lgB, nth >> lgB);
 
     float2* __restrict__ Y = J.Y + (long long)f * a.M;
     const unsigned mask = (1u << a.twshift) - 1u;
     for (int idx = tid; idx < N1 * B; idx += nth) {
-        const int k1 = idx / B, b = idx - k1 * B;
+        const int k1 = idx >> lgB, b = idx & (B - 1);
         const int n2 = c0 + b;
-        const long long e = ((long long)n2 * k1) % a.M;
+        const unsigned e = (unsigned)n2 * (unsigned)k1;   // < N1*N2 = M: no reduction needed
         const float2 w = cmul(a.twA[e >> a.twshift], a.twB[e & mask]);
-        Y[(long long)k1 * N2 + n2] = cmul(lds[k1 * B + b], w);
+        Y[(long long)k1 * N2 + n2] = cmul(lds[idx], w);
     }
 }
 
@@ -281,8 +315,8 @@ __device__ __forceinline__ void kahan_add(float* __restrict__ p, float v)
     p[1] = c;
 }
 
-template <int MAXPTS, int NSRC, int MAXBINS>
-__global__ void __launch_bounds__(1024) k_pass2(P2Args a)
+template <int MAXPTS, int NSRC>
+__global__ void __launch_bounds__(512) k_pass2(P2Args a)
 {
     extern __shared__ float2 lds[];
     const int p = blockIdx.x, grp = blockIdx.y;
@@ -292,8 +326,11 @@ __global__ void __launch_bounds__(1024) k_pass2(P2Args a)
     const bool self = (rowA == rowB);
     const int tid = threadIdx.x, nth = blockDim.x;
     constexpr int NSL = 2 * NSRC;
-    const int T = nth / NSL;
-    const int slot = tid / T, q0 = tid - slot * T;
+    const int T = a.sp.T;                   // blockDim == NSL * T
+    int slot = 0;
+#pragma unroll
+    for (int s = 1; s < NSL; s++) slot += (tid >= s * T) ? 1 : 0;
+    const int q0 = tid - slot * T;
     float2* Yp[NSRC];
     float* accp[NSRC];
 #pragma unroll
@@ -302,92 +339,88 @@ __global__ void __launch_bounds__(1024) k_pass2(P2Args a)
         accp[s] = a.jobs[grp * NSRC + s].acc;
     }
     const int nbins = (self ? 1 : 2) * N2;
-
-    float accP[NSRC][MAXBINS];
-    float2 accX[MAXBINS];
+    // chunk sums in LDS after the FFT rows: |X|^2 per source, then X0*conj(X1)
+    float* __restrict__ accl = (float*)(lds + NSL * N2);
+    float2* __restrict__ accxl = (float2*)(accl + NSRC * 2 * N2);
+    for (int i = tid; i < NSRC * 2 * N2; i += nth) accl[i] = 0.f;
+    if (NSRC == 2)
+        for (int i = tid; i < 2 * N2; i += nth) accxl[i] = make_float2(0.f, 0.f);
     float nyqP[NSRC];
     float nyqX = 0.f;
-#pragma unroll
-    for (int i = 0; i < MAXBINS; i++) {
-#pragma unroll
-        for (int s = 0; s < NSRC; s++) accP[s][i] = 0.f;
-        accX[i] = make_float2(0.f, 0.f);
-    }
 #pragma unroll
     for (int s = 0; s < NSRC; s++) nyqP[s] = 0.f;
 
     for (int f = 0; f < a.nfft; f++) {
-        for (int idx = tid; idx < NSL * N2; idx += nth) {
-            const int sl = idx / N2, e = idx - sl * N2;
+#pragma unroll
+        for (int sl = 0; sl < NSL; sl++) {
             const int row = (sl & 1) ? rowB : rowA;
-            const float2* Ys = (NSRC == 2 && (sl >> 1)) ? Yp[NSRC - 1] : Yp[0];
-            lds[sl * N2 + e] = Ys[(long long)f * a.M + (long long)row * N2 + e];
+            const float2* __restrict__ Ys = Yp[sl >> 1] + (long long)f * a.M + (long long)row * N2;
+            for (int e = tid; e < N2; e += nth) lds[sl * N2 + e] = Ys[e];
         }
         __syncthreads();
-        fft_lds<MAXPTS>(lds, a.sp, a.stw, slot * N2, 1, q0, T);   // blockDim == NSL * T
+        fft_lds<MAXPTS>(lds, a.sp, a.stw, slot * N2, 1, q0, T);
 
+        // each thread keeps the same bins for every FFT of the chunk
+        for (int idx = tid; idx < nbins; idx += nth) {
+            const int which = idx >= N2 ? 1 : 0, k2 = idx - which * N2;
+            const int row = which ? rowB : rowA;
+            int msl, mk2;
+            if (row == 0) { msl = which; mk2 = (k2 == 0) ? 0 : N2 - k2; }
+            else { msl = self ? which : 1 - which; mk2 = N2 - 1 - k2; }
+            const float2 w = cmul(a.rA[row], a.rB[k2]);
+            float2 X[NSRC];
 #pragma unroll
-        for (int i = 0; i < MAXBINS; i++) {
-            const int idx = tid + i * nth;
-            if (idx < nbins) {
-                const int which = idx / N2, k2 = idx - which * N2;
-                const int row = which ? rowB : rowA;
-                int msl, mk2;
-                if (row == 0) { msl = which; mk2 = (k2 == 0) ? 0 : N2 - k2; }
-                else { msl = self ? which : 1 - which; mk2 = N2 - 1 - k2; }
-                const float2 w = cmul(a.rA[row], a.rB[k2]);
-                float2 X[NSRC];
+            for (int s = 0; s < NSRC; s++) {
+                const float2 Z = lds[(2 * s + which) * N2 + k2];
+                const float2 Zm = lds[(2 * s + msl) * N2 + mk2];
+                X[s] = r2c_bin(Z, Zm, w);
+                accl[s * 2 * N2 + idx] += X[s].x * X[s].x + X[s].y * X[s].y;
+            }
+            if (NSRC == 2) {
+                // X0 * conj(X1)
+                float2 c = accxl[idx];
+                c.x += X[0].x * X[NSRC - 1].x + X[0].y * X[NSRC - 1].y;
+                c.y += X[0].y * X[NSRC - 1].x - X[0].x * X[NSRC - 1].y;
+                accxl[idx] = c;
+            }
+            if (row == 0 && k2 == 0) {
+                float xn[NSRC];
 #pragma unroll
                 for (int s = 0; s < NSRC; s++) {
-                    const float2 Z = lds[(2 * s + which) * N2 + k2];
-                    const float2 Zm = lds[(2 * s + msl) * N2 + mk2];
-                    X[s] = r2c_bin(Z, Zm, w);
-                    accP[s][i] += X[s].x * X[s].x + X[s].y * X[s].y;
+                    const float2 Z = lds[(2 * s) * N2];
+                    xn[s] = Z.x - Z.y;   // X[M] = Re Z0 - Im Z0
+                    nyqP[s] += xn[s] * xn[s];
                 }
-                if (NSRC == 2) {
-                    // X0 * conj(X1)
-                    accX[i].x += X[0].x * X[NSRC - 1].x + X[0].y * X[NSRC - 1].y;
-                    accX[i].y += X[0].y * X[NSRC - 1].x - X[0].x * X[NSRC - 1].y;
-                }
-                if (row == 0 && k2 == 0) {
-                    float xn[NSRC];
-#pragma unroll
-                    for (int s = 0; s < NSRC; s++) {
-                        const float2 Z = lds[(2 * s) * N2];
-                        xn[s] = Z.x - Z.y;   // X[M] = Re Z0 - Im Z0
-                        nyqP[s] += xn[s] * xn[s];
-                    }
-                    if (NSRC == 2) nyqX += xn[0] * xn[NSRC - 1];
-                }
+                if (NSRC == 2) nyqX += xn[0] * xn[NSRC - 1];
             }
         }
         __syncthreads();
     }
 
-#pragma unroll
-    for (int i = 0; i < MAXBINS; i++) {
-        const int idx = tid + i * nth;
-        if (idx < nbins) {
-            const int which = idx / N2, k2 = idx - which * N2;
+    for (int idx = tid; idx < nbins; idx += nth) {
+        {
+            const int which = idx >= N2 ? 1 : 0, k2 = idx - which * N2;
             const int row = which ? rowB : rowA;
             const long long o = (long long)row * N2 + k2;
 #pragma unroll
             for (int s = 0; s < NSRC; s++) {
                 float* acc = accp[s] + 2 * o;
-                if (a.kahan) kahan_add(acc, accP[s][i]);
-                else acc[0] += accP[s][i];
+                const float v = accl[s * 2 * N2 + idx];
+                if (a.kahan) kahan_add(acc, v);
+                else acc[0] += v;
             }
             if (NSRC == 2) {
                 float* xa = a.xp[grp].acc + 4 * o;
+                const float2 v = accxl[idx];
                 if (a.kahan) {
                     float s0 = xa[0], s1 = xa[1], c0 = xa[2], c1 = xa[3];
-                    float y0 = accX[i].x - c0, y1 = accX[i].y - c1;
+                    float y0 = v.x - c0, y1 = v.y - c1;
                     float t0 = s0 + y0, t1 = s1 + y1;
                     xa[2] = (t0 - s0) - y0; xa[3] = (t1 - s1) - y1;
                     xa[0] = t0; xa[1] = t1;
                 } else {
-                    xa[0] += accX[i].x;
-                    xa[1] += accX[i].y;
+                    xa[0] += v.x;
+                    xa[1] += v.y;
                 }
             }
             if (row == 0 && k2 == 0) {
@@ -487,33 +520,119 @@ __global__ void __launch_bounds__(1024) k_pcal(PcArgs a)
     if (tid < T) {
         double sr = 0.0, si = 0.0;
         for (int k = tid; k < nth; k += T) { sr += red[k].x; si += red[k].y; }
-        double* part = J.pcal_part + 2 * (((long long)f * a.g.nblk + blk) * T + tid);
+        double* part = J.pcal_part + 2 * (((long long)f * a.g.nblk_max + blk) * T + tid);
+        part[0] = sr;
+        part[1] = si;
+    }
+}
+
+// Fast fold.  Thread owns groups of 4 consecutive rotator positions
+// r = 4g + u (g = tid + i*nth, i < GPT); a block covers `ppb` rotator periods
+// of one FFT.  Across periods the rotator value of a position is fixed, so the
+// per-sample work is exact integer counting of the two 2-bit magnitudes:
+//   sum_p x[pL + r] = A_r * 1 + B_r * 3.3359f   (A_r, B_r signed counts)
+// and the rotator is applied once per position in FP64.  With 4*nth % T == 0
+// every group of a thread folds into the same four bins, reduced in LDS in a
+// fixed order.  STEP = bytes per time sample (multi-channel), 0 = 1-channel.
+template <int STEP, int GPT>
+__global__ void __launch_bounds__(1024) k_pcal_fast(PcArgs a)
+{
+    extern __shared__ double2 red4[];   // [nth][4]
+    const JobDev J = a.jobs[blockIdx.z];
+    const int f = blockIdx.y, blk = blockIdx.x;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int T = a.g.T, L = a.g.L;
+    const int ngroups = L >> 2;
+    double accr[4] = {0.0, 0.0, 0.0, 0.0}, acci[4] = {0.0, 0.0, 0.0, 0.0};
+    if ((J.pcal_mask >> f) & 1ull) {
+        const uint8_t* __restrict__ raw = J.raw + (long long)f * J.fft_stride;
+        const int nper = (int)(a.N / L);
+        const int p0 = blk * a.g.ppb;
+        const int p1 = min(p0 + a.g.ppb, nper);
+        const int cbyte = J.up.cbyte, shift = J.up.shift;
+#pragma unroll
+        for (int gi = 0; gi < GPT; gi++) {
+            const int g = tid + gi * nth;
+            if (g >= ngroups) break;
+            int A[4] = {0, 0, 0, 0}, Bc[4] = {0, 0, 0, 0};
+            for (int p = p0; p < p1; p++) {
+                const long long j0 = (long long)p * L + 4 * g;
+                unsigned fld[4];
+                if (STEP == 0) {
+                    const unsigned b = raw[j0 >> 2];   // 4 samples, oldest in the MSB
+#pragma unroll
+                    for (int u = 0; u < 4; u++) fld[u] = (b >> (6 - 2 * u)) & 3u;
+                } else {
+                    const uint32_t* w = (const uint32_t*)(raw + j0 * STEP);
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int bi = u * STEP + cbyte;
+                        fld[u] = (w[(u * STEP) >> 2] >> (8 * (bi & 3) + shift)) & 3u;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int s = 1 - 2 * (int)(fld[u] & 1u);
+                    const int big = (int)(fld[u] >> 1);
+                    A[u] += s * (1 - big);
+                    Bc[u] += s * big;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const float2 r = a.g.d_rot[4 * g + u];
+                const double v = (double)A[u] + (double)3.3359f * (double)Bc[u];
+                accr[u] += v * (double)r.x;
+                acci[u] += v * (double)r.y;
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) red4[4 * tid + u] = make_double2(accr[u], acci[u]);
+    __syncthreads();
+    if (tid < T) {
+        // bin t collects threads h with (4h mod T) == t - t%4, column t%4
+        const int q = T >> 2, u = tid & 3;
+        double sr = 0.0, si = 0.0;
+        for (int h = tid >> 2; h < nth; h += q) { sr += red4[4 * h + u].x; si += red4[4 * h + u].y; }
+        double* part = J.pcal_part + 2 * (((long long)f * a.g.nblk_max + blk) * T + tid);
         part[0] = sr;
         part[1] = si;
     }
 }
 
 struct PcRedArgs {
-    int T, nblk, nfft;
+    int T, nblk, stride, nfft;
     JobDev jobs[kMaxJobs];
 };
 
-__global__ void k_pcal_reduce(PcRedArgs a)
+// Sum the per-block partials of a chunk into the job's accumulator.  blockDim
+// = lanes * T: lane l of bin t sums entries l, l+lanes, ... of the (fft, block)
+// list, then lane 0 adds the lanes in order (fixed order, deterministic).
+__global__ void __launch_bounds__(1024) k_pcal_reduce(PcRedArgs a)
 {
+    __shared__ double2 red[1024];
     const JobDev J = a.jobs[blockIdx.x];
-    const int t = threadIdx.x;
-    if (t >= a.T) return;
+    const int T = a.T;
+    const int tid = threadIdx.x, lanes = blockDim.x / T;
+    const int t = tid % T, l = tid / T;
     double sr = 0.0, si = 0.0;
-    for (int f = 0; f < a.nfft; f++) {
-        if (!((J.pcal_mask >> f) & 1ull)) continue;
-        for (int b = 0; b < a.nblk; b++) {
-            const double* part = J.pcal_part + 2 * (((long long)f * a.nblk + b) * a.T + t);
-            sr += part[0];
-            si += part[1];
+    const int nent = a.nfft * a.nblk;
+    for (int e = l; e < nent; e += lanes) {
+        const int f = e / a.nblk, b = e - f * a.nblk;
+        if ((J.pcal_mask >> f) & 1ull) {
+            const double2 v = *(const double2*)(J.pcal_part + 2 * (((long long)f * a.stride + b) * T + t));
+            sr += v.x;
+            si += v.y;
         }
     }
-    J.pcal_acc[2 * t] += sr;
-    J.pcal_acc[2 * t + 1] += si;
+    red[tid] = make_double2(sr, si);
+    __syncthreads();
+    if (l == 0) {
+        for (int k = 1; k < lanes; k++) { sr += red[k * T + t].x; si += red[k * T + t].y; }
+        J.pcal_acc[2 * t] += sr;
+        J.pcal_acc[2 * t + 1] += si;
+    }
 }
 
 // Folded vector out (FP32, as the reference's pcal sink carries) and tones in
@@ -563,13 +682,18 @@ __global__ void k_unpack(const uint8_t* __restrict__ raw, Unpack up, long long c
 // ---------------------------------------------------------------------------
 
 static size_t p1_lds(const Plan& p) { return (size_t)p.N1 * p.B * sizeof(float2); }
-static size_t p2_lds(const Plan& p, int nsrc) { return (size_t)2 * nsrc * p.N2 * sizeof(float2); }
+// FFT rows + |X|^2 chunk sums (+ cross-power chunk sums)
+static size_t p2_lds(const Plan& p, int nsrc)
+{
+    return (size_t)2 * nsrc * p.N2 * sizeof(float2) + (size_t)nsrc * 2 * p.N2 * sizeof(float) +
+           (nsrc == 2 ? (size_t)2 * p.N2 * sizeof(float2) : 0);
+}
 
 hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, hipStream_t s)
 {
     if (njobs < 1 || njobs > kMaxJobs || nfft < 1) return hipErrorInvalidValue;
     P1Args a;
-    a.N1 = p.N1; a.N2 = p.N2; a.M = p.M; a.B = p.B; a.twshift = p.twshift;
+    a.N1 = p.N1; a.N2 = p.N2; a.M = p.M; a.B = p.B; a.lgB = p.lgB; a.twshift = p.twshift;
     a.sp = p.sp1; a.stw = p.d_stw1; a.twA = p.d_twA; a.twB = p.d_twB; a.window = p.d_window;
     for (int j = 0; j < njobs; j++) a.jobs[j] = jobs[j];
     dim3 grid(p.N2 / p.B, nfft, njobs);
@@ -595,18 +719,17 @@ hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const Xpol
     const int threads = nsrc * 2 * p.sp2.T;
     dim3 grid(p.npairs, njobs / nsrc);
     const size_t lds = p2_lds(p, nsrc);
-    const int bins = (2 * p.N2 + threads - 1) / threads;
-    if (bins > 16) return hipErrorInvalidValue;
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
     if (nsrc == 1) {
         switch (p.p2_maxpts) {
-        case 16: hipLaunchKernelGGL((k_pass2<16, 1, 16>), grid, dim3(threads), lds, s, a); break;
-        case 32: hipLaunchKernelGGL((k_pass2<32, 1, 16>), grid, dim3(threads), lds, s, a); break;
+        case 16: hipLaunchKernelGGL((k_pass2<16, 1>), grid, dim3(threads), lds, s, a); break;
+        case 32: hipLaunchKernelGGL((k_pass2<32, 1>), grid, dim3(threads), lds, s, a); break;
         default: return hipErrorInvalidValue;
         }
     } else {
         switch (p.p2_maxpts) {
-        case 16: hipLaunchKernelGGL((k_pass2<16, 2, 16>), grid, dim3(threads), lds, s, a); break;
-        case 32: hipLaunchKernelGGL((k_pass2<32, 2, 16>), grid, dim3(threads), lds, s, a); break;
+        case 16: hipLaunchKernelGGL((k_pass2<16, 2>), grid, dim3(threads), lds, s, a); break;
+        case 32: hipLaunchKernelGGL((k_pass2<32, 2>), grid, dim3(threads), lds, s, a); break;
         default: return hipErrorInvalidValue;
         }
     }
@@ -641,19 +764,80 @@ hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s)
     return hipGetLastError();
 }
 
+void pcal_geometry(PcalGeom& g, long long N)
+{
+    const int T = g.T, L = g.L;
+    // generic kernel: ~32 samples per thread per block, thread count a multiple of T
+    int th = (256 / T) * T;
+    if (th < T) th = T;
+    while (th > L) th -= T;
+    g.threads = th;
+    long long chunk = (long long)th * 32;
+    chunk = (chunk / T) * T;
+    g.chunk = (int)chunk;
+    g.nblk = (int)((N + chunk - 1) / chunk);
+    g.fast = 0;
+    // fast kernel: groups of 4 positions, 4*nth % T == 0, <= 4 groups per thread
+    if ((L % 4) == 0 && (T % 4) == 0 && (N % L) == 0) {
+        const int ngroups = L / 4, q = T / 4;
+        int nth = std::min(ngroups, 1024);
+        nth -= nth % q;
+        if (nth >= q && nth >= 64) {
+            const int gpt = (ngroups + nth - 1) / nth;
+            if (gpt <= 4) {
+                const long long nper = N / L;
+                g.fast = 1;
+                g.threads_fast = nth;
+                g.gpt = gpt;
+                const int nb = (int)std::min<long long>(nper, 50);
+                g.ppb = (int)((nper + nb - 1) / nb);
+                g.nblk_fast = (int)((nper + g.ppb - 1) / g.ppb);
+            }
+        }
+    }
+    g.nblk_max = std::max(g.nblk, g.nblk_fast);
+}
+
 hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int njobs, int nfft, hipStream_t s)
 {
     if (njobs < 1 || njobs > kMaxJobs || nfft < 1 || nfft > 64) return hipErrorInvalidValue;
     PcArgs a;
     a.g = g; a.N = N;
-    for (int j = 0; j < njobs; j++) a.jobs[j] = jobs[j];
-    hipLaunchKernelGGL(k_pcal, dim3(g.nblk, nfft, njobs), dim3(g.threads), 0, s, a);
+    bool aligned = true;
+    int step = -1;
+    for (int j = 0; j < njobs; j++) {
+        a.jobs[j] = jobs[j];
+        aligned &= ((uintptr_t)jobs[j].raw % 4 == 0) && (jobs[j].fft_stride % 4 == 0);
+        const int st = jobs[j].up.kind == kUnpackSingleChannel ? 0 : jobs[j].up.step;
+        if (step < 0) step = st;
+        else if (step != st) step = -2;
+    }
+    const bool fast = g.fast && aligned && (step == 0 || step == 1 || step == 2 || step == 4);
+    if (fast) {
+        const dim3 grid(g.nblk_fast, nfft, njobs), block(g.threads_fast);
+        const size_t lds = (size_t)g.threads_fast * 4 * sizeof(double2);
+#define SWSPEC_PCAL_FAST(ST)                                                                            \
+    switch (g.gpt) {                                                                                  \
+    case 1: hipLaunchKernelGGL((k_pcal_fast<ST, 1>), grid, block, lds, s, a); break;                   \
+    case 2: hipLaunchKernelGGL((k_pcal_fast<ST, 2>), grid, block, lds, s, a); break;                   \
+    default: hipLaunchKernelGGL((k_pcal_fast<ST, 4>), grid, block, lds, s, a); break;                  \
+    }
+        switch (step) {
+        case 0: SWSPEC_PCAL_FAST(0) break;
+        case 1: SWSPEC_PCAL_FAST(1) break;
+        case 2: SWSPEC_PCAL_FAST(2) break;
+        default: SWSPEC_PCAL_FAST(4) break;
+        }
+#undef SWSPEC_PCAL_FAST
+    } else {
+        hipLaunchKernelGGL(k_pcal, dim3(g.nblk, nfft, njobs), dim3(g.threads), 0, s, a);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     PcRedArgs r;
-    r.T = g.T; r.nblk = g.nblk; r.nfft = nfft;
+    r.T = g.T; r.nblk = fast ? g.nblk_fast : g.nblk; r.stride = g.nblk_max; r.nfft = nfft;
     for (int j = 0; j < njobs; j++) r.jobs[j] = jobs[j];
-    const int th = ((g.T + 63) / 64) * 64;
+    const int th = g.T <= 256 ? (256 / g.T) * g.T : g.T;   // lanes * T, <= 1024
     hipLaunchKernelGGL(k_pcal_reduce, dim3(njobs), dim3(th), 0, s, r);
     return hipGetLastError();
 }
@@ -697,6 +881,7 @@ static bool subplan_radices(int L, SubPlan& sp)
     auto push = [&](int R) {
         sp.radix[sp.nst] = R;
         sp.ns[sp.nst] = ns;
+        sp.mag[sp.nst] = ns > 1 ? (unsigned)(((1ULL << 32) + (unsigned long long)ns - 1) / (unsigned long long)ns) : 0u;
         sp.twoff[sp.nst] = twl;
         twl += ns * (R - 1);
         ns *= R;
@@ -746,28 +931,47 @@ std::string plan_geometry(Plan& p, long long N)
     if (p.N2 > 16384) return "FFT length too large for two passes";
     if (!subplan_radices(p.N1, p.sp1) || !subplan_radices(p.N2, p.sp2)) return "radix decomposition failed";
 
-    // pass 1: B columns of N1 points, LDS budget 80 KB, <= 1024 threads
-    p.p1_maxpts = 16;
+    // pass 1: B columns of N1 points (B a power of two dividing N2), LDS budget
+    // 80 KB, <= 512 threads; more points per thread when that buys columns
+    auto pick_B = [&](int T1) {
+        for (int c : {16, 8, 4, 2, 1})
+            if ((p.N2 % c) == 0 && (size_t)c * p.N1 * sizeof(float2) <= 80 * 1024 && c * T1 <= 512) return c;
+        return 0;
+    };
     int T1 = subplan_threads(p.sp1, 16);
-    if (T1 > 512) { p.p1_maxpts = 32; T1 = subplan_threads(p.sp1, 32); }
-    if (T1 > 1024) return "column FFT too large";
-    p.sp1.T = T1;
-    int B = 1;
-    for (int c : {16, 8, 4, 2, 1}) {
-        if ((p.N2 % c) == 0 && (size_t)c * p.N1 * sizeof(float2) <= 80 * 1024 && c * T1 <= 1024) {
-            B = c;
-            break;
-        }
+    int B = pick_B(T1);
+    p.p1_maxpts = 16;
+    {
+        const int T1b = subplan_threads(p.sp1, 32);
+        const int Bb = pick_B(T1b);
+        if (Bb > B && B < 8) { B = Bb; T1 = T1b; p.p1_maxpts = 32; }
     }
+    // tuning overrides: SWSPEC_P1_MAXPTS (16|32), SWSPEC_P1_BMAX (upper bound on B)
+    if (const char* e = getenv("SWSPEC_P1_MAXPTS")) {
+        const int mp = atoi(e);
+        if (mp == 16 || mp == 32) { p.p1_maxpts = mp; T1 = subplan_threads(p.sp1, mp); B = pick_B(T1); }
+    }
+    if (const char* e = getenv("SWSPEC_P1_BMAX")) {
+        const int bm = atoi(e);
+        while (B > 1 && B > bm) B >>= 1;
+    }
+    if (B == 0) return "column FFT too large";
+    p.sp1.T = T1;
     p.B = B;
+    p.lgB = 0;
+    while ((1 << p.lgB) < B) p.lgB++;
     p.p1_threads = B * T1;
 
-    // pass 2: 2 rows (x2 for cross-power pairs) of N2 points, <= 16 bins/thread
+    // pass 2: 2 rows (x2 for cross-power pairs) of N2 points, <= 16 bins/thread,
+    // <= 512 threads with cross-power
     p.p2_maxpts = 16;
-    int T2 = subplan_threads(p.sp2, 16);
-    while (2 * p.N2 > 16 * 2 * T2) T2++;
-    if (4 * T2 > 1024) { p.p2_maxpts = 32; T2 = subplan_threads(p.sp2, 32); while (2 * p.N2 > 16 * 2 * T2) T2++; }
-    if (4 * T2 > 1024) return "row FFT too large";
+    if (const char* e = getenv("SWSPEC_P2_MAXPTS")) {
+        const int mp = atoi(e);
+        if (mp == 16 || mp == 32) p.p2_maxpts = mp;
+    }
+    int T2 = subplan_threads(p.sp2, p.p2_maxpts);
+    if (4 * T2 > 512) { p.p2_maxpts = 32; T2 = subplan_threads(p.sp2, 32); }
+    if (4 * T2 > 512) return "row FFT too large";
     p.sp2.T = T2;
     p.p2_threads = 2 * T2;
     p.npairs = p.N1 / 2 + 1;

@@ -103,6 +103,44 @@ def test_multichannel_mk5b_and_pcal(hip, oracle):
     core.finalize()
 
 
+@pytest.mark.parametrize("fmt,nch,incr,bw,n", [
+    ("RawSigned", 1, 1, 16e6, 32000),     # fast fold, 4 samples per byte
+    ("RawSigned", 8, 0, 16e6, 6400),      # fast fold, 2 bytes per time sample
+    ("Mk5B", 8, 1, 32e6, 12800),          # 64 Ms/s: 64 tone bins, 6400-sample rotator
+    ("RawSigned", 4, 1, 9e6, 18000),      # 18 tone bins: generic fold kernel
+])
+def test_pcal_formats(hip, oracle, fmt, nch, incr, bw, n):
+    say("pcal", fmt, nch, incr, bw, n)
+    fs = 2 * bw
+    nf = 3
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=1, WindowType="Hann",
+                            BandwidthHz=bw, BitsPerSample=2, SourceChannels=nch, SourceFormat=fmt,
+                            ChannelOrderIncreasing=bool(incr), ExtractPCal=True, MaxSourceBufferMB=4096,
+                            UseFile1Channel=nch)
+    assert s.extract_pcal == 1
+    rng = np.random.default_rng(int(bw) % 1000 + n)
+    comb = synth.pcal_comb(fs, bw, amp=0.2, rng=rng)
+    if nch == 1:
+        raw = synth.raw_1ch(n, n * nf, tones=comb, fs=fs)
+    else:
+        raw = synth.raw_multi(n, nch, n * nf, tones_per_ch=[comb] * nch, fs=fs, increasing=bool(incr))
+    m, auto, _, pc, core = run_device(s, [raw])
+    os_ = oracle_settings(oracle, s)
+    _, a64, _, p64 = oracle.domaths(os_, [raw], 64)
+    x = oracle.unpack(os_, raw, n * nf, nch - 1).astype(np.float64)
+    acc = np.zeros(2 * s.pcal_tonebins)
+    for f in range(nf):
+        acc = oracle.pcal_fold64(x[f * n:(f + 1) * n], fs, 10e3, s.pcal_tonebins, s.pcal_rotatorlen, rot32=True,
+                                 acc=acc)
+    ep = relerr(pc[0][0], acc[0::2] + 1j * acc[1::2])
+    tg = core.pcal_tones(0, 0)
+    tr = oracle.pcal_tones64(acc, len(tg), 1e6, fs)
+    say("  T=%d L=%d pcal %.3e vs-fp64 %.3e tones %.3e spectrum %.3e" % (
+        s.pcal_tonebins, s.pcal_rotatorlen, ep, relerr(pc[0][0], p64[0][0]), relerr(tg, tr), relerr(auto[0], a64[0])))
+    assert m == 1 and ep <= 1e-6 and relerr(tg, tr) <= 1e-6 and relerr(auto[0], a64[0]) <= TOL
+    core.finalize()
+
+
 def test_xpol_vs_oracle(hip, oracle):
     n, nf, fs = 1 << 16, 4, 64e6
     say("xpol", n, nf)
