@@ -45,6 +45,23 @@ struct SubPlan {
     int twlen = 0;           // total twiddles
 };
 
+// Window generated in the column pass: for real sample j = 2(N2 e + c) + r,
+//   theta_j = 2 pi (j + delta) / D = phi_e + psi_{c,r}
+//   w_j = a0 + a1 cos(theta) + a2 sin(theta) + a3 cos(2 theta)
+// phi_e comes from a per-row table (FP64-computed, FP32-stored); psi is
+// computed once per thread in FP64.  The reference windows
+// (TaskCoreIPP.cpp:569-605) in this form:
+//   None: kind 0;  Cosine: a2 = 1, D = 2N, delta = 1/2;
+//   Cosine2: a0 = 1/2, a1 = -1/2, D = N, delta = 1/2;
+//   Hamming: 0.54, -0.46, D = N-1;  Hann: 1/2, -1/2, D = N-1;
+//   Blackman(alpha = 0.5): 0.75, -0.5, a3 = -0.25, D = N-1.
+struct WindowGen {
+    int kind = 0;            // 0: rectangular
+    float a0 = 1.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    double D = 1.0, delta = 0.0;
+    const float2* phi = nullptr;
+};
+
 // Two-pass four-step real FFT plan for N real points (M = N/2 complex).
 struct Plan {
     long long N = 0, M = 0;
@@ -64,7 +81,8 @@ struct Plan {
     float2* d_twB = nullptr;
     float2* d_rA = nullptr;     // W_N^{k1}, k1 < N1
     float2* d_rB = nullptr;     // W_N^{N1 k2}, k2 < N2
-    float* d_window = nullptr;  // N window values
+    float2* d_winphi = nullptr; // N1 window row phases (cos, sin)
+    WindowGen win;
 };
 
 // Build a plan (host tables in FP64, rounded to FP32 once).  Returns an

@@ -238,36 +238,75 @@ struct P1Args {
     const float2* stw;
     const float2* twA;
     const float2* twB;
-    const float* window;
+    WindowGen win;
     JobDev jobs[kMaxJobs];
 };
+
+// Workgroup id -> (x, y, z) such that consecutive x land on the same XCD.
+// Workgroups are dispatched round-robin over the 8 XCDs by linear id; when the
+// grid size is a multiple of 8, linear id L runs on XCD L % 8, so logical id
+// (L % 8) * (total / 8) + L / 8 keeps neighbouring column blocks (which share
+// cache lines of the input and of Y) in one L2.
+__device__ __forceinline__ void xcd_remap(unsigned& x, unsigned& y, unsigned& z)
+{
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned total = gx * gy * gridDim.z;
+    unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if ((total & 7u) == 0) L = (L & 7u) * (total >> 3) + (L >> 3);
+    x = L % gx;
+    const unsigned yz = L / gx;
+    y = yz % gy;
+    z = yz / gy;
+}
 
 template <int MAXPTS>
 __global__ void __launch_bounds__(512) k_pass1(P1Args a)
 {
     extern __shared__ float2 lds[];
-    const JobDev J = a.jobs[blockIdx.z];
-    const int f = blockIdx.y;
+    unsigned bx, by, bz;
+    xcd_remap(bx, by, bz);
+    const JobDev J = a.jobs[bz];
+    const int f = (int)by;
     const int B = a.B, lgB = a.lgB, N1 = a.N1, N2 = a.N2;
-    const int c0 = blockIdx.x * B;
+    const int c0 = (int)bx * B;
     const int tid = threadIdx.x, nth = blockDim.x;
     const uint8_t* __restrict__ raw = J.raw + (long long)f * J.fft_stride;
-    const float* __restrict__ win = a.window;
     const Unpack up = J.up;
     const bool even = ((uintptr_t)raw & 1) == 0;
 
+    // Window on the fly: sample j = 2(N2 e + c) + r has angle phi_e + psi_{c,r}
+    // with phi_e from a per-row table and psi fixed per thread (its column c
+    // never changes: nth is a multiple of B).  w = c0 + c1 cos + c2 sin + c3 cos2.
+    const WindowGen wg = a.win;
+    const int bcol = tid & (B - 1);
+    float2 psi0 = make_float2(1.f, 0.f), psi1 = make_float2(1.f, 0.f);
+    if (wg.kind != 0) {
+        double s, c;
+        sincospi(2.0 * ((double)(2 * (c0 + bcol)) + wg.delta) / wg.D, &s, &c);
+        psi0 = make_float2((float)c, (float)s);
+        sincospi(2.0 * ((double)(2 * (c0 + bcol) + 1) + wg.delta) / wg.D, &s, &c);
+        psi1 = make_float2((float)c, (float)s);
+    }
+
     // lds[e*B + b] = z[N2*e + c0 + b] windowed; idx == e*B + b
     for (int idx = tid; idx < N1 * B; idx += nth) {
-        const int e = idx >> lgB, b = idx & (B - 1);
-        const long long n = (long long)N2 * e + c0 + b;
+        const int e = idx >> lgB;
+        const long long n = (long long)N2 * e + c0 + bcol;
         unsigned f0, f1;
         fields_pair(raw, up, n, even, f0, f1);
-        const float2 wv = *(const float2*)(win + 2 * n);
-        lds[idx] = make_float2(map2bit(f0) * wv.x, map2bit(f1) * wv.y);
+        float w0 = 1.f, w1 = 1.f;
+        if (wg.kind != 0) {
+            const float2 ph = wg.phi[e];
+            const float cs0 = ph.x * psi0.x - ph.y * psi0.y, sn0 = ph.y * psi0.x + ph.x * psi0.y;
+            const float cs1 = ph.x * psi1.x - ph.y * psi1.y, sn1 = ph.y * psi1.x + ph.x * psi1.y;
+            w0 = wg.a0 + wg.a1 * cs0 + wg.a2 * sn0 + wg.a3 * (2.f * cs0 * cs0 - 1.f);
+            w1 = wg.a0 + wg.a1 * cs1 + wg.a2 * sn1 + wg.a3 * (2.f * cs1 * cs1 - 1.f);
+        }
+        lds[idx] = make_float2(map2bit(f0) * w0, map2bit(f1) * w1);
     }
     __syncthreads();
 
-    fft_lds<MAXPTS>(lds, a.sp, a.stw, tid & (B - 1), B, tid >> lgB, nth >> lgB);
+    fft_lds<MAXPTS>(lds, a.sp, a.stw, bcol, B, tid >> lgB, nth >> lgB);
 
     float2* __restrict__ Y = J.Y + (long long)f * a.M;
     const unsigned mask = (1u << a.twshift) - 1u;
@@ -694,7 +733,7 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
     if (njobs < 1 || njobs > kMaxJobs || nfft < 1) return hipErrorInvalidValue;
     P1Args a;
     a.N1 = p.N1; a.N2 = p.N2; a.M = p.M; a.B = p.B; a.lgB = p.lgB; a.twshift = p.twshift;
-    a.sp = p.sp1; a.stw = p.d_stw1; a.twA = p.d_twA; a.twB = p.d_twB; a.window = p.d_window;
+    a.sp = p.sp1; a.stw = p.d_stw1; a.twA = p.d_twA; a.twB = p.d_twB; a.win = p.win;
     for (int j = 0; j < njobs; j++) a.jobs[j] = jobs[j];
     dim3 grid(p.N2 / p.B, nfft, njobs);
     const size_t lds = p1_lds(p);
@@ -944,7 +983,8 @@ std::string plan_geometry(Plan& p, long long N)
     {
         const int T1b = subplan_threads(p.sp1, 32);
         const int Bb = pick_B(T1b);
-        if (Bb > B && B < 8) { B = Bb; T1 = T1b; p.p1_maxpts = 32; }
+        // 32 points/thread halves occupancy (~230 VGPRs): only when 16 leaves B < 4
+        if (Bb > B && B < 4) { B = Bb; T1 = T1b; p.p1_maxpts = 32; }
     }
     // tuning overrides: SWSPEC_P1_MAXPTS (16|32), SWSPEC_P1_BMAX (upper bound on B)
     if (const char* e = getenv("SWSPEC_P1_MAXPTS")) {
@@ -1001,26 +1041,24 @@ static float2 wroot(long long e, long long D)
     return make_float2((float)cos(a), (float)sin(a));
 }
 
-// Window values as the reference computes them (TaskCoreIPP.cpp:569-605);
-// the IPP windows follow Intel's published symmetric formulas.
-static double window_value(int type, long long i, long long n)
+// The reference's windows (TaskCoreIPP.cpp:569-605) as a cosine series in
+// theta = 2 pi (j + delta) / D.  Cosine: cos(pi (j - N/2 + 1/2)/N) =
+// sin(pi (j + 1/2)/N); Cosine2 is its square = (1 - cos(2 pi (j + 1/2)/N))/2;
+// Hamming, Hann and Blackman(alpha = 0.5) follow Intel's published symmetric
+// formulas (0.54 - 0.46 cos, 0.5 - 0.5 cos, (a+1)/2 - cos/2 - (a/2) cos 2x over
+// 2 pi n / (N - 1)).
+static WindowGen window_gen(int type, long long n)
 {
-    const double n1 = (double)(n - 1);
+    WindowGen g;
     switch (type) {
-    case 0: return 1.0;
-    case 1: return cos(M_PI * ((double)(float)i - 0.5 * (double)(float)n + 0.5) / (double)(float)n);
-    case 2: {
-        const double w = cos(M_PI * ((double)(float)i - 0.5 * (double)(float)n + 0.5) / (double)(float)n);
-        return w * w;
+    case 1: g.kind = 1; g.a0 = 0.f; g.a2 = 1.f; g.D = 2.0 * n; g.delta = 0.5; break;
+    case 2: g.kind = 1; g.a0 = 0.5f; g.a1 = -0.5f; g.D = (double)n; g.delta = 0.5; break;
+    case 3: g.kind = 1; g.a0 = 0.54f; g.a1 = -0.46f; g.D = (double)(n - 1); break;
+    case 4: g.kind = 1; g.a0 = 0.5f; g.a1 = -0.5f; g.D = (double)(n - 1); break;
+    case 5: g.kind = 1; g.a0 = 0.75f; g.a1 = -0.5f; g.a3 = -0.25f; g.D = (double)(n - 1); break;
+    default: g.kind = 0; break;
     }
-    case 3: return 0.54 - 0.46 * cos(2.0 * M_PI * (double)i / n1);
-    case 4: return 0.5 - 0.5 * cos(2.0 * M_PI * (double)i / n1);
-    case 5: {
-        const double al = 0.5, x = 2.0 * M_PI * (double)i / n1;
-        return (al + 1.0) / 2.0 - 0.5 * cos(x) - (al / 2.0) * cos(2.0 * x);
-    }
-    default: return 1.0;
-    }
+    return g;
 }
 
 template <typename T>
@@ -1060,9 +1098,16 @@ std::string plan_create(Plan& p, long long N, int window, hipStream_t stream)
             return "device allocation failed (split twiddles)";
     }
     {
-        std::vector<float> w(p.N);
-        for (long long i = 0; i < p.N; i++) w[i] = (float)window_value(window, i, p.N);
-        if (upload(&p.d_window, w, stream) != hipSuccess) return "device allocation failed (window)";
+        // per-row window phase phi_e = 2 pi (2 N2 e mod D) / D, in FP64
+        p.win = window_gen(window, p.N);
+        std::vector<float2> ph(p.N1);
+        for (int e = 0; e < p.N1; e++) {
+            const double r = fmod(2.0 * (double)p.N2 * (double)e, p.win.D);
+            const double a = 2.0 * M_PI * r / p.win.D;
+            ph[e] = make_float2((float)cos(a), (float)sin(a));
+        }
+        if (upload(&p.d_winphi, ph, stream) != hipSuccess) return "device allocation failed (window)";
+        p.win.phi = p.d_winphi;
     }
     return "";
 }
@@ -1070,10 +1115,10 @@ std::string plan_create(Plan& p, long long N, int window, hipStream_t stream)
 void plan_destroy(Plan& p)
 {
     for (void* q : {(void*)p.d_stw1, (void*)p.d_stw2, (void*)p.d_twA, (void*)p.d_twB, (void*)p.d_rA,
-                    (void*)p.d_rB, (void*)p.d_window})
+                    (void*)p.d_rB, (void*)p.d_winphi})
         if (q) (void)hipFree(q);
     p.d_stw1 = p.d_stw2 = p.d_twA = p.d_twB = p.d_rA = p.d_rB = nullptr;
-    p.d_window = nullptr;
+    p.d_winphi = nullptr;
 }
 
 }  // namespace swspec
