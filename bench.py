@@ -123,6 +123,107 @@ def cpu_baseline(raw_host, threads, ffts_per_thread=12):
                                                                            samples / 1e6, dt)}
 
 
+def plain_cfg2(torch, swspec_amd, raw, s, reps=2):
+    """Per-job plain launches: one single-channel core per channel, one after
+    the other (the reference's one-run-per-UseFile1Channel pattern)."""
+    from swspec_amd import TaskCoreHIP
+    dev = torch.device("cuda:0")
+    cores, outs = [], []
+    for ch in range(NCH):
+        sc = swspec_amd.derive_settings(**dict(s.keys, UseFile1Channel=ch + 1))
+        c = TaskCoreHIP(ch)
+        c.prepare(sc)
+        cores.append(c)
+        outs.append((torch.empty(s.fft_ssb_points, device=dev), torch.empty(2 * s.pcal_tonebins, device=dev)))
+    for c, (o, p) in zip(cores, outs):   # warm
+        c.run_device([raw], [SAMPLES], [o], None, [p])
+        c.join()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for c, (o, p) in zip(cores, outs):
+            c.run_device([raw], [SAMPLES], [o], None, [p])
+            c.join()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    for c in cores:
+        c.finalize()
+    return NCH * SAMPLES / dt / 1e9, dt * 1e3
+
+
+def batch_cfg4(torch, replays=3, seed=4):
+    """64 streams of mixed FFT sizes: one hipGraph replay of the whole batch
+    vs plain per-stream launches (each stream alone, step by step)."""
+    from swspec_amd import workloads as W
+    from swspec_amd.server import Server
+    streams = W.batch_workload(seed=seed, nstreams=64)
+    pool = W.RingPool(torch, ring_ffts=8)
+
+    def graph_run(nrep):
+        srv = Server(max_active=1024)
+        _, tot = W.submit_batch(torch, srv, streams, pool)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        srv.run_batch(use_graph=True, replays=nrep)   # capture + instantiate + nrep replays
+        dt = time.perf_counter() - t0
+        srv.close()
+        return dt, tot
+
+    graph_run(1)                         # warm (plans, pools)
+    t1, total = graph_run(1)
+    tn, _ = graph_run(1 + replays)
+    dt_graph = max(tn - t1, 1e-9) / replays   # per replay; capture cost cancels
+    capture_s = max(t1 - dt_graph, 0.0)
+    # plain: one stream at a time
+    srv = Server(max_active=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for st in streams:
+        W.submit_batch(torch, srv, [st], pool)
+        srv.run_batch(use_graph=False)
+    dt_plain = time.perf_counter() - t0
+    srv.close()
+    # graph timing includes one capture + instantiate; report per replay from the difference
+    return {"streams": len(streams), "samples": total, "graph_replays": replays,
+            "batched_gsps": round(total / dt_graph / 1e9, 3),
+            "plain_gsps": round(total / dt_plain / 1e9, 3),
+            "batched_s_per_replay": round(dt_graph, 4), "graph_capture_instantiate_s": round(capture_s, 3),
+            "plain_wall_s": round(dt_plain, 3),
+            "speedup_vs_plain": round(dt_plain / dt_graph, 2),
+            "sizes": sorted(set(s["fft_points"] for s in streams))}
+
+
+def serve_cfg5(torch, seconds, rate=200.0, seed=5, plain_seconds=None):
+    from swspec_amd import workloads as W
+    from swspec_amd.server import Server
+    trace = W.serving_trace(seed=seed, rate=rate, duration=seconds)
+    pool = W.RingPool(torch, ring_ffts=8)
+    # pre-build every ring the trace needs (outside the timed replay)
+    for jb in trace:
+        fmt = "raw1" if jb["nch"] == 1 else "mk5b4"
+        pool.get(fmt, jb["fft_points"], jb["fs"], 0, [[(0.1, 4.8e6 * jb["fs"] / 32e6, 0.0)]] * 4,
+                 17 + jb["fft_points"])
+    torch.cuda.synchronize()
+
+    def run(tr, cap, max_wall):
+        srv = Server(max_active=max(cap, 4))
+        r = W.run_serving(torch, srv, tr, pool, max_active_channels=cap, max_wall=max_wall)
+        srv.close()
+        lat = np.array(r["latencies"]) if r["latencies"] else np.array([np.nan])
+        return {"jobs": r["submitted"], "completed": r["completed"], "truncated": r["truncated"],
+                "gsps": round(r["samples"] / r["wall"] / 1e9, 3),
+                "p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
+                "wall_s": round(r["wall"], 2)}
+
+    served = run(trace, 256, 3 * seconds + 30)
+    ptrace = [j for j in trace if j["t"] < (plain_seconds if plain_seconds else seconds)]
+    plain = run(ptrace, 4, 40)   # one job (<= 4 channels) at a time
+    offered = sum(j["nffts"] * j["fft_points"] * j["nch"] for j in trace) / seconds / 1e9
+    return {"rate_jobs_per_s": rate, "arrival_window_s": seconds, "offered_gsps": round(offered, 2),
+            "continuous_batching": served, "plain_one_job_at_a_time": plain,
+            "plain_arrival_window_s": plain_seconds if plain_seconds else seconds}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +231,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--chunk", type=int, default=0, help="FFTs per chunk (0 = automatic)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-stream batch")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving replay")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="Poisson arrival window")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the workload is one GPU's share; "
@@ -190,6 +294,17 @@ def main():
     # sanity on the output: spectra finite, tone comb visible in the phase-cal
     a0 = d_auto[0].cpu().numpy()
     assert np.isfinite(a0).all()
+    core.finalize()
+
+    log("plain per-channel cores ...")
+    plain_value, plain_ms = plain_cfg2(torch, swspec_amd, raw, s)
+    batch = serve = None
+    if not args.no_batch:
+        log("64-stream batch ...")
+        batch = batch_cfg4(torch)
+    if not args.no_serve:
+        log("serving replay ...")
+        serve = serve_cfg5(torch, args.serve_seconds, plain_seconds=min(args.serve_seconds, 3.0))
 
     cpu = None
     if not args.no_cpu:
@@ -209,10 +324,14 @@ def main():
         "roofline": roofline,
         "kernels": per_kernel,
         "cpu_baseline": cpu,
+        "plain": {"value": round(plain_value, 3), "ms_per_step": round(plain_ms, 3),
+                  "how": "4 single-channel cores, sequential run/join (one per UseFile1Channel)"},
+        "batched_vs_plain": round(value / plain_value, 2),
+        "batch_64_streams": batch,
+        "serving": serve,
     }
     if cpu:
         out["gpu_vs_cpu"] = round(value / cpu["value"], 1)
-    core.finalize()
     print(json.dumps(out), flush=True)
 
 

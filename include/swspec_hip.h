@@ -230,6 +230,61 @@ double swspec_hip_last_run_seconds(const swspec_hip_core* core);
 int swspec_hip_set_profiling(swspec_hip_core* core, int on);
 int swspec_hip_kernel_stats(const swspec_hip_core* core, int kernel, double* total_ms, int64_t* launches);
 
+/*
+ * ---- batching and serving --------------------------------------------------
+ * No TaskCore counterpart: the reference runs one job (file set) per process.
+ * A job is one channel of one device-resident raw stream, integrated over
+ * `nffts` consecutive non-overlapped FFTs into one spectrum scaled by 1/nffts
+ * (the reference's 1/K with K = nffts, TaskCoreIPP.cpp:124), with optional
+ * phase-cal (folded vector + FP64 tones).  The server advances every active
+ * job by one chunk of FFTs per step, grouping jobs of equal FFT length into
+ * shared launches; partial integrations stay resident on the device between
+ * steps; admission is capped by a device-memory budget and a job count.
+ */
+#define SWSPEC_EAGAIN (-6)      /* no capacity right now: retry after jobs finish */
+
+typedef struct swspec_hip_job {
+    uint32_t struct_size;          /* sizeof(swspec_hip_job) */
+    int32_t  window;               /* swspec_window */
+    int64_t  fft_points;
+    int32_t  source_format;        /* RawSigned/RawUnsigned/Mk5B (payload) */
+    int32_t  source_channels;
+    int32_t  bits_per_sample;      /* 2 */
+    int32_t  channelorder_increasing;
+    int32_t  channel;              /* 0-based */
+    int32_t  nffts;                /* FFTs integrated (>= 1) */
+    int32_t  extract_pcal;
+    int32_t  pad_;
+    double   samplingfreq;
+    double   pcaloffsethz;
+    double   pcalharmonicshz;
+    const void* d_raw;             /* device bytes; FFT k starts at k * raw bytes per FFT */
+    uint64_t raw_len;
+    float*   d_out_auto;           /* fft_points/2 + 1 floats (device) */
+    float*   d_out_pcal;           /* 2 * tone bins floats (device) or NULL */
+    double*  d_tones;              /* 2 * tones doubles (device) or NULL */
+} swspec_hip_job;
+
+typedef struct swspec_hip_server swspec_hip_server;
+
+/* hbm_budget_bytes <= 0: half of the free device memory.  max_active <= 0: 256. */
+int swspec_hip_server_create(int device, int64_t hbm_budget_bytes, int max_active, swspec_hip_server** out);
+/* Admit a job (allocates its resident state).  SWSPEC_EAGAIN when full. */
+int swspec_hip_server_submit(swspec_hip_server* s, const swspec_hip_job* job, int64_t* job_id);
+/* Enqueue one step: every active job advances by one chunk.  Non-blocking
+ * (at most two steps in flight).  *nactive: jobs still active afterwards. */
+int swspec_hip_server_step(swspec_hip_server* s, int* nactive);
+/* Ids of jobs whose results are complete on the device (non-blocking). */
+int swspec_hip_server_poll(swspec_hip_server* s, int64_t* done_ids, int max_ids, int* ndone);
+/* Block until all enqueued work is complete. */
+int swspec_hip_server_sync(swspec_hip_server* s);
+/* Run all admitted jobs to completion.  use_graph: capture the whole batch
+ * once into a hipGraph and replay it `replays` times (the job set is kept for
+ * replays); otherwise step-by-step launches, once. */
+int swspec_hip_server_run_batch(swspec_hip_server* s, int use_graph, int replays);
+int swspec_hip_server_destroy(swspec_hip_server* s);
+const char* swspec_hip_server_last_error(const swspec_hip_server* s);
+
 /* Pinned host memory for Buffer rings (hipHostMalloc) and its release. */
 void* swspec_hip_host_alloc(size_t bytes);
 void swspec_hip_host_free(void* p);

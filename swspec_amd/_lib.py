@@ -23,7 +23,22 @@ EXPORTS = [
     "swspec_hip_finalize", "swspec_hip_last_error", "swspec_hip_pcal_tones", "swspec_hip_pcal_ntones",
     "swspec_hip_last_run_seconds", "swspec_hip_host_alloc", "swspec_hip_host_free",
     "swspec_hip_unpack_device", "swspec_hip_plan_info", "swspec_hip_set_profiling", "swspec_hip_kernel_stats",
+    "swspec_hip_server_create", "swspec_hip_server_submit", "swspec_hip_server_step", "swspec_hip_server_poll",
+    "swspec_hip_server_sync", "swspec_hip_server_run_batch", "swspec_hip_server_destroy",
+    "swspec_hip_server_last_error",
 ]
+
+
+class Job(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32), ("window", C.c_int32), ("fft_points", C.c_int64),
+        ("source_format", C.c_int32), ("source_channels", C.c_int32), ("bits_per_sample", C.c_int32),
+        ("channelorder_increasing", C.c_int32), ("channel", C.c_int32), ("nffts", C.c_int32),
+        ("extract_pcal", C.c_int32), ("pad_", C.c_int32), ("samplingfreq", C.c_double),
+        ("pcaloffsethz", C.c_double), ("pcalharmonicshz", C.c_double), ("d_raw", C.c_void_p),
+        ("raw_len", C.c_uint64), ("d_out_auto", C.c_void_p), ("d_out_pcal", C.c_void_p),
+        ("d_tones", C.c_void_p),
+    ]
 
 
 class Cfg(C.Structure):
@@ -111,6 +126,15 @@ def load():
     L.swspec_hip_plan_info.argtypes = [C.c_int64, C.POINTER(C.c_int32), C.c_int]
     L.swspec_hip_set_profiling.argtypes = [P, C.c_int]
     L.swspec_hip_kernel_stats.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.swspec_hip_server_create.argtypes = [C.c_int, C.c_int64, C.c_int, C.POINTER(P)]
+    L.swspec_hip_server_submit.argtypes = [P, C.POINTER(Job), C.POINTER(C.c_int64)]
+    L.swspec_hip_server_step.argtypes = [P, C.POINTER(C.c_int)]
+    L.swspec_hip_server_poll.argtypes = [P, C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_int)]
+    L.swspec_hip_server_sync.argtypes = [P]
+    L.swspec_hip_server_run_batch.argtypes = [P, C.c_int, C.c_int]
+    L.swspec_hip_server_destroy.argtypes = [P]
+    L.swspec_hip_server_last_error.argtypes = [P]
+    L.swspec_hip_server_last_error.restype = C.c_char_p
     _lib = L
     return L
 

@@ -302,12 +302,13 @@ static int core_alloc(swspec_hip_core* c)
 {
     const Plan& p = c->plan;
     const size_t M1 = (size_t)p.M + 1;
+    const size_t S = (size_t)p.smax;   // accumulator slices per job
     HIPCHK(c, hipMalloc(&c->d_Y, (size_t)c->njobs * c->G * (size_t)p.M * sizeof(float2)));
-    HIPCHK(c, hipMalloc(&c->d_acc, (size_t)c->njobs * 2 * M1 * sizeof(float)));
-    HIPCHK(c, (hipError_t)launch_zero(c->d_acc, (size_t)c->njobs * 2 * M1 * sizeof(float), c->stream));
+    HIPCHK(c, hipMalloc(&c->d_acc, (size_t)c->njobs * S * 2 * M1 * sizeof(float)));
+    HIPCHK(c, (hipError_t)launch_zero(c->d_acc, (size_t)c->njobs * S * 2 * M1 * sizeof(float), c->stream));
     if (c->nx) {
-        HIPCHK(c, hipMalloc(&c->d_xacc, 4 * M1 * sizeof(float)));
-        HIPCHK(c, (hipError_t)launch_zero(c->d_xacc, 4 * M1 * sizeof(float), c->stream));
+        HIPCHK(c, hipMalloc(&c->d_xacc, S * 4 * M1 * sizeof(float)));
+        HIPCHK(c, (hipError_t)launch_zero(c->d_xacc, S * 4 * M1 * sizeof(float), c->stream));
     }
     if (c->cfg.extract_pcal) {
         const size_t part = (size_t)c->njobs * c->G * c->pg.nblk_max * c->pg.T * 2;
@@ -355,12 +356,15 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
     if (!e.empty()) return fail(c, SWSPEC_EUNSUPPORTED, e);
 
     const long long M = c->plan.M;
+    // FFTs per chunk: phase-cal masks hold 64 FFTs; without phase-cal up to 1024
+    const int gcap = cfg->extract_pcal ? 64 : 1024;
     if (cfg->ffts_per_chunk > 0) {
-        c->G = (int)std::min<uint32_t>(cfg->ffts_per_chunk, 64);
+        c->G = (int)std::min<uint32_t>(cfg->ffts_per_chunk, (uint32_t)gcap);
     } else {
-        // keep the chunk's intermediate near 256 MB (MALL-sized), 1..32 FFTs
+        // chunk intermediate up to ~1 GB: the 3.2M-point four-channel case
+        // (20 FFTs per chunk) measured fastest there; fewer accumulator passes
         const double per = (double)njobs * M * sizeof(float2);
-        c->G = (int)std::max(1.0, std::min(32.0, std::floor(256.0e6 / per)));
+        c->G = (int)std::max(1.0, std::min((double)gcap, std::floor(1.05e9 / per)));
     }
     c->G = std::max(1, std::min(c->G, cfg->core_overlapped_ffts));
 
@@ -506,9 +510,10 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
                 JobDev& J = jobs[j];
                 J.raw = d_in[c->job_src[j]] + w.start + (long long)f0 * g.raw_overlap_bytes;
                 J.fft_stride = g.raw_overlap_bytes;
+                J.nf = nf;
                 J.up = c->up[j];
                 J.Y = c->d_Y + (size_t)j * c->G * (size_t)p.M;
-                J.acc = c->d_acc + (size_t)j * 2 * M1;
+                J.acc = c->d_acc + (size_t)j * p.smax * 2 * M1;
                 if (g.extract_pcal) {
                     J.pcal_part = c->d_pcal_part + (size_t)j * c->G * c->pg.nblk_max * c->pg.T * 2;
                     J.pcal_acc = c->d_pcal_acc + (size_t)j * c->pg.T * 2;
@@ -532,20 +537,20 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
             {
                 ProfScope ps(c, SWSPEC_K_PASS2);
                 HIPCHK(c, launch_pass2(p, jobs, c->njobs, c->nx ? &xp : nullptr, nf,
-                                       (g.flags & SWSPEC_FLAG_NO_KAHAN) ? 0 : 1, c->stream));
+                                       (g.flags & SWSPEC_FLAG_NO_KAHAN) ? 0 : 1, p.smax, c->stream));
             }
         }
         ProfScope psf(c, SWSPEC_K_FINALIZE);
         // TaskCoreIPP.cpp:491-498 scale 1/K for complete spectra; partial unscaled
         const double scale = w.complete ? 1.0 / (double)g.core_overlapped_ffts : 1.0;
         for (int j = 0; j < c->njobs; j++) {
-            float* acc = c->d_acc + (size_t)j * 2 * M1;
-            HIPCHK(c, launch_finalize(p, acc, d_auto[j] + sp * (size_t)ssb, scale, c->stream));
-            HIPCHK(c, launch_zero(acc, 2 * M1 * sizeof(float), c->stream));
+            float* acc = c->d_acc + (size_t)j * p.smax * 2 * M1;
+            HIPCHK(c, launch_finalize(p, acc, d_auto[j] + sp * (size_t)ssb, scale, p.smax, c->stream));
+            HIPCHK(c, launch_zero(acc, (size_t)p.smax * 2 * M1 * sizeof(float), c->stream));
         }
         if (c->nx) {
-            HIPCHK(c, launch_finalize_xpol(p, c->d_xacc, d_xpol + sp * 2 * (size_t)ssb, scale, c->stream));
-            HIPCHK(c, launch_zero(c->d_xacc, 4 * M1 * sizeof(float), c->stream));
+            HIPCHK(c, launch_finalize_xpol(p, c->d_xacc, d_xpol + sp * 2 * (size_t)ssb, scale, p.smax, c->stream));
+            HIPCHK(c, launch_zero(c->d_xacc, (size_t)p.smax * 4 * M1 * sizeof(float), c->stream));
         }
         if (g.extract_pcal) {
             for (int j = 0; j < c->njobs; j++) {

@@ -17,7 +17,7 @@
 namespace swspec {
 
 constexpr int kMaxStages = 16;
-constexpr int kMaxJobs = 8;       // jobs per engine launch (kernel-argument tables)
+constexpr int kMaxJobs = 16;      // jobs per engine launch (kernel-argument tables)
 
 // 2-bit decoder kinds (src/IA-32/DataUnpackerFactory.h:38-108)
 enum UnpackKind : int {
@@ -70,6 +70,8 @@ struct Plan {
     int lgB = 0;             // log2(B)
     int p1_threads = 0, p2_threads = 0;
     int p1_maxpts = 0, p2_maxpts = 0;
+    int p1_fpw = 1;          // FFT pipelines per pass-1 workgroup (short columns)
+    int smax = 1;            // accumulator slices per job (pass-2 FFT parallelism)
     int npairs = 0;          // pass-2 workgroups per FFT (row pairs)
     int twshift = 0;         // four-step twiddle split W_M^e = A[e>>s] * B[e & (2^s-1)]
     SubPlan sp1, sp2;
@@ -94,14 +96,17 @@ std::string plan_geometry(Plan& p, long long N);
 
 // One job as the kernels see it.
 struct JobDev {
-    const uint8_t* raw = nullptr;  // device pointer to the first byte of FFT 0 of the chunk
-    long long fft_stride = 0;      // bytes between consecutive FFT starts in the chunk
+    const uint8_t* raw = nullptr;  // device pointer: FFT i of the stream starts at raw + i*fft_stride
+    long long fft_stride = 0;      // bytes between consecutive FFT starts
+    long long fft0 = 0;            // stream index of FFT 0 of the chunk
+    long long nwrap = 0;           // > 0: input is a ring of nwrap FFTs (index taken mod nwrap)
     Unpack up;
     float2* Y = nullptr;           // pass-1 output [G][N1][N2]
     float* acc = nullptr;          // accumulator: (sum, comp) pairs, permuted layout, M+1 bins
     double* pcal_part = nullptr;   // pcal partial sums [G][nblk][T] double2
     double* pcal_acc = nullptr;    // pcal accumulator [T] double2
     unsigned long long pcal_mask = 0;  // bit f: FFT f of the chunk gets phase-cal
+    int nf = 0;                    // FFTs of this job in the chunk (<= launch nfft)
 };
 
 struct XpolDev {
@@ -127,12 +132,24 @@ struct PcalGeom {
 // chunk, fast, ppb, gpt from T and L).
 void pcal_geometry(PcalGeom& g, long long N);
 
+// Start of FFT f of a chunk.
+__host__ __device__ inline const uint8_t* job_fft_ptr(const JobDev& J, int f)
+{
+    long long i = J.fft0 + f;
+    if (J.nwrap > 0) i %= J.nwrap;
+    return J.raw + i * J.fft_stride;
+}
+
 // Launchers (engine.hip).
 hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, hipStream_t s);
+// Each job's accumulator holds `smax` slices of 2*(M+1) floats (cross-power:
+// 4*(M+1)); pass 2 spreads a chunk's FFTs over up to smax slices and
+// finalize sums all slices in order.
 hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const XpolDev* xp, int nfft,
-                        int kahan, hipStream_t s);
-hipError_t launch_finalize(const Plan& p, const float* acc, float* out, double scale, hipStream_t s);
-hipError_t launch_finalize_xpol(const Plan& p, const float* acc, float* out, double scale, hipStream_t s);
+                        int kahan, int smax, hipStream_t s);
+hipError_t launch_finalize(const Plan& p, const float* acc, float* out, double scale, int nslices, hipStream_t s);
+hipError_t launch_finalize_xpol(const Plan& p, const float* acc, float* out, double scale, int nslices,
+                                hipStream_t s);
 hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s);
 hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int njobs, int nfft,
                        hipStream_t s);

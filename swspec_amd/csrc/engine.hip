@@ -233,6 +233,7 @@ struct P1Args {
     int N1, N2;
     long long M;
     int B, lgB;
+    int fpw, pth;     // FFT pipelines per workgroup, threads per pipeline (B * T1)
     int twshift;
     SubPlan sp;
     const float2* stw;
@@ -266,11 +267,18 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     unsigned bx, by, bz;
     xcd_remap(bx, by, bz);
     const JobDev J = a.jobs[bz];
-    const int f = (int)by;
+    // a workgroup runs `fpw` independent pipelines, one FFT each (short columns)
+    const int fpw = a.fpw, pth = a.pth;
+    if ((int)by * fpw >= J.nf) return;   // no FFT of this job here (whole workgroup)
+    int pipe = 0;
+    while (pipe + 1 < fpw && threadIdx.x >= (unsigned)((pipe + 1) * pth)) pipe++;
+    const int f = (int)by * fpw + pipe;
+    const bool active = f < J.nf;
     const int B = a.B, lgB = a.lgB, N1 = a.N1, N2 = a.N2;
     const int c0 = (int)bx * B;
-    const int tid = threadIdx.x, nth = blockDim.x;
-    const uint8_t* __restrict__ raw = J.raw + (long long)f * J.fft_stride;
+    const int tid = threadIdx.x - pipe * pth, nth = pth;
+    float2* __restrict__ lp = lds + pipe * (N1 * B);
+    const uint8_t* __restrict__ raw = job_fft_ptr(J, active ? f : 0);
     const Unpack up = J.up;
     const bool even = ((uintptr_t)raw & 1) == 0;
 
@@ -288,8 +296,8 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
         psi1 = make_float2((float)c, (float)s);
     }
 
-    // lds[e*B + b] = z[N2*e + c0 + b] windowed; idx == e*B + b
-    for (int idx = tid; idx < N1 * B; idx += nth) {
+    // lp[e*B + b] = z[N2*e + c0 + b] windowed; idx == e*B + b
+    for (int idx = tid; active && idx < N1 * B; idx += nth) {
         const int e = idx >> lgB;
         const long long n = (long long)N2 * e + c0 + bcol;
         unsigned f0, f1;
@@ -302,20 +310,20 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
             w0 = wg.a0 + wg.a1 * cs0 + wg.a2 * sn0 + wg.a3 * (2.f * cs0 * cs0 - 1.f);
             w1 = wg.a0 + wg.a1 * cs1 + wg.a2 * sn1 + wg.a3 * (2.f * cs1 * cs1 - 1.f);
         }
-        lds[idx] = make_float2(map2bit(f0) * w0, map2bit(f1) * w1);
+        lp[idx] = make_float2(map2bit(f0) * w0, map2bit(f1) * w1);
     }
     __syncthreads();
 
-    fft_lds<MAXPTS>(lds, a.sp, a.stw, bcol, B, tid >> lgB, nth >> lgB);
+    fft_lds<MAXPTS>(lds, a.sp, a.stw, pipe * (N1 * B) + bcol, B, tid >> lgB, nth >> lgB);
 
-    float2* __restrict__ Y = J.Y + (long long)f * a.M;
+    float2* __restrict__ Y = J.Y + (long long)(active ? f : 0) * a.M;
     const unsigned mask = (1u << a.twshift) - 1u;
-    for (int idx = tid; idx < N1 * B; idx += nth) {
+    for (int idx = tid; active && idx < N1 * B; idx += nth) {
         const int k1 = idx >> lgB, b = idx & (B - 1);
         const int n2 = c0 + b;
         const unsigned e = (unsigned)n2 * (unsigned)k1;   // < N1*N2 = M: no reduction needed
         const float2 w = cmul(a.twA[e >> a.twshift], a.twB[e & mask]);
-        Y[(long long)k1 * N2 + n2] = cmul(lds[idx], w);
+        Y[(long long)k1 * N2 + n2] = cmul(lp[idx], w);
     }
 }
 
@@ -328,6 +336,12 @@ struct P2Args {
     long long M;
     int nfft;
     int kahan;
+    int S;            // slices: slice s sums FFTs [s*per, (s+1)*per) into accumulator slice s
+    int fpw;          // slice pipelines per workgroup
+    int pth;          // threads per pipeline (2 * NSRC * T)
+    int pipe_lds;     // float2 units of LDS per pipeline
+    long long accstride;   // floats between accumulator slices of a job (2*(M+1))
+    long long xaccstride;  // floats between cross-power slices (4*(M+1))
     SubPlan sp;
     const float2* stw;
     const float2* rA;
@@ -357,15 +371,22 @@ __device__ __forceinline__ void kahan_add(float* __restrict__ p, float v)
 template <int MAXPTS, int NSRC>
 __global__ void __launch_bounds__(512) k_pass2(P2Args a)
 {
-    extern __shared__ float2 lds[];
+    extern __shared__ float2 lds_all[];
     const int p = blockIdx.x, grp = blockIdx.y;
     const int N1 = a.N1, N2 = a.N2;
     const int rowA = p;
     const int rowB = (p == 0) ? 0 : N1 - p;
     const bool self = (rowA == rowB);
-    const int tid = threadIdx.x, nth = blockDim.x;
+    // pipeline (slice) of this thread: blockDim == fpw * pth
+    const int pth = a.pth;
+    int pipe = 0;
+    while (pipe + 1 < a.fpw && threadIdx.x >= (unsigned)((pipe + 1) * pth)) pipe++;
+    const int tid = threadIdx.x - pipe * pth, nth = pth;
+    const int sl_i = blockIdx.z * a.fpw + pipe;          // slice index
+    const bool sact = sl_i < a.S;
+    float2* __restrict__ lds = lds_all + pipe * a.pipe_lds;
     constexpr int NSL = 2 * NSRC;
-    const int T = a.sp.T;                   // blockDim == NSL * T
+    const int T = a.sp.T;                   // pth == NSL * T
     int slot = 0;
 #pragma unroll
     for (int s = 1; s < NSL; s++) slot += (tid >= s * T) ? 1 : 0;
@@ -375,7 +396,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
 #pragma unroll
     for (int s = 0; s < NSRC; s++) {
         Yp[s] = a.jobs[grp * NSRC + s].Y;
-        accp[s] = a.jobs[grp * NSRC + s].acc;
+        accp[s] = a.jobs[grp * NSRC + s].acc + (sact ? sl_i : 0) * a.accstride;
     }
     const int nbins = (self ? 1 : 2) * N2;
     // chunk sums in LDS after the FFT rows: |X|^2 per source, then X0*conj(X1)
@@ -389,18 +410,25 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
 #pragma unroll
     for (int s = 0; s < NSRC; s++) nyqP[s] = 0.f;
 
-    for (int f = 0; f < a.nfft; f++) {
+    const int nfj = a.jobs[grp * NSRC].nf;   // FFTs of this job (pair) in the chunk
+    const int per = (nfj + a.S - 1) / a.S;   // FFTs per slice (uniform trip count)
+    const int fb = sl_i * per;
+    for (int it = 0; it < per; it++) {
+        const int f = fb + it;
+        const bool act = sact && f < nfj;
+        if (act) {
 #pragma unroll
-        for (int sl = 0; sl < NSL; sl++) {
-            const int row = (sl & 1) ? rowB : rowA;
-            const float2* __restrict__ Ys = Yp[sl >> 1] + (long long)f * a.M + (long long)row * N2;
-            for (int e = tid; e < N2; e += nth) lds[sl * N2 + e] = Ys[e];
+            for (int sl = 0; sl < NSL; sl++) {
+                const int row = (sl & 1) ? rowB : rowA;
+                const float2* __restrict__ Ys = Yp[sl >> 1] + (long long)f * a.M + (long long)row * N2;
+                for (int e = tid; e < N2; e += nth) lds[sl * N2 + e] = Ys[e];
+            }
         }
         __syncthreads();
-        fft_lds<MAXPTS>(lds, a.sp, a.stw, slot * N2, 1, q0, T);
+        fft_lds<MAXPTS>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * N2, 1, q0, T);
 
-        // each thread keeps the same bins for every FFT of the chunk
-        for (int idx = tid; idx < nbins; idx += nth) {
+        // each thread keeps the same bins for every FFT of the slice
+        for (int idx = tid; act && idx < nbins; idx += nth) {
             const int which = idx >= N2 ? 1 : 0, k2 = idx - which * N2;
             const int row = which ? rowB : rowA;
             int msl, mk2;
@@ -436,7 +464,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
         __syncthreads();
     }
 
-    for (int idx = tid; idx < nbins; idx += nth) {
+    for (int idx = tid; sact && idx < nbins; idx += nth) {
         {
             const int which = idx >= N2 ? 1 : 0, k2 = idx - which * N2;
             const int row = which ? rowB : rowA;
@@ -449,7 +477,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
                 else acc[0] += v;
             }
             if (NSRC == 2) {
-                float* xa = a.xp[grp].acc + 4 * o;
+                float* xa = a.xp[grp].acc + sl_i * a.xaccstride + 4 * o;
                 const float2 v = accxl[idx];
                 if (a.kahan) {
                     float s0 = xa[0], s1 = xa[1], c0 = xa[2], c1 = xa[3];
@@ -470,7 +498,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
                     else acc[0] += nyqP[s];
                 }
                 if (NSRC == 2) {
-                    float* xa = a.xp[grp].acc + 4 * a.M;
+                    float* xa = a.xp[grp].acc + sl_i * a.xaccstride + 4 * a.M;
                     if (a.kahan) {
                         float s0 = xa[0], c0 = xa[2];
                         float y0 = nyqX - c0, t0 = s0 + y0;
@@ -489,25 +517,34 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
 // scaled in FP64, accumulators cleared for the next spectrum.
 // ---------------------------------------------------------------------------
 
+// Slices are summed in slice order in FP64 (sum - compensation per slice).
 __global__ void k_finalize(float* __restrict__ acc, float* __restrict__ out, long long M, int N1, int N2,
-                           double scale)
+                           double scale, int S, long long stride)
 {
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k <= M;
          k += (long long)gridDim.x * blockDim.x) {
         const long long idx = (k == M) ? M : (k % N1) * (long long)N2 + k / N1;
-        const double v = (double)acc[2 * idx] - (double)acc[2 * idx + 1];
+        double v = 0.0;
+        for (int s = 0; s < S; s++) {
+            const float* a = acc + s * stride + 2 * idx;
+            v += (double)a[0] - (double)a[1];
+        }
         out[k] = (float)(v * scale);
     }
 }
 
 __global__ void k_finalize_xpol(float* __restrict__ acc, float* __restrict__ out, long long M, int N1, int N2,
-                                double scale)
+                                double scale, int S, long long stride)
 {
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k <= M;
          k += (long long)gridDim.x * blockDim.x) {
         const long long idx = (k == M) ? M : (k % N1) * (long long)N2 + k / N1;
-        const double re = (double)acc[4 * idx] - (double)acc[4 * idx + 2];
-        const double im = (double)acc[4 * idx + 1] - (double)acc[4 * idx + 3];
+        double re = 0.0, im = 0.0;
+        for (int s = 0; s < S; s++) {
+            const float* a = acc + s * stride + 4 * idx;
+            re += (double)a[0] - (double)a[2];
+            im += (double)a[1] - (double)a[3];
+        }
         out[2 * k] = (float)(re * scale);
         // DC and Nyquist cross products are real (TaskCoreIPP.cpp:482-485)
         out[2 * k + 1] = (k == 0 || k == M) ? 0.0f : (float)(im * scale);
@@ -540,7 +577,7 @@ __global__ void __launch_bounds__(1024) k_pcal(PcArgs a)
     const int T = a.g.T, L = a.g.L;
     double ar = 0.0, ai = 0.0;
     if ((J.pcal_mask >> f) & 1ull) {
-        const uint8_t* __restrict__ raw = J.raw + (long long)f * J.fft_stride;
+        const uint8_t* __restrict__ raw = job_fft_ptr(J, f);
         const Unpack up = J.up;
         const long long s0 = (long long)blk * a.g.chunk;
         const long long s1 = min(s0 + (long long)a.g.chunk, a.N);
@@ -584,7 +621,7 @@ __global__ void __launch_bounds__(1024) k_pcal_fast(PcArgs a)
     const int ngroups = L >> 2;
     double accr[4] = {0.0, 0.0, 0.0, 0.0}, acci[4] = {0.0, 0.0, 0.0, 0.0};
     if ((J.pcal_mask >> f) & 1ull) {
-        const uint8_t* __restrict__ raw = J.raw + (long long)f * J.fft_stride;
+        const uint8_t* __restrict__ raw = job_fft_ptr(J, f);
         const int nper = (int)(a.N / L);
         const int p0 = blk * a.g.ppb;
         const int p1 = min(p0 + a.g.ppb, nper);
@@ -734,62 +771,83 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
     P1Args a;
     a.N1 = p.N1; a.N2 = p.N2; a.M = p.M; a.B = p.B; a.lgB = p.lgB; a.twshift = p.twshift;
     a.sp = p.sp1; a.stw = p.d_stw1; a.twA = p.d_twA; a.twB = p.d_twB; a.win = p.win;
+    a.fpw = std::max(1, std::min(p.p1_fpw, nfft));
+    a.pth = p.p1_threads;
     for (int j = 0; j < njobs; j++) a.jobs[j] = jobs[j];
-    dim3 grid(p.N2 / p.B, nfft, njobs);
-    const size_t lds = p1_lds(p);
+    dim3 grid(p.N2 / p.B, (nfft + a.fpw - 1) / a.fpw, njobs);
+    const size_t lds = p1_lds(p) * a.fpw;
+    const dim3 block(a.fpw * a.pth);
     switch (p.p1_maxpts) {
-    case 16: hipLaunchKernelGGL(k_pass1<16>, grid, dim3(p.p1_threads), lds, s, a); break;
-    case 32: hipLaunchKernelGGL(k_pass1<32>, grid, dim3(p.p1_threads), lds, s, a); break;
+    case 16: hipLaunchKernelGGL(k_pass1<16>, grid, block, lds, s, a); break;
+    case 32: hipLaunchKernelGGL(k_pass1<32>, grid, block, lds, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+// Slices per job for a chunk of nfft FFTs: enough pipelines to fill the
+// device (~16k row-pair pipelines), at most smax (allocated accumulator slices).
+static int pass2_slices(const Plan& p, int ngroups, int nfft, int smax)
+{
+    const long long wg = (long long)p.npairs * ngroups;
+    int S = (int)std::min<long long>(std::max<long long>(1, (16384 + wg - 1) / wg), 1024);
+    return std::max(1, std::min(S, std::min(smax, nfft)));
+}
+
 hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const XpolDev* xp, int nfft,
-                        int kahan, hipStream_t s)
+                        int kahan, int smax, hipStream_t s)
 {
     const int nsrc = xp ? 2 : 1;
-    if (njobs < 1 || njobs > kMaxJobs || (njobs % nsrc) != 0 || nfft < 1) return hipErrorInvalidValue;
+    if (njobs < 1 || njobs > kMaxJobs || (njobs % nsrc) != 0 || nfft < 1 || smax < 1) return hipErrorInvalidValue;
     P2Args a;
     a.N1 = p.N1; a.N2 = p.N2; a.M = p.M; a.nfft = nfft; a.kahan = kahan;
     a.sp = p.sp2; a.stw = p.d_stw2; a.rA = p.d_rA; a.rB = p.d_rB;
     for (int j = 0; j < njobs; j++) a.jobs[j] = jobs[j];
     if (xp) for (int g = 0; g < njobs / 2; g++) a.xp[g] = xp[g];
-    const int threads = nsrc * 2 * p.sp2.T;
-    dim3 grid(p.npairs, njobs / nsrc);
-    const size_t lds = p2_lds(p, nsrc);
+    a.S = pass2_slices(p, njobs / nsrc, nfft, smax);
+    a.pth = nsrc * 2 * p.sp2.T;
+    const size_t lds1 = p2_lds(p, nsrc);
+    a.pipe_lds = (int)(lds1 / sizeof(float2));
+    // pack short rows: up to 256 threads and 64 KB of LDS per workgroup
+    a.fpw = std::max(1, std::min({256 / a.pth, (int)((64 * 1024) / lds1), a.S}));
+    a.accstride = 2 * (p.M + 1);
+    a.xaccstride = 4 * (p.M + 1);
+    const dim3 grid(p.npairs, njobs / nsrc, (a.S + a.fpw - 1) / a.fpw);
+    const dim3 block(a.fpw * a.pth);
+    const size_t lds = lds1 * a.fpw;
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     if (nsrc == 1) {
         switch (p.p2_maxpts) {
-        case 16: hipLaunchKernelGGL((k_pass2<16, 1>), grid, dim3(threads), lds, s, a); break;
-        case 32: hipLaunchKernelGGL((k_pass2<32, 1>), grid, dim3(threads), lds, s, a); break;
+        case 16: hipLaunchKernelGGL((k_pass2<16, 1>), grid, block, lds, s, a); break;
+        case 32: hipLaunchKernelGGL((k_pass2<32, 1>), grid, block, lds, s, a); break;
         default: return hipErrorInvalidValue;
         }
     } else {
         switch (p.p2_maxpts) {
-        case 16: hipLaunchKernelGGL((k_pass2<16, 2>), grid, dim3(threads), lds, s, a); break;
-        case 32: hipLaunchKernelGGL((k_pass2<32, 2>), grid, dim3(threads), lds, s, a); break;
+        case 16: hipLaunchKernelGGL((k_pass2<16, 2>), grid, block, lds, s, a); break;
+        case 32: hipLaunchKernelGGL((k_pass2<32, 2>), grid, block, lds, s, a); break;
         default: return hipErrorInvalidValue;
         }
     }
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(const Plan& p, const float* acc, float* out, double scale, hipStream_t s)
+hipError_t launch_finalize(const Plan& p, const float* acc, float* out, double scale, int nslices, hipStream_t s)
 {
     const long long n = p.M + 1;
     const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, s, const_cast<float*>(acc), out, p.M, p.N1,
-                       p.N2, scale);
+                       p.N2, scale, nslices, 2 * (p.M + 1));
     return hipGetLastError();
 }
 
-hipError_t launch_finalize_xpol(const Plan& p, const float* acc, float* out, double scale, hipStream_t s)
+hipError_t launch_finalize_xpol(const Plan& p, const float* acc, float* out, double scale, int nslices,
+                                hipStream_t s)
 {
     const long long n = p.M + 1;
     const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_finalize_xpol, dim3(blocks), dim3(256), 0, s, const_cast<float*>(acc), out, p.M,
-                       p.N1, p.N2, scale);
+                       p.N1, p.N2, scale, nslices, 4 * (p.M + 1));
     return hipGetLastError();
 }
 
@@ -1016,6 +1074,11 @@ std::string plan_geometry(Plan& p, long long N)
     p.p2_threads = 2 * T2;
     p.npairs = p.N1 / 2 + 1;
     p.twshift = 10;
+    // short columns: several FFTs per pass-1 workgroup (<= 256 threads, <= 64 KB LDS)
+    p.p1_fpw = std::max(1, std::min(256 / p.p1_threads,
+                                    (int)((64 * 1024) / ((size_t)p.N1 * p.B * sizeof(float2)))));
+    // accumulator slices: up to 1024, at most 16 MiB of slices per job
+    p.smax = (int)std::max<long long>(1, std::min<long long>(1024, (16LL << 20) / (8 * (p.M + 1))));
     return "";
 }
 

@@ -1,0 +1,168 @@
+"""GPU tests of the batched/served engine (swspec_hip_server_*).
+
+Each server job = one channel integrated over `nffts` non-overlapped FFTs,
+scaled 1/nffts: the oracle's doMaths with K = nffts on the same bytes.
+Tolerance: normwise relative error <= 1e-5 vs the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import relerr, say
+from swspec_amd import synth
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _oracle_mean(oracle, raw, n, nffts, fmt, nch, ch, window, fs, pcal=False):
+    s = oracle.derive(fft_points=n, fft_integ_seconds=n * nffts / fs, fft_overlap_factor=1, window=window,
+                      bandwidth_hz=fs / 2, bits_per_sample=2, source_channels=nch, source_format=fmt,
+                      use_channel_file1=ch + 1, max_source_buffer_mb=1 << 16, extract_pcal=1 if pcal else 0)
+    assert s.core_overlapped_ffts == nffts
+    m, auto, _, pc = oracle.domaths(s, [raw], 64)
+    assert m == 1
+    return auto[0][0], (pc[0][0] if pcal else None), s
+
+
+def _mkjob(kind, n, seed, nffts_data, fs):
+    if kind == "raw1":
+        return synth.raw_1ch(seed, n * nffts_data, tones=[(0.2, 0.21 * fs, 0.1)], fs=fs), "RawSigned", 1
+    rng = np.random.default_rng(seed)
+    tones = [synth.pcal_comb(fs, fs / 2, amp=0.15, rng=rng) for _ in range(4)]
+    return synth.raw_multi(seed, 4, n * nffts_data, tones_per_ch=tones, fs=fs), "Mk5B", 4
+
+
+def test_server_jobs_vs_oracle(hip, oracle):
+    import torch
+    from swspec_amd.server import Server
+    fs = 32e6
+    specs = [("raw1", 4096, 7, "Hann", 0), ("mk5b4", 20000, 5, "Cosine2", 2), ("raw1", 65536, 3, "Hamming", 0),
+             ("mk5b4", 32000, 9, "Blackman", 1), ("raw1", 4096, 200, "Cosine2", 0), ("mk5b4", 320000, 3, "Cosine", 3)]
+    srv = Server()
+    outs, refs = [], []
+    for i, (kind, n, nffts, win, ch) in enumerate(specs):
+        say("server job", kind, n, nffts, win, ch)
+        raw, fmt, nch = _mkjob(kind, n, 100 + i, nffts, fs)
+        d_raw = torch.from_numpy(raw).cuda()
+        out = torch.full((n // 2 + 1,), float("nan"), device="cuda")
+        jid = srv.submit(n, nffts, d_raw, len(raw), out, window=win, source_format=fmt, source_channels=nch,
+                         channel=ch, samplingfreq=fs, keep=d_raw)
+        assert jid is not None
+        ref, _, _ = _oracle_mean(oracle, raw, n, nffts, fmt, nch, ch, win, fs)
+        outs.append(out)
+        refs.append(ref)
+    srv.run_batch(use_graph=False)
+    for (spec, o, r) in zip(specs, outs, refs):
+        e = relerr(o.cpu().numpy(), r)
+        say("  ", spec, "err %.3e" % e)
+        assert e <= TOL
+    srv.close()
+
+
+def test_server_ring_wrap_and_pcal(hip, oracle):
+    """A job longer than its input ring reads slot k mod ring; phase-cal tones."""
+    import torch
+    from swspec_amd.server import Server
+    fs, n, ring_ffts, nffts = 32e6, 32000, 3, 8
+    raw, fmt, nch = _mkjob("mk5b4", n, 11, ring_ffts, fs)
+    tiled = np.concatenate([raw] * (nffts // ring_ffts + 1))[: n * nffts]
+    ref, pref, s = _oracle_mean(oracle, tiled, n, nffts, fmt, nch, 1, "Cosine2", fs, pcal=True)
+    srv = Server()
+    d_raw = torch.from_numpy(raw).cuda()
+    out = torch.empty(n // 2 + 1, device="cuda")
+    pc = torch.empty(2 * s.pcal_tonebins, device="cuda")
+    nt = oracle.pcal_ntones(fs / 2, 1e6, 10e3)
+    tones = torch.empty(2 * nt, dtype=torch.float64, device="cuda")
+    srv.submit(n, nffts, d_raw, len(raw), out, window="Cosine2", source_format=fmt, source_channels=nch, channel=1,
+               samplingfreq=fs, extract_pcal=True, d_out_pcal=pc, d_tones=tones)
+    srv.run_batch(use_graph=False)
+    e = relerr(out.cpu().numpy(), ref)
+    p = pc.cpu().numpy()
+    ep = relerr(p[0::2] + 1j * p[1::2], pref)
+    acc = np.stack([pref.real, pref.imag], 1).reshape(-1)
+    tr = oracle.pcal_tones64(acc, nt, 1e6, fs)
+    t = tones.cpu().numpy()
+    et = relerr(t[0::2] + 1j * t[1::2], tr)
+    say("ring/pcal: spectrum %.3e pcal %.3e tones %.3e" % (e, ep, et))
+    assert e <= TOL and ep <= 1e-5 and et <= 1e-5
+    srv.close()
+
+
+def _batch(torch, Server, use_graph, replays=1, seed=3):
+    from swspec_amd import workloads as W
+    streams = W.batch_workload(seed=seed, nstreams=12)
+    for s in streams:
+        s["nffts"] = min(s["nffts"], 40)
+        s["fft_points"] = min(s["fft_points"], 320000)
+    srv = Server()
+    pool = W.RingPool(torch, ring_ffts=4)
+    outs, total = W.submit_batch(torch, srv, streams, pool)
+    srv.run_batch(use_graph=use_graph, replays=replays)
+    res = [o.cpu().numpy() for o in outs]
+    srv.close()
+    return res, total
+
+
+def test_server_graph_replay_matches_steps(hip):
+    import torch
+    from swspec_amd.server import Server
+    say("graph vs steps")
+    a, _ = _batch(torch, Server, use_graph=False)
+    b, _ = _batch(torch, Server, use_graph=True, replays=3)
+    for x, y in zip(a, b):
+        assert np.isfinite(x).all()
+        assert np.array_equal(x, y)
+
+
+def test_result_same_alone_and_in_mix(hip, oracle):
+    """Continuous batching: a job's result is the same alone and mixed with others."""
+    import torch
+    from swspec_amd.server import Server
+    fs, n, nffts = 32e6, 320000, 12
+    raw, fmt, nch = _mkjob("mk5b4", n, 21, nffts, fs)
+    d_raw = torch.from_numpy(raw).cuda()
+    srv = Server()
+    alone = torch.empty(n // 2 + 1, device="cuda")
+    srv.submit(n, nffts, d_raw, len(raw), alone, source_format=fmt, source_channels=4, channel=0, samplingfreq=fs)
+    srv.run_batch(use_graph=False)
+    # in a mix: same job admitted mid-run while other jobs of several sizes are active
+    others = []
+    for i, (m, k) in enumerate([(4096, 300), (320000, 30), (65536, 50), (320000, 7)]):
+        r, f, c = _mkjob("raw1", m, 300 + i, min(k, 8), fs)
+        d = torch.from_numpy(r).cuda()
+        o = torch.empty(m // 2 + 1, device="cuda")
+        srv.submit(m, k, d, len(r), o, source_format=f, source_channels=c, samplingfreq=fs, keep=d)
+        others.append(o)
+    srv.step()
+    srv.step()
+    mixed = torch.empty(n // 2 + 1, device="cuda")
+    srv.submit(n, nffts, d_raw, len(raw), mixed, source_format=fmt, source_channels=4, channel=0, samplingfreq=fs)
+    while srv.step() > 0:
+        pass
+    srv.sync()
+    ref, _, _ = _oracle_mean(oracle, raw, n, nffts, fmt, nch, 0, "Cosine2", fs)
+    a, m = alone.cpu().numpy(), mixed.cpu().numpy()
+    say("alone vs mix %.3e, alone vs oracle %.3e, mix vs oracle %.3e" % (relerr(m, a), relerr(a, ref), relerr(m, ref)))
+    assert relerr(m, a) <= TOL and relerr(a, ref) <= TOL and relerr(m, ref) <= TOL
+    for o in others:
+        assert np.isfinite(o.cpu().numpy()).all()
+    srv.close()
+
+
+def test_server_admission_cap(hip):
+    import torch
+    from swspec_amd.server import Server
+    srv = Server(max_active=2)
+    d = torch.zeros(4096, dtype=torch.uint8, device="cuda")
+    outs = [torch.empty(2049, device="cuda") for _ in range(3)]
+    assert srv.submit(4096, 4, d, 1024, outs[0]) is not None
+    assert srv.submit(4096, 4, d, 1024, outs[1]) is not None
+    assert srv.submit(4096, 4, d, 1024, outs[2]) is None       # full: caller retries
+    while srv.step() > 0:
+        pass
+    srv.sync()
+    done = srv.poll()
+    assert len(done) == 2
+    assert srv.submit(4096, 4, d, 1024, outs[2]) is not None    # capacity freed
+    srv.run_batch(use_graph=False)
+    srv.close()
