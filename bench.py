@@ -234,6 +234,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-stream batch")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving replay")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="Poisson arrival window")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (3x): the reference point that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the workload is one GPU's share; "
@@ -241,6 +244,16 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        n = 1 << 28   # float32 elements = 1 GiB
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": n * 4, "launches": 3}), flush=True)
+        return
+
     import swspec_amd
     from swspec_amd import TaskCoreHIP, plan_info
 

@@ -295,7 +295,10 @@ void swspec_hip_host_free(void* p);
 int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d_raw, int channel, int64_t count,
                              float* d_out);
 
-/* Plan geometry for an FFT length (no device work): N1, N2, B, threads. */
+/* Plan geometry for an FFT length (no device work).  info[0..11] = N1, N2,
+ * pass-1 columns per workgroup, pass-1 threads, pass-2 threads, row pairs,
+ * pass-1 / pass-2 points per thread, pass-1 / pass-2 composite-radix flags,
+ * pass-1 / pass-2 LDS stages. */
 int swspec_hip_plan_info(int64_t fft_points, int32_t* info, int ninfo);
 
 #ifdef __cplusplus

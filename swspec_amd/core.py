@@ -268,7 +268,8 @@ def unpack_device(settings, d_raw, channel, count, d_out, device=0):
 
 
 def plan_info(fft_points):
-    info = (C.c_int32 * 8)()
-    check(load().swspec_hip_plan_info(fft_points, info, 8), None, "swspec_hip_plan_info")
-    keys = ["N1", "N2", "B", "p1_threads", "p2_threads", "npairs", "p1_maxpts", "p2_maxpts"]
+    info = (C.c_int32 * 12)()
+    check(load().swspec_hip_plan_info(fft_points, info, 12), None, "swspec_hip_plan_info")
+    keys = ["N1", "N2", "B", "p1_threads", "p2_threads", "npairs", "p1_maxpts", "p2_maxpts",
+            "p1_cx", "p2_cx", "p1_stages", "p2_stages"]
     return dict(zip(keys, list(info)))

@@ -804,7 +804,8 @@ extern "C" int swspec_hip_plan_info(int64_t fft_points, int32_t* info, int ninfo
     Plan p;
     std::string e = plan_geometry(p, fft_points);
     if (!e.empty()) return fail(nullptr, SWSPEC_EUNSUPPORTED, e);
-    int32_t v[8] = {p.N1, p.N2, p.B, p.p1_threads, p.sp2.T * 2, p.npairs, p.p1_maxpts, p.p2_maxpts};
-    for (int i = 0; i < ninfo && i < 8; i++) info[i] = v[i];
+    int32_t v[12] = {p.N1,        p.N2,        p.B,      p.p1_threads, p.sp2.T * 2, p.npairs,
+                     p.p1_maxpts, p.p2_maxpts, p.p1_cx, p.p2_cx,      p.sp1.nst,   p.sp2.nst};
+    for (int i = 0; i < ninfo && i < 12; i++) info[i] = v[i];
     return SWSPEC_OK;
 }

@@ -70,6 +70,7 @@ struct Plan {
     int lgB = 0;             // log2(B)
     int p1_threads = 0, p2_threads = 0;
     int p1_maxpts = 0, p2_maxpts = 0;
+    int p1_cx = 0, p2_cx = 0;    // sub-plan uses composite register radices (10..32)
     int p1_fpw = 1;          // FFT pipelines per pass-1 workgroup (short columns)
     int smax = 1;            // accumulator slices per job (pass-2 FFT parallelism)
     int npairs = 0;          // pass-2 workgroups per FFT (row pairs)

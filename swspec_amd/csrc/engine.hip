@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 namespace swspec {
 
@@ -117,6 +118,105 @@ template <> struct Dft<8> {
     }
 };
 
+// Register radices.  R is a plain radix (2, 3, 4, 5, 8) or a composite
+// R = A*B (10 = 2*5, 16 = 4*4, 20 = 4*5, 25 = 5*5, 32 = 4*8).  A stage with
+// ns > 1 multiplies input r by W_D^{k r} (D = ns*R, k = q % ns).  A plain
+// radix reads those R-1 twiddles from its table; a composite one reads A-1
+// values W_D^{k B n1} and B-1 values W_D^{k n2} and applies them around its
+// A-point DFTs, so a butterfly loads A+B-2 twiddles instead of R-1.
+__host__ __device__ constexpr int radix_a(int R)
+{
+    return R == 10 ? 2 : R == 16 ? 4 : R == 20 ? 4 : R == 25 ? 5 : R == 32 ? 4 : R;
+}
+__host__ __device__ constexpr int radix_ntw(int R)
+{
+    return radix_a(R) == R ? R - 1 : radix_a(R) + R / radix_a(R) - 2;
+}
+
+// Roots of unity at compile time: Taylor series on the angle reduced to (-pi, pi].
+constexpr double kPiD = 3.14159265358979323846264338327950288;
+constexpr double cx_sin(double x)
+{
+    double t = x, s = x;
+    for (int i = 1; i < 18; i++) { t *= -x * x / ((2.0 * i) * (2.0 * i + 1)); s += t; }
+    return s;
+}
+constexpr double cx_cos(double x)
+{
+    double t = 1.0, s = 1.0;
+    for (int i = 1; i < 18; i++) { t *= -x * x / ((2.0 * i - 1) * (2.0 * i)); s += t; }
+    return s;
+}
+template <int R> struct RootTable {
+    float re[R], im[R];
+    constexpr RootTable() : re(), im()
+    {
+        for (int e = 0; e < R; e++) {
+            double x = -2.0 * kPiD * e / R;       // W_R^e
+            if (x <= -kPiD) x += 2.0 * kPiD;
+            re[e] = (float)cx_cos(x);
+            im[e] = (float)cx_sin(x);
+        }
+    }
+};
+
+// Twiddle (when tw) + butterfly; output r goes to st(r, value).
+template <int R, bool COMP = (radix_a(R) != R)> struct DftStore {
+    template <class St>
+    __device__ __forceinline__ static void run(float2* v, const float2* __restrict__ w, bool tw, St st)
+    {
+        if (tw) {
+#pragma unroll
+            for (int r = 1; r < R; r++) v[r] = cmul(v[r], w[r - 1]);
+        }
+        Dft<R>::run(v);
+#pragma unroll
+        for (int r = 0; r < R; r++) st(r, v[r]);
+    }
+};
+
+// Composite R = A*B: n = B n1 + n2, k = k1 + A k2,
+//   X[k1 + A k2] = sum_n2 W_B^{n2 k2} [W_D^{k n2} W_R^{n2 k1}]
+//                         sum_n1 (v[B n1 + n2] W_D^{k B n1}) W_A^{n1 k1}.
+// Step 1 (A-point DFTs + twiddles) runs in place; step 2 (B-point DFTs) hands
+// each output straight to the store, so no second register array is needed.
+// The twiddles are applied unconditionally: a first stage (ns = 1, k = 0)
+// reads its table's ones, which keeps one code path.
+template <int R> struct DftStore<R, true> {
+    static constexpr int A = radix_a(R), B = R / A;
+    static constexpr RootTable<R> kRoots{};
+    template <class St>
+    __device__ __forceinline__ static void run(float2* v, const float2* __restrict__ w, bool, St st)
+    {
+#pragma unroll
+        for (int n2 = 0; n2 < B; n2++) {
+            float2 u[A];
+#pragma unroll
+            for (int n1 = 0; n1 < A; n1++) {
+                u[n1] = v[B * n1 + n2];
+                if (n1 > 0) u[n1] = cmul(u[n1], w[n1 - 1]);
+            }
+            Dft<A>::run(u);
+#pragma unroll
+            for (int k1 = 0; k1 < A; k1++) {
+                float2 x = u[k1];
+                if (n2 > 0) x = cmul(x, w[A - 1 + n2 - 1]);
+                const int e = (n2 * k1) % R;
+                v[B * k1 + n2] = (e == 0) ? x : cmul(x, make_float2(kRoots.re[e], kRoots.im[e]));
+            }
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < A; k1++) {
+            float2 u[B];
+#pragma unroll
+            for (int n2 = 0; n2 < B; n2++) u[n2] = v[B * k1 + n2];
+            Dft<B>::run(u);
+#pragma unroll
+            for (int k2 = 0; k2 < B; k2++) st(k1 + A * k2, u[k2]);
+        }
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Stockham stage in LDS.  Element e of the FFT lives at lds[ebase + e*estr].
 // Butterfly q (q < L/R) reads q + r*L/R, writes (q - q%ns)*R + q%ns + r*ns.
@@ -152,21 +252,19 @@ __device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int n
         const int q = q0 + i * T;
         if (q < nb) {
             const int k = (ns > 1) ? mod_magic(q, ns, mag) : 0;
-            if (ns > 1) {
-                const float2* w = tw + k * (R - 1);
-#pragma unroll
-                for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], w[r - 1]);
-            }
-            Dft<R>::run(v[i]);
             const int o = (q - k) * R + k;
-#pragma unroll
-            for (int r = 0; r < R; r++) lds[ebase + (o + r * ns) * estr] = v[i][r];
+            float2* __restrict__ dst = lds + ebase + o * estr;
+            const int dstr = ns * estr;
+            DftStore<R>::run(v[i], tw + k * radix_ntw(R), ns > 1, [&](int r, float2 x) { dst[r * dstr] = x; });
         }
     }
     __syncthreads();
 }
 
-template <int MAXPTS>
+// All radices a kernel instantiation supports are compiled into one switch and
+// the register allocation covers the largest, so the composite radices (CX)
+// live in their own instantiations: a plan uses them only where they pay.
+template <int MAXPTS, bool CX>
 __device__ __forceinline__ void fft_lds(float2* lds, const SubPlan& sp, const float2* __restrict__ stw,
                                         int ebase, int estr, int q0, int T)
 {
@@ -174,12 +272,32 @@ __device__ __forceinline__ void fft_lds(float2* lds, const SubPlan& sp, const fl
         const float2* tw = stw + sp.twoff[s];
         const int ns = sp.ns[s];
         const unsigned mag = sp.mag[s];
+        // Opaque per-stage copies: otherwise every case's per-thread index math
+        // is hoisted out of the stage loop and held live across all stages.
+        int q0s = q0, ebs = ebase;
+        asm volatile("" : "+v"(q0s), "+v"(ebs));
         switch (sp.radix[s]) {
-        case 2: stage_lds<2, MAXPTS / 2>(lds, sp.L, ns, mag, tw, ebase, estr, q0, T); break;
-        case 3: stage_lds<3, MAXPTS / 3>(lds, sp.L, ns, mag, tw, ebase, estr, q0, T); break;
-        case 4: stage_lds<4, MAXPTS / 4>(lds, sp.L, ns, mag, tw, ebase, estr, q0, T); break;
-        case 5: stage_lds<5, MAXPTS / 5>(lds, sp.L, ns, mag, tw, ebase, estr, q0, T); break;
-        default: stage_lds<8, MAXPTS / 8>(lds, sp.L, ns, mag, tw, ebase, estr, q0, T); break;
+        case 2: stage_lds<2, MAXPTS / 2>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T); break;
+        case 3: stage_lds<3, MAXPTS / 3>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T); break;
+        case 4: stage_lds<4, MAXPTS / 4>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T); break;
+        case 5: stage_lds<5, MAXPTS / 5>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T); break;
+        case 8: stage_lds<8, MAXPTS / 8>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T); break;
+        case 10:
+            if constexpr (CX) stage_lds<10, MAXPTS / 10>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T);
+            break;
+        case 16:
+            if constexpr (CX) stage_lds<16, MAXPTS / 16>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T);
+            break;
+        case 20:
+            if constexpr (CX && MAXPTS >= 20) stage_lds<20, MAXPTS / 20>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T);
+            break;
+        case 25:
+            if constexpr (CX && MAXPTS >= 25) stage_lds<25, MAXPTS / 25>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T);
+            break;
+        case 32:
+            if constexpr (CX && MAXPTS >= 32) stage_lds<32, MAXPTS / 32>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T);
+            break;
+        default: break;
         }
     }
 }
@@ -209,17 +327,19 @@ __device__ __forceinline__ unsigned field2(const uint8_t* __restrict__ raw, cons
 // pass 1: unpack + window + column FFTs + four-step twiddle
 // ---------------------------------------------------------------------------
 
-// Both samples of complex point n (real samples 2n, 2n+1) from one load where
-// the layout allows it.
+// Both samples of complex point n (real samples 2n, 2n+1).  MODE 0: single
+// channel, 4 samples per byte (both in one byte); 1: one byte per time sample
+// at an even address (one 16-bit load); 2: byte groups of `step` bytes.
+template <int MODE>
 __device__ __forceinline__ void fields_pair(const uint8_t* __restrict__ raw, const Unpack& up, long long n,
-                                            bool even, unsigned& f0, unsigned& f1)
+                                            unsigned& f0, unsigned& f1)
 {
-    if (up.kind == kUnpackSingleChannel) {
+    if (MODE == 0) {
         const unsigned b = raw[n >> 1];
         const int sh = 6 - 4 * (int)(n & 1);
         f0 = (b >> sh) & 3u;
         f1 = (b >> (sh - 2)) & 3u;
-    } else if (up.step == 1 && even) {
+    } else if (MODE == 1) {
         const unsigned w = *(const uint16_t*)(raw + 2 * n);   // time samples 2n, 2n+1
         f0 = (w >> up.shift) & 3u;
         f1 = (w >> (8 + up.shift)) & 3u;
@@ -227,6 +347,15 @@ __device__ __forceinline__ void fields_pair(const uint8_t* __restrict__ raw, con
         f0 = (raw[2 * n * up.step + up.cbyte] >> up.shift) & 3u;
         f1 = (raw[(2 * n + 1) * up.step + up.cbyte] >> up.shift) & 3u;
     }
+}
+
+// Strided loops over [tid, total) issue their global loads in batches of U per
+// thread before using any of them: the batch index is clamped into range, so
+// every load is valid and a batch is one straight-line block the compiler can
+// keep in flight together (a plain loop waits for each load in turn).
+template <int U> __device__ __forceinline__ int batch_idx(int base, int u, int nth, int total)
+{
+    return min(base + u * nth, total - 1);
 }
 
 struct P1Args {
@@ -260,7 +389,7 @@ __device__ __forceinline__ void xcd_remap(unsigned& x, unsigned& y, unsigned& z)
     z = yz / gy;
 }
 
-template <int MAXPTS>
+template <int MAXPTS, bool CX>
 __global__ void __launch_bounds__(512) k_pass1(P1Args a)
 {
     extern __shared__ float2 lds[];
@@ -285,6 +414,8 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     // Window on the fly: sample j = 2(N2 e + c) + r has angle phi_e + psi_{c,r}
     // with phi_e from a per-row table and psi fixed per thread (its column c
     // never changes: nth is a multiple of B).  w = c0 + c1 cos + c2 sin + c3 cos2.
+    // Without a window the table holds phi = 0, psi = 0 and (c0..c3) = (1,0,0,0),
+    // so the same arithmetic gives w = 1 exactly.
     const WindowGen wg = a.win;
     const int bcol = tid & (B - 1);
     float2 psi0 = make_float2(1.f, 0.f), psi1 = make_float2(1.f, 0.f);
@@ -295,35 +426,67 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
         sincospi(2.0 * ((double)(2 * (c0 + bcol) + 1) + wg.delta) / wg.D, &s, &c);
         psi1 = make_float2((float)c, (float)s);
     }
+    const float wa0 = wg.kind ? wg.a0 : 1.f, wa1 = wg.kind ? wg.a1 : 0.f, wa2 = wg.kind ? wg.a2 : 0.f,
+                wa3 = wg.kind ? wg.a3 : 0.f;
 
     // lp[e*B + b] = z[N2*e + c0 + b] windowed; idx == e*B + b
-    for (int idx = tid; active && idx < N1 * B; idx += nth) {
-        const int e = idx >> lgB;
-        const long long n = (long long)N2 * e + c0 + bcol;
-        unsigned f0, f1;
-        fields_pair(raw, up, n, even, f0, f1);
-        float w0 = 1.f, w1 = 1.f;
-        if (wg.kind != 0) {
-            const float2 ph = wg.phi[e];
-            const float cs0 = ph.x * psi0.x - ph.y * psi0.y, sn0 = ph.y * psi0.x + ph.x * psi0.y;
-            const float cs1 = ph.x * psi1.x - ph.y * psi1.y, sn1 = ph.y * psi1.x + ph.x * psi1.y;
-            w0 = wg.a0 + wg.a1 * cs0 + wg.a2 * sn0 + wg.a3 * (2.f * cs0 * cs0 - 1.f);
-            w1 = wg.a0 + wg.a1 * cs1 + wg.a2 * sn1 + wg.a3 * (2.f * cs1 * cs1 - 1.f);
+    const int total = N1 * B;
+    const long long cb = c0 + bcol;
+    auto fill = [&](auto mode) {
+        constexpr int MODE = decltype(mode)::value;
+        constexpr int U = 8;
+        for (int base = tid; base < total; base += U * nth) {
+            unsigned f0[U], f1[U];
+            float2 ph[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = batch_idx<U>(base, u, nth, total) >> lgB;
+                fields_pair<MODE>(raw, up, (long long)N2 * e + cb, f0[u], f1[u]);
+                ph[u] = wg.phi[e];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int idx = base + u * nth;
+                if (idx < total) {
+                    const float2 p = ph[u];
+                    const float cs0 = p.x * psi0.x - p.y * psi0.y, sn0 = p.y * psi0.x + p.x * psi0.y;
+                    const float cs1 = p.x * psi1.x - p.y * psi1.y, sn1 = p.y * psi1.x + p.x * psi1.y;
+                    const float w0 = wa0 + wa1 * cs0 + wa2 * sn0 + wa3 * (2.f * cs0 * cs0 - 1.f);
+                    const float w1 = wa0 + wa1 * cs1 + wa2 * sn1 + wa3 * (2.f * cs1 * cs1 - 1.f);
+                    lp[idx] = make_float2(map2bit(f0[u]) * w0, map2bit(f1[u]) * w1);
+                }
+            }
         }
-        lp[idx] = make_float2(map2bit(f0) * w0, map2bit(f1) * w1);
+    };
+    if (active) {
+        if (up.kind == kUnpackSingleChannel) fill(std::integral_constant<int, 0>());
+        else if (up.step == 1 && even) fill(std::integral_constant<int, 1>());
+        else fill(std::integral_constant<int, 2>());
     }
     __syncthreads();
 
-    fft_lds<MAXPTS>(lds, a.sp, a.stw, pipe * (N1 * B) + bcol, B, tid >> lgB, nth >> lgB);
+    fft_lds<MAXPTS, CX>(lds, a.sp, a.stw, pipe * (N1 * B) + bcol, B, tid >> lgB, nth >> lgB);
 
     float2* __restrict__ Y = J.Y + (long long)(active ? f : 0) * a.M;
     const unsigned mask = (1u << a.twshift) - 1u;
-    for (int idx = tid; active && idx < N1 * B; idx += nth) {
-        const int k1 = idx >> lgB, b = idx & (B - 1);
-        const int n2 = c0 + b;
-        const unsigned e = (unsigned)n2 * (unsigned)k1;   // < N1*N2 = M: no reduction needed
-        const float2 w = cmul(a.twA[e >> a.twshift], a.twB[e & mask]);
-        Y[(long long)k1 * N2 + n2] = cmul(lp[idx], w);
+    constexpr int U = 8;
+    for (int base = tid; active && base < total; base += U * nth) {
+        float2 tA[U], tB[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int idx = batch_idx<U>(base, u, nth, total);
+            const unsigned e = (unsigned)(c0 + (idx & (B - 1))) * (unsigned)(idx >> lgB);   // n2 k1 < M
+            tA[u] = a.twA[e >> a.twshift];
+            tB[u] = a.twB[e & mask];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int idx = base + u * nth;
+            if (idx < total) {
+                const int k1 = idx >> lgB, n2 = c0 + (idx & (B - 1));
+                Y[(long long)k1 * N2 + n2] = cmul(lp[idx], cmul(tA[u], tB[u]));
+            }
+        }
     }
 }
 
@@ -368,7 +531,7 @@ __device__ __forceinline__ void kahan_add(float* __restrict__ p, float v)
     p[1] = c;
 }
 
-template <int MAXPTS, int NSRC>
+template <int MAXPTS, int NSRC, bool CX>
 __global__ void __launch_bounds__(512) k_pass2(P2Args a)
 {
     extern __shared__ float2 lds_all[];
@@ -417,15 +580,27 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
         const int f = fb + it;
         const bool act = sact && f < nfj;
         if (act) {
+            const float2* __restrict__ Ys[NSL];
 #pragma unroll
-            for (int sl = 0; sl < NSL; sl++) {
-                const int row = (sl & 1) ? rowB : rowA;
-                const float2* __restrict__ Ys = Yp[sl >> 1] + (long long)f * a.M + (long long)row * N2;
-                for (int e = tid; e < N2; e += nth) lds[sl * N2 + e] = Ys[e];
+            for (int sl = 0; sl < NSL; sl++)
+                Ys[sl] = Yp[sl >> 1] + (long long)f * a.M + (long long)((sl & 1) ? rowB : rowA) * N2;
+            // all rows' loads of a batch in flight together
+            constexpr int U = 16 / NSL;
+            for (int base = tid; base < N2; base += U * nth) {
+                float2 t[NSL][U];
+#pragma unroll
+                for (int sl = 0; sl < NSL; sl++)
+#pragma unroll
+                    for (int u = 0; u < U; u++) t[sl][u] = Ys[sl][batch_idx<U>(base, u, nth, N2)];
+#pragma unroll
+                for (int sl = 0; sl < NSL; sl++)
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        if (base + u * nth < N2) lds[sl * N2 + base + u * nth] = t[sl][u];
             }
         }
         __syncthreads();
-        fft_lds<MAXPTS>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * N2, 1, q0, T);
+        fft_lds<MAXPTS, CX>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * N2, 1, q0, T);
 
         // each thread keeps the same bins for every FFT of the slice
         for (int idx = tid; act && idx < nbins; idx += nth) {
@@ -464,31 +639,56 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
         __syncthreads();
     }
 
-    for (int idx = tid; sact && idx < nbins; idx += nth) {
-        {
+    // accumulator (sum, compensation) pairs: loads batched like the row loads
+    float* __restrict__ xbase = NSRC == 2 ? a.xp[grp].acc + sl_i * a.xaccstride : nullptr;
+    constexpr int UA = 4;
+    for (int base = tid; sact && base < nbins; base += UA * nth) {
+        float2 cur[NSRC][UA];
+        float4 xcur[UA];
+#pragma unroll
+        for (int u = 0; u < UA; u++) {
+            const int idx = batch_idx<UA>(base, u, nth, nbins);
+            const int which = idx >= N2 ? 1 : 0;
+            const long long o = (long long)(which ? rowB : rowA) * N2 + (idx - which * N2);
+#pragma unroll
+            for (int s = 0; s < NSRC; s++) cur[s][u] = *(const float2*)(accp[s] + 2 * o);
+            if (NSRC == 2) xcur[u] = *(const float4*)(xbase + 4 * o);
+        }
+#pragma unroll
+        for (int u = 0; u < UA; u++) {
+            const int idx = base + u * nth;
+            if (idx >= nbins) continue;
             const int which = idx >= N2 ? 1 : 0, k2 = idx - which * N2;
             const int row = which ? rowB : rowA;
             const long long o = (long long)row * N2 + k2;
 #pragma unroll
             for (int s = 0; s < NSRC; s++) {
-                float* acc = accp[s] + 2 * o;
+                float2 sc = cur[s][u];
                 const float v = accl[s * 2 * N2 + idx];
-                if (a.kahan) kahan_add(acc, v);
-                else acc[0] += v;
+                if (a.kahan) {
+                    const float y = v - sc.y, t = sc.x + y;
+                    sc.y = (t - sc.x) - y;
+                    sc.x = t;
+                } else {
+                    sc.x += v;
+                }
+                *(float2*)(accp[s] + 2 * o) = sc;
             }
             if (NSRC == 2) {
-                float* xa = a.xp[grp].acc + sl_i * a.xaccstride + 4 * o;
+                float4 x = xcur[u];
                 const float2 v = accxl[idx];
                 if (a.kahan) {
-                    float s0 = xa[0], s1 = xa[1], c0 = xa[2], c1 = xa[3];
-                    float y0 = v.x - c0, y1 = v.y - c1;
-                    float t0 = s0 + y0, t1 = s1 + y1;
-                    xa[2] = (t0 - s0) - y0; xa[3] = (t1 - s1) - y1;
-                    xa[0] = t0; xa[1] = t1;
+                    const float y0 = v.x - x.z, y1 = v.y - x.w;
+                    const float t0 = x.x + y0, t1 = x.y + y1;
+                    x.z = (t0 - x.x) - y0;
+                    x.w = (t1 - x.y) - y1;
+                    x.x = t0;
+                    x.y = t1;
                 } else {
-                    xa[0] += v.x;
-                    xa[1] += v.y;
+                    x.x += v.x;
+                    x.y += v.y;
                 }
+                *(float4*)(xbase + 4 * o) = x;
             }
             if (row == 0 && k2 == 0) {
 #pragma unroll
@@ -777,9 +977,11 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
     dim3 grid(p.N2 / p.B, (nfft + a.fpw - 1) / a.fpw, njobs);
     const size_t lds = p1_lds(p) * a.fpw;
     const dim3 block(a.fpw * a.pth);
-    switch (p.p1_maxpts) {
-    case 16: hipLaunchKernelGGL(k_pass1<16>, grid, block, lds, s, a); break;
-    case 32: hipLaunchKernelGGL(k_pass1<32>, grid, block, lds, s, a); break;
+    switch (p.p1_maxpts * 2 + (p.p1_cx ? 1 : 0)) {
+    case 32: hipLaunchKernelGGL((k_pass1<16, false>), grid, block, lds, s, a); break;
+    case 33: hipLaunchKernelGGL((k_pass1<16, true>), grid, block, lds, s, a); break;
+    case 64: hipLaunchKernelGGL((k_pass1<32, false>), grid, block, lds, s, a); break;
+    case 65: hipLaunchKernelGGL((k_pass1<32, true>), grid, block, lds, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -808,26 +1010,29 @@ hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const Xpol
     a.pth = nsrc * 2 * p.sp2.T;
     const size_t lds1 = p2_lds(p, nsrc);
     a.pipe_lds = (int)(lds1 / sizeof(float2));
-    // pack short rows: up to 256 threads and 64 KB of LDS per workgroup
-    a.fpw = std::max(1, std::min({256 / a.pth, (int)((64 * 1024) / lds1), a.S}));
+    // pack short rows: up to 256 threads (SWSPEC_P2_WG_THREADS) and 64 KB of LDS per workgroup
+    static const int wg_threads = [] {
+        const char* e = getenv("SWSPEC_P2_WG_THREADS");
+        const int v = e ? atoi(e) : 256;
+        return (v >= 64 && v <= 512) ? v : 256;
+    }();
+    a.fpw = std::max(1, std::min({wg_threads / a.pth, (int)((64 * 1024) / lds1), a.S}));
     a.accstride = 2 * (p.M + 1);
     a.xaccstride = 4 * (p.M + 1);
     const dim3 grid(p.npairs, njobs / nsrc, (a.S + a.fpw - 1) / a.fpw);
     const dim3 block(a.fpw * a.pth);
     const size_t lds = lds1 * a.fpw;
     if (lds > 160 * 1024) return hipErrorInvalidValue;
-    if (nsrc == 1) {
-        switch (p.p2_maxpts) {
-        case 16: hipLaunchKernelGGL((k_pass2<16, 1>), grid, block, lds, s, a); break;
-        case 32: hipLaunchKernelGGL((k_pass2<32, 1>), grid, block, lds, s, a); break;
-        default: return hipErrorInvalidValue;
-        }
-    } else {
-        switch (p.p2_maxpts) {
-        case 16: hipLaunchKernelGGL((k_pass2<16, 2>), grid, block, lds, s, a); break;
-        case 32: hipLaunchKernelGGL((k_pass2<32, 2>), grid, block, lds, s, a); break;
-        default: return hipErrorInvalidValue;
-        }
+    switch (p.p2_maxpts * 4 + (nsrc - 1) * 2 + (p.p2_cx ? 1 : 0)) {
+    case 64: hipLaunchKernelGGL((k_pass2<16, 1, false>), grid, block, lds, s, a); break;
+    case 65: hipLaunchKernelGGL((k_pass2<16, 1, true>), grid, block, lds, s, a); break;
+    case 66: hipLaunchKernelGGL((k_pass2<16, 2, false>), grid, block, lds, s, a); break;
+    case 67: hipLaunchKernelGGL((k_pass2<16, 2, true>), grid, block, lds, s, a); break;
+    case 128: hipLaunchKernelGGL((k_pass2<32, 1, false>), grid, block, lds, s, a); break;
+    case 129: hipLaunchKernelGGL((k_pass2<32, 1, true>), grid, block, lds, s, a); break;
+    case 130: hipLaunchKernelGGL((k_pass2<32, 2, false>), grid, block, lds, s, a); break;
+    case 131: hipLaunchKernelGGL((k_pass2<32, 2, true>), grid, block, lds, s, a); break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
@@ -968,43 +1173,71 @@ static bool smooth235(long long n)
     return n == 1;
 }
 
-// Radix list for an L-point sub-FFT: 5s, 3s, then 8s, a 4 or 2.
-static bool subplan_radices(int L, SubPlan& sp)
-{
-    int n = L, ns = 1;
-    sp.L = L;
-    sp.nst = 0;
-    int twl = 0;
-    auto push = [&](int R) {
-        sp.radix[sp.nst] = R;
-        sp.ns[sp.nst] = ns;
-        sp.mag[sp.nst] = ns > 1 ? (unsigned)(((1ULL << 32) + (unsigned long long)ns - 1) / (unsigned long long)ns) : 0u;
-        sp.twoff[sp.nst] = twl;
-        twl += ns * (R - 1);
-        ns *= R;
-        sp.nst++;
-    };
-    while (n % 5 == 0) { if (sp.nst >= kMaxStages) return false; push(5); n /= 5; }
-    while (n % 3 == 0) { if (sp.nst >= kMaxStages) return false; push(3); n /= 3; }
-    while (n % 8 == 0) { if (sp.nst >= kMaxStages) return false; push(8); n /= 8; }
-    if (n % 4 == 0) { push(4); n /= 4; }
-    if (n % 2 == 0) { push(2); n /= 2; }
-    sp.twlen = twl;
-    return n == 1 && sp.nst > 0;
-}
-
-// Smallest thread count per FFT such that each stage's butterflies per thread
-// fit the register budget MAXPTS/R.
-static int subplan_threads(const SubPlan& sp, int maxpts)
+// Threads per FFT such that each stage's butterflies per thread fit the
+// register budget maxpts/R.
+static int radices_threads(int L, const int* rad, int nst, int maxpts)
 {
     int T = 1;
-    for (int s = 0; s < sp.nst; s++) {
-        const int R = sp.radix[s];
-        const int nb = sp.L / R;
-        const int per = maxpts / R;
+    for (int s = 0; s < nst; s++) {
+        const int per = maxpts / rad[s];
+        const int nb = L / rad[s];
         T = std::max(T, (nb + per - 1) / per);
     }
     return T;
+}
+
+// Radix list for an L-point sub-FFT.  Every stage is one LDS round trip, so the
+// search takes the fewest stages from the register radices that fit maxpts
+// points per thread (with `composite`, 10, 16, 20, 25, 32 as well as 2, 3, 4,
+// 5, 8), then the fewest threads per FFT.
+static bool subplan_radices(int L, SubPlan& sp, int maxpts, bool composite)
+{
+    static const int kAll[] = {32, 25, 20, 16, 10, 8, 5, 4, 3, 2};
+    static const int kBasic[] = {8, 5, 4, 3, 2};
+    const int* set = composite ? kAll : kBasic;
+    const int nset = composite ? 10 : 5;
+    int best[kMaxStages], bn = 0, bT = 0;
+    int cur[kMaxStages];
+    // depth-first over non-increasing radix sequences
+    auto dfs = [&](auto&& self, int n, int depth, int from) -> void {
+        if (n == 1) {
+            const int T = radices_threads(L, cur, depth, maxpts);
+            if (bn == 0 || depth < bn || (depth == bn && T < bT)) {
+                bn = depth;
+                bT = T;
+                for (int i = 0; i < depth; i++) best[i] = cur[i];
+            }
+            return;
+        }
+        if (depth >= kMaxStages || (bn != 0 && depth >= bn)) return;
+        for (int j = from; j < nset; j++) {
+            const int R = set[j];
+            if (R > maxpts || n % R) continue;
+            cur[depth] = R;
+            self(self, n / R, depth + 1, j);
+        }
+    };
+    dfs(dfs, L, 0, 0);
+    if (bn == 0) return false;
+    int ns = 1, twl = 0;
+    sp.L = L;
+    sp.nst = bn;
+    for (int s = 0; s < bn; s++) {
+        const int R = best[s];
+        sp.radix[s] = R;
+        sp.ns[s] = ns;
+        sp.mag[s] = ns > 1 ? (unsigned)(((1ULL << 32) + (unsigned long long)ns - 1) / (unsigned long long)ns) : 0u;
+        sp.twoff[s] = twl;
+        twl += ns * radix_ntw(R);
+        ns *= R;
+    }
+    sp.twlen = twl;
+    return true;
+}
+
+static int subplan_threads(const SubPlan& sp, int maxpts)
+{
+    return radices_threads(sp.L, sp.radix, sp.nst, maxpts);
 }
 
 std::string plan_geometry(Plan& p, long long N)
@@ -1021,33 +1254,74 @@ std::string plan_geometry(Plan& p, long long N)
         best1 = d;
     }
     if (best1 < 2) return "no usable factorisation";
+    // tuning override: SWSPEC_N1 picks the column length when it divides M
+    if (const char* e = getenv("SWSPEC_N1")) {
+        const long long n1 = atoll(e);
+        if (n1 >= 2 && (M % n1) == 0 && smooth235(n1) && smooth235(M / n1) && M / n1 <= 16384) best1 = n1;
+    }
     p.N = N;
     p.M = M;
     p.N1 = (int)best1;
     p.N2 = (int)(M / best1);
     if (p.N2 > 16384) return "FFT length too large for two passes";
-    if (!subplan_radices(p.N1, p.sp1) || !subplan_radices(p.N2, p.sp2)) return "radix decomposition failed";
+    // composite radices per pass: SWSPEC_COMPOSITE bit 0 = pass 1, bit 1 = pass 2.
+    // Default pass 1 only: for 1280-point rows 16*16*5 ran slower than 8*8*5*4
+    // (one stage fewer but 16-point butterflies leave 80 of 86 threads busy in
+    // two stages, and the row loop is load-bound).
+    int cxmask = 1;
+    if (const char* e = getenv("SWSPEC_COMPOSITE")) cxmask = atoi(e) & 3;
+    // sub-plan for (maxpts, composite); composite only where it saves a stage
+    auto subplan = [&](int L, int maxpts, bool cx, SubPlan& out, int& use_cx) -> bool {
+        SubPlan b;
+        if (!subplan_radices(L, b, maxpts, false)) return false;
+        out = b;
+        use_cx = 0;
+        if (cx) {
+            SubPlan c;
+            if (subplan_radices(L, c, maxpts, true) && c.nst < b.nst) { out = c; use_cx = 1; }
+        }
+        return true;
+    };
+    SubPlan c16, c32;
+    int cx16 = 0, cx32 = 0;
+    if (!subplan(p.N1, 16, cxmask & 1, c16, cx16) || !subplan(p.N1, 32, cxmask & 1, c32, cx32))
+        return "radix decomposition failed";
 
     // pass 1: B columns of N1 points (B a power of two dividing N2), LDS budget
-    // 80 KB, <= 512 threads; more points per thread when that buys columns
+    // 80 KB, <= 512 threads; 32 points per thread when that saves an LDS stage
+    // without losing columns, or when 16 leaves fewer than 4 columns and 32
+    // buys more
     auto pick_B = [&](int T1) {
         for (int c : {16, 8, 4, 2, 1})
             if ((p.N2 % c) == 0 && (size_t)c * p.N1 * sizeof(float2) <= 80 * 1024 && c * T1 <= 512) return c;
         return 0;
     };
-    int T1 = subplan_threads(p.sp1, 16);
+    int T1 = subplan_threads(c16, 16);
     int B = pick_B(T1);
     p.p1_maxpts = 16;
+    p.sp1 = c16;
+    p.p1_cx = cx16;
     {
-        const int T1b = subplan_threads(p.sp1, 32);
+        const int T1b = subplan_threads(c32, 32);
         const int Bb = pick_B(T1b);
-        // 32 points/thread halves occupancy (~230 VGPRs): only when 16 leaves B < 4
-        if (Bb > B && B < 4) { B = Bb; T1 = T1b; p.p1_maxpts = 32; }
+        if ((Bb > B && B < 4) || (Bb >= B && c32.nst < c16.nst)) {
+            B = Bb;
+            T1 = T1b;
+            p.p1_maxpts = 32;
+            p.sp1 = c32;
+            p.p1_cx = cx32;
+        }
     }
     // tuning overrides: SWSPEC_P1_MAXPTS (16|32), SWSPEC_P1_BMAX (upper bound on B)
     if (const char* e = getenv("SWSPEC_P1_MAXPTS")) {
         const int mp = atoi(e);
-        if (mp == 16 || mp == 32) { p.p1_maxpts = mp; T1 = subplan_threads(p.sp1, mp); B = pick_B(T1); }
+        if (mp == 16 || mp == 32) {
+            p.p1_maxpts = mp;
+            p.sp1 = mp == 16 ? c16 : c32;
+            p.p1_cx = mp == 16 ? cx16 : cx32;
+            T1 = subplan_threads(p.sp1, mp);
+            B = pick_B(T1);
+        }
     }
     if (const char* e = getenv("SWSPEC_P1_BMAX")) {
         const int bm = atoi(e);
@@ -1062,13 +1336,19 @@ std::string plan_geometry(Plan& p, long long N)
 
     // pass 2: 2 rows (x2 for cross-power pairs) of N2 points, <= 16 bins/thread,
     // <= 512 threads with cross-power
+    SubPlan r16, r32;
+    int rx16 = 0, rx32 = 0;
+    if (!subplan(p.N2, 16, cxmask & 2, r16, rx16) || !subplan(p.N2, 32, cxmask & 2, r32, rx32))
+        return "radix decomposition failed";
     p.p2_maxpts = 16;
     if (const char* e = getenv("SWSPEC_P2_MAXPTS")) {
         const int mp = atoi(e);
         if (mp == 16 || mp == 32) p.p2_maxpts = mp;
     }
+    p.sp2 = p.p2_maxpts == 16 ? r16 : r32;
+    p.p2_cx = p.p2_maxpts == 16 ? rx16 : rx32;
     int T2 = subplan_threads(p.sp2, p.p2_maxpts);
-    if (4 * T2 > 512) { p.p2_maxpts = 32; T2 = subplan_threads(p.sp2, 32); }
+    if (4 * T2 > 512) { p.p2_maxpts = 32; p.sp2 = r32; p.p2_cx = rx32; T2 = subplan_threads(p.sp2, 32); }
     if (4 * T2 > 512) return "row FFT too large";
     p.sp2.T = T2;
     p.p2_threads = 2 * T2;
@@ -1078,7 +1358,9 @@ std::string plan_geometry(Plan& p, long long N)
     p.p1_fpw = std::max(1, std::min(256 / p.p1_threads,
                                     (int)((64 * 1024) / ((size_t)p.N1 * p.B * sizeof(float2)))));
     // accumulator slices: up to 1024, at most 16 MiB of slices per job
-    p.smax = (int)std::max<long long>(1, std::min<long long>(1024, (16LL << 20) / (8 * (p.M + 1))));
+    long long slice_mb = 16;
+    if (const char* e = getenv("SWSPEC_SLICE_MB")) slice_mb = std::max(1LL, atoll(e));
+    p.smax = (int)std::max<long long>(1, std::min<long long>(1024, (slice_mb << 20) / (8 * (p.M + 1))));
     return "";
 }
 
@@ -1087,13 +1369,21 @@ static void stage_twiddles(const SubPlan& sp, std::vector<float2>& out)
     out.assign(std::max(sp.twlen, 1), make_float2(1.f, 0.f));
     for (int s = 0; s < sp.nst; s++) {
         const int R = sp.radix[s], ns = sp.ns[s];
+        const int A = radix_a(R), Bc = R / A, nt = radix_ntw(R);
         const long long D = (long long)ns * R;
-        for (int k = 0; k < ns; k++)
-            for (int r = 1; r < R; r++) {
-                const long long e = ((long long)k * r) % D;
-                const double a = -2.0 * M_PI * (double)e / (double)D;
-                out[sp.twoff[s] + k * (R - 1) + (r - 1)] = make_float2((float)cos(a), (float)sin(a));
+        auto put = [&](int k, int slot, long long r) {
+            const long long e = ((long long)k * r) % D;
+            const double a = -2.0 * M_PI * (double)e / (double)D;
+            out[sp.twoff[s] + k * nt + slot] = make_float2((float)cos(a), (float)sin(a));
+        };
+        for (int k = 0; k < ns; k++) {
+            if (A == R) {
+                for (int r = 1; r < R; r++) put(k, r - 1, r);          // W_D^{k r}
+            } else {
+                for (int n1 = 1; n1 < A; n1++) put(k, n1 - 1, (long long)Bc * n1);   // W_D^{k B n1}
+                for (int n2 = 1; n2 < Bc; n2++) put(k, A - 1 + n2 - 1, n2);          // W_D^{k n2}
             }
+        }
     }
 }
 
