@@ -218,9 +218,11 @@ template <int R> struct DftStore<R, true> {
 };
 
 // ---------------------------------------------------------------------------
-// Stockham stage in LDS.  Element e of the FFT lives at lds[ebase + e*estr].
+// Stockham stage in LDS.  Element e of the FFT lives at lds[ebase + (e << lge)]
+// (lge = log2 of the element stride: columns interleaved in pass 1).
 // Butterfly q (q < L/R) reads q + r*L/R, writes (q - q%ns)*R + q%ns + r*ns.
 // Every thread of the workgroup must call it (it contains barriers).
+// Index products use 24-bit multiplies (full rate; every factor < 2^24).
 // ---------------------------------------------------------------------------
 
 // q % ns via a multiply-high with mag = ceil(2^32/ns): exact while q*ns < 2^32
@@ -228,34 +230,37 @@ template <int R> struct DftStore<R, true> {
 __device__ __forceinline__ int mod_magic(int q, int ns, unsigned mag)
 {
     const int d = (int)__umulhi((unsigned)q, mag);
-    return q - d * ns;
+    return q - (int)__umul24((unsigned)d, (unsigned)ns);
 }
 
 template <int R, int NB>
 __device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int ns, unsigned mag,
-                                          const float2* __restrict__ tw, int ebase, int estr,
+                                          const float2* __restrict__ tw, int ebase, int lge,
                                           int q0, int T)
 {
     const int nb = L / R;
+    const int rstr = nb << lge;    // uniform
     float2 v[NB][R];
 #pragma unroll
     for (int i = 0; i < NB; i++) {
         const int q = q0 + i * T;
         if (q < nb) {
+            const float2* __restrict__ src = lds + ebase + (q << lge);
 #pragma unroll
-            for (int r = 0; r < R; r++) v[i][r] = lds[ebase + (q + r * nb) * estr];
+            for (int r = 0; r < R; r++) v[i][r] = src[r * rstr];
         }
     }
     __syncthreads();
+    const int dstr = ns << lge;    // uniform
 #pragma unroll
     for (int i = 0; i < NB; i++) {
         const int q = q0 + i * T;
         if (q < nb) {
             const int k = (ns > 1) ? mod_magic(q, ns, mag) : 0;
-            const int o = (q - k) * R + k;
-            float2* __restrict__ dst = lds + ebase + o * estr;
-            const int dstr = ns * estr;
-            DftStore<R>::run(v[i], tw + k * radix_ntw(R), ns > 1, [&](int r, float2 x) { dst[r * dstr] = x; });
+            const int o = (int)__umul24((unsigned)(q - k), (unsigned)R) + k;
+            float2* __restrict__ dst = lds + ebase + (o << lge);
+            DftStore<R>::run(v[i], tw + (int)__umul24((unsigned)k, (unsigned)radix_ntw(R)), ns > 1,
+                             [&](int r, float2 x) { dst[r * dstr] = x; });
         }
     }
     __syncthreads();
@@ -266,7 +271,7 @@ __device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int n
 // live in their own instantiations: a plan uses them only where they pay.
 template <int MAXPTS, bool CX>
 __device__ __forceinline__ void fft_lds(float2* lds, const SubPlan& sp, const float2* __restrict__ stw,
-                                        int ebase, int estr, int q0, int T)
+                                        int ebase, int lge, int q0, int T)
 {
     for (int s = 0; s < sp.nst; s++) {
         const float2* tw = stw + sp.twoff[s];
@@ -277,25 +282,25 @@ __device__ __forceinline__ void fft_lds(float2* lds, const SubPlan& sp, const fl
         int q0s = q0, ebs = ebase;
         asm volatile("" : "+v"(q0s), "+v"(ebs));
         switch (sp.radix[s]) {
-        case 2: stage_lds<2, MAXPTS / 2>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T); break;
-        case 3: stage_lds<3, MAXPTS / 3>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T); break;
-        case 4: stage_lds<4, MAXPTS / 4>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T); break;
-        case 5: stage_lds<5, MAXPTS / 5>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T); break;
-        case 8: stage_lds<8, MAXPTS / 8>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T); break;
+        case 2: stage_lds<2, MAXPTS / 2>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T); break;
+        case 3: stage_lds<3, MAXPTS / 3>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T); break;
+        case 4: stage_lds<4, MAXPTS / 4>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T); break;
+        case 5: stage_lds<5, MAXPTS / 5>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T); break;
+        case 8: stage_lds<8, MAXPTS / 8>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T); break;
         case 10:
-            if constexpr (CX) stage_lds<10, MAXPTS / 10>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T);
+            if constexpr (CX) stage_lds<10, MAXPTS / 10>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T);
             break;
         case 16:
-            if constexpr (CX) stage_lds<16, MAXPTS / 16>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T);
+            if constexpr (CX) stage_lds<16, MAXPTS / 16>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T);
             break;
         case 20:
-            if constexpr (CX && MAXPTS >= 20) stage_lds<20, MAXPTS / 20>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T);
+            if constexpr (CX && MAXPTS >= 20) stage_lds<20, MAXPTS / 20>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T);
             break;
         case 25:
-            if constexpr (CX && MAXPTS >= 25) stage_lds<25, MAXPTS / 25>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T);
+            if constexpr (CX && MAXPTS >= 25) stage_lds<25, MAXPTS / 25>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T);
             break;
         case 32:
-            if constexpr (CX && MAXPTS >= 32) stage_lds<32, MAXPTS / 32>(lds, sp.L, ns, mag, tw, ebs, estr, q0s, T);
+            if constexpr (CX && MAXPTS >= 32) stage_lds<32, MAXPTS / 32>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T);
             break;
         default: break;
         }
@@ -429,19 +434,25 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     const float wa0 = wg.kind ? wg.a0 : 1.f, wa1 = wg.kind ? wg.a1 : 0.f, wa2 = wg.kind ? wg.a2 : 0.f,
                 wa3 = wg.kind ? wg.a3 : 0.f;
 
-    // lp[e*B + b] = z[N2*e + c0 + b] windowed; idx == e*B + b
+    // lp[e*B + b] = z[N2*e + c0 + b] windowed; idx == e*B + b.  A thread keeps
+    // its column b = bcol; its row e advances by estep per strided step.  Rows
+    // are clamped to N1 - 1 for the loads of a partial batch (values unused).
+    // Point indices are < M <= 2^30 and the factors < 2^14: 24-bit products.
     const int total = N1 * B;
-    const long long cb = c0 + bcol;
+    const int cb = c0 + bcol;
+    const int estep = nth >> lgB;
     auto fill = [&](auto mode) {
         constexpr int MODE = decltype(mode)::value;
         constexpr int U = 8;
         for (int base = tid; base < total; base += U * nth) {
             unsigned f0[U], f1[U];
             float2 ph[U];
+            const int e0 = base >> lgB;
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int e = batch_idx<U>(base, u, nth, total) >> lgB;
-                fields_pair<MODE>(raw, up, (long long)N2 * e + cb, f0[u], f1[u]);
+                const int e = min(e0 + u * estep, N1 - 1);
+                fields_pair<MODE>(raw, up, (long long)(__umul24((unsigned)N2, (unsigned)e) + (unsigned)cb), f0[u],
+                                  f1[u]);
                 ph[u] = wg.phi[e];
             }
 #pragma unroll
@@ -465,26 +476,28 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     }
     __syncthreads();
 
-    fft_lds<MAXPTS, CX>(lds, a.sp, a.stw, pipe * (N1 * B) + bcol, B, tid >> lgB, nth >> lgB);
+    fft_lds<MAXPTS, CX>(lds, a.sp, a.stw, pipe * (N1 * B) + bcol, lgB, tid >> lgB, nth >> lgB);
 
     float2* __restrict__ Y = J.Y + (long long)(active ? f : 0) * a.M;
     const unsigned mask = (1u << a.twshift) - 1u;
     constexpr int U = 8;
+    // Y[k1*N2 + n2] = lp[k1*B + bcol] * W_M^{n2 k1}, n2 = cb fixed per thread
     for (int base = tid; active && base < total; base += U * nth) {
         float2 tA[U], tB[U];
+        const int k0 = base >> lgB;
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int idx = batch_idx<U>(base, u, nth, total);
-            const unsigned e = (unsigned)(c0 + (idx & (B - 1))) * (unsigned)(idx >> lgB);   // n2 k1 < M
+            const int k1 = min(k0 + u * estep, N1 - 1);
+            const unsigned e = __umul24((unsigned)cb, (unsigned)k1);   // n2 k1 < M
             tA[u] = a.twA[e >> a.twshift];
             tB[u] = a.twB[e & mask];
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int idx = base + u * nth;
-            if (idx < total) {
-                const int k1 = idx >> lgB, n2 = c0 + (idx & (B - 1));
-                Y[(long long)k1 * N2 + n2] = cmul(lp[idx], cmul(tA[u], tB[u]));
+            const int k1 = k0 + u * estep;
+            if (k1 < N1) {
+                const unsigned o = __umul24((unsigned)k1, (unsigned)N2) + (unsigned)cb;
+                Y[o] = cmul(lp[base + u * nth], cmul(tA[u], tB[u]));
             }
         }
     }
@@ -576,6 +589,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
     const int nfj = a.jobs[grp * NSRC].nf;   // FFTs of this job (pair) in the chunk
     const int per = (nfj + a.S - 1) / a.S;   // FFTs per slice (uniform trip count)
     const int fb = sl_i * per;
+    constexpr int U = 16 / NSL;
     for (int it = 0; it < per; it++) {
         const int f = fb + it;
         const bool act = sact && f < nfj;
@@ -585,7 +599,6 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
             for (int sl = 0; sl < NSL; sl++)
                 Ys[sl] = Yp[sl >> 1] + (long long)f * a.M + (long long)((sl & 1) ? rowB : rowA) * N2;
             // all rows' loads of a batch in flight together
-            constexpr int U = 16 / NSL;
             for (int base = tid; base < N2; base += U * nth) {
                 float2 t[NSL][U];
 #pragma unroll
@@ -600,7 +613,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
             }
         }
         __syncthreads();
-        fft_lds<MAXPTS, CX>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * N2, 1, q0, T);
+        fft_lds<MAXPTS, CX>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * N2, 0, q0, T);
 
         // each thread keeps the same bins for every FFT of the slice
         for (int idx = tid; act && idx < nbins; idx += nth) {
@@ -1023,17 +1036,14 @@ hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const Xpol
     const dim3 block(a.fpw * a.pth);
     const size_t lds = lds1 * a.fpw;
     if (lds > 160 * 1024) return hipErrorInvalidValue;
-    switch (p.p2_maxpts * 4 + (nsrc - 1) * 2 + (p.p2_cx ? 1 : 0)) {
-    case 64: hipLaunchKernelGGL((k_pass2<16, 1, false>), grid, block, lds, s, a); break;
-    case 65: hipLaunchKernelGGL((k_pass2<16, 1, true>), grid, block, lds, s, a); break;
-    case 66: hipLaunchKernelGGL((k_pass2<16, 2, false>), grid, block, lds, s, a); break;
-    case 67: hipLaunchKernelGGL((k_pass2<16, 2, true>), grid, block, lds, s, a); break;
-    case 128: hipLaunchKernelGGL((k_pass2<32, 1, false>), grid, block, lds, s, a); break;
-    case 129: hipLaunchKernelGGL((k_pass2<32, 1, true>), grid, block, lds, s, a); break;
-    case 130: hipLaunchKernelGGL((k_pass2<32, 2, false>), grid, block, lds, s, a); break;
-    case 131: hipLaunchKernelGGL((k_pass2<32, 2, true>), grid, block, lds, s, a); break;
-    default: return hipErrorInvalidValue;
-    }
+    using K = void (*)(P2Args);
+    // [maxpts 16|32][nsrc 1|2][composite]
+    static const K table[2][2][2] = {
+        {{k_pass2<16, 1, false>, k_pass2<16, 1, true>}, {k_pass2<16, 2, false>, k_pass2<16, 2, true>}},
+        {{k_pass2<32, 1, false>, k_pass2<32, 1, true>}, {k_pass2<32, 2, false>, k_pass2<32, 2, true>}}};
+    if (p.p2_maxpts != 16 && p.p2_maxpts != 32) return hipErrorInvalidValue;
+    const K k = table[p.p2_maxpts == 32][nsrc - 1][p.p2_cx ? 1 : 0];
+    hipLaunchKernelGGL(k, grid, block, lds, s, a);
     return hipGetLastError();
 }
 
@@ -1219,6 +1229,15 @@ static bool subplan_radices(int L, SubPlan& sp, int maxpts, bool composite)
     };
     dfs(dfs, L, 0, 0);
     if (bn == 0) return false;
+    // Stage order: odd radices first (largest first), then even ones smallest
+    // first.  A first stage (ns = 1) writes with a stride of R elements, which
+    // for an even R maps many lanes of a b64 access onto few LDS banks; an odd
+    // stride spreads them.  (Order does not change the arithmetic.)
+    std::stable_sort(best, best + bn, [](int x, int y) {
+        const bool ox = (x & 1) != 0, oy = (y & 1) != 0;
+        if (ox != oy) return ox;
+        return ox ? x > y : x < y;
+    });
     int ns = 1, twl = 0;
     sp.L = L;
     sp.nst = bn;
@@ -1264,11 +1283,8 @@ std::string plan_geometry(Plan& p, long long N)
     p.N1 = (int)best1;
     p.N2 = (int)(M / best1);
     if (p.N2 > 16384) return "FFT length too large for two passes";
-    // composite radices per pass: SWSPEC_COMPOSITE bit 0 = pass 1, bit 1 = pass 2.
-    // Default pass 1 only: for 1280-point rows 16*16*5 ran slower than 8*8*5*4
-    // (one stage fewer but 16-point butterflies leave 80 of 86 threads busy in
-    // two stages, and the row loop is load-bound).
-    int cxmask = 1;
+    // composite radices per pass: SWSPEC_COMPOSITE bit 0 = pass 1, bit 1 = pass 2
+    int cxmask = 3;
     if (const char* e = getenv("SWSPEC_COMPOSITE")) cxmask = atoi(e) & 3;
     // sub-plan for (maxpts, composite); composite only where it saves a stage
     auto subplan = [&](int L, int maxpts, bool cx, SubPlan& out, int& use_cx) -> bool {
@@ -1287,14 +1303,27 @@ std::string plan_geometry(Plan& p, long long N)
     if (!subplan(p.N1, 16, cxmask & 1, c16, cx16) || !subplan(p.N1, 32, cxmask & 1, c32, cx32))
         return "radix decomposition failed";
 
-    // pass 1: B columns of N1 points (B a power of two dividing N2), LDS budget
-    // 80 KB, <= 512 threads; 32 points per thread when that saves an LDS stage
-    // without losing columns, or when 16 leaves fewer than 4 columns and 32
-    // buys more
+    // pass 1: B columns of N1 points (B a power of two dividing N2), <= 512
+    // threads, LDS <= 40 KB per workgroup (80 KB when that is the only way to 4
+    // columns).  Resident waves per CU are set by LDS (about 20K points per CU)
+    // whatever B is, so smaller workgroups only add independent barrier domains
+    // (3.2M: B = 4 beat B = 8 by 4%).  SWSPEC_P1_LDS_KB overrides the budget.
+    // 32 points per thread when that saves an LDS stage without losing
+    // columns, or when 16 leaves fewer than 4 columns and 32 buys more.
+    static const size_t p1_lds_env = [] {
+        const char* e = getenv("SWSPEC_P1_LDS_KB");
+        const long v = e ? atol(e) : 0;
+        return (size_t)((v >= 4 && v <= 160) ? v : 0) * 1024;
+    }();
     auto pick_B = [&](int T1) {
-        for (int c : {16, 8, 4, 2, 1})
-            if ((p.N2 % c) == 0 && (size_t)c * p.N1 * sizeof(float2) <= 80 * 1024 && c * T1 <= 512) return c;
-        return 0;
+        auto fit = [&](size_t budget) {
+            for (int c : {16, 8, 4, 2, 1})
+                if ((p.N2 % c) == 0 && (size_t)c * p.N1 * sizeof(float2) <= budget && c * T1 <= 512) return c;
+            return 0;
+        };
+        if (p1_lds_env) return fit(p1_lds_env);
+        const int b40 = fit(40 * 1024);
+        return b40 >= 4 ? b40 : std::max(b40, fit(80 * 1024));
     };
     int T1 = subplan_threads(c16, 16);
     int B = pick_B(T1);
