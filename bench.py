@@ -85,6 +85,28 @@ def flops_per_sample(plan):
     return {"pass1": 1.0 + 2.5 * l1, "pass2": 2.0 + 2.5 * l2, "pcal": 4.0, "finalize": 0.0}
 
 
+def profiled_traffic(kernel, plan, jobs_per_launch):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary under profiles/ (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by
+    what the same counters read for a plain 1 GiB copy).  Used only when the
+    summary's plan and FFT-jobs per launch match this run; else None."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as fh:
+            for line in fh:
+                m = re.match(r"\s*traffic:\s*(\{.*\})\s*$", line)
+                if not m:
+                    continue
+                rec = json.loads(m.group(1))
+                if rec.get("kernel") != kernel or rec.get("fft_jobs_per_launch") != jobs_per_launch:
+                    continue
+                if all(plan.get(k) == v for k, v in rec.get("plan", {}).items()):
+                    rec["source"] = os.path.relpath(path, ROOT)
+                    return rec
+    return None
+
+
 def cpu_baseline(raw_host, threads, ffts_per_thread=12):
     """FP32 restatement of TaskCoreIPP::doMaths (oracle/) on `threads` threads,
     each a core object working on its own 3.2M-point FFTs of one channel."""
@@ -233,6 +255,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="FFTs per chunk (0 = automatic)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-stream batch")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving replay")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain per-channel leg (profiling runs: only the measured launches)")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="Poisson arrival window")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (3x): the reference point that scales "
@@ -300,8 +324,16 @@ def main():
                              "avg_launch_us": ms / n * 1e3, "algorithmic_tflops": tf}
     dom = max(per_kernel, key=lambda k: per_kernel[k]["ms_per_step"])
     ach = per_kernel[dom]["algorithmic_tflops"]
+    jobs_per_launch = int(round(NCH * NFFT / per_kernel[dom]["launches_per_step"]))
+    tr = profiled_traffic("k_" + dom, plan, jobs_per_launch)
     roofline = {"bound": "valu", "kernel": "k_" + dom, "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4),
+                "traffic": tr["bytes_per_launch"] if tr else None,
+                "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                "traffic_source": tr["source"] if tr else None,
+                "traffic_gbps_at_measured_launch": round(tr["bytes_per_launch"] / (per_kernel[dom]["avg_launch_us"] * 1e3), 1)
+                if tr else None,
+                "fft_jobs_per_launch": jobs_per_launch,
                 "flop_per_sample": round(fps[dom], 3)}
 
     # sanity on the output: spectra finite, tone comb visible in the phase-cal
@@ -309,8 +341,10 @@ def main():
     assert np.isfinite(a0).all()
     core.finalize()
 
-    log("plain per-channel cores ...")
-    plain_value, plain_ms = plain_cfg2(torch, swspec_amd, raw, s)
+    plain_value = plain_ms = None
+    if not args.no_plain:
+        log("plain per-channel cores ...")
+        plain_value, plain_ms = plain_cfg2(torch, swspec_amd, raw, s)
     batch = serve = None
     if not args.no_batch:
         log("64-stream batch ...")
@@ -337,9 +371,10 @@ def main():
         "roofline": roofline,
         "kernels": per_kernel,
         "cpu_baseline": cpu,
-        "plain": {"value": round(plain_value, 3), "ms_per_step": round(plain_ms, 3),
-                  "how": "4 single-channel cores, sequential run/join (one per UseFile1Channel)"},
-        "batched_vs_plain": round(value / plain_value, 2),
+        "plain": None if plain_value is None else {
+            "value": round(plain_value, 3), "ms_per_step": round(plain_ms, 3),
+            "how": "4 single-channel cores, sequential run/join (one per UseFile1Channel)"},
+        "batched_vs_plain": None if plain_value is None else round(value / plain_value, 2),
         "batch_64_streams": batch,
         "serving": serve,
     }
