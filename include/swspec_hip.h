@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SWSPEC_HIP_ABI_VERSION 1
+#define SWSPEC_HIP_ABI_VERSION 2
 
 /* error codes */
 #define SWSPEC_OK 0
@@ -74,6 +74,9 @@ typedef struct swspec_hip_cfg {
     int64_t  max_rawbuf_bytes;     /* largest raw buffer run() will be given */
     uint32_t flags;                /* SWSPEC_FLAG_* */
     uint32_t ffts_per_chunk;       /* 0 = automatic */
+    char     mark5_format[32];     /* cfg->sourceformat_str for the mark5access formats
+                                      (VLBA: "VLBA1_<fanout>-<Mbps>-<nchan>-<nbit>[/<decim>]",
+                                      mark5_stream.c:556-571); NUL-terminated */
 } swspec_hip_cfg;
 
 typedef struct swspec_hip_core swspec_hip_core;

@@ -90,6 +90,12 @@ def lib():
         L.so_pcal_reference64.argtypes = [P, C.c_long, C.c_double, C.c_double, C.c_double, C.c_uint64, P]
         L.so_mk5b_strip.restype = C.c_long
         L.so_mk5b_strip.argtypes = [P, C.c_long, P]
+        L.so_vlba_modulate.argtypes = [P]
+        L.so_vlba_modulate.restype = None
+        L.so_vlba_blank.argtypes = [P, C.c_long, C.c_long, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.so_vlba_blank.restype = None
+        L.so_vlba_unpack.argtypes = [P, C.c_long, C.c_size_t, C.c_int, P]
+        L.so_vlba_unpack.restype = C.c_size_t
         _lib = L
     return _lib
 
@@ -109,7 +115,8 @@ def make_ini(**kw):
         if k == "window" and isinstance(v, str):
             v = WINDOWS[v]
         if k == "source_format" and isinstance(v, str):
-            v = FORMATS[v]
+            # FileSource::open (src/FileSource.cpp:41-106): mark5access names by prefix
+            v = FORMATS.get(v, FORMATS["VLBA"] if v.upper().startswith("VLBA") else -1)
         d[k] = v
     ini = Ini()
     for k, v in d.items():
@@ -242,3 +249,28 @@ def mk5b_strip(frames):
     if n < 0:
         raise ValueError("Mk5B sync word missing")
     return out[:n]
+
+
+VLBA_PAYLOAD = 20000
+VLBA_CHUNK = 40128
+
+
+def vlba_modulate():
+    out = np.zeros(VLBA_PAYLOAD, dtype=np.uint8)
+    lib().so_vlba_modulate(_p(out))
+    return out
+
+
+def vlba_blank(buf, seg, avail=None):
+    buf = np.ascontiguousarray(buf, dtype=np.uint8)
+    vs, ve = C.c_int(), C.c_int()
+    lib().so_vlba_blank(_p(buf), len(buf) if avail is None else avail, seg, C.byref(vs), C.byref(ve))
+    return vs.value, ve.value
+
+
+def vlba_unpack(raw, count, channel, avail=None):
+    """VLBAUnpacker::extract_samples for VLBA1_1-*-4-2 (one byte per time sample)."""
+    raw = np.ascontiguousarray(raw, dtype=np.uint8)
+    out = np.zeros(count, dtype=np.float32)
+    lib().so_vlba_unpack(_p(raw), len(raw) if avail is None else avail, count, channel, _p(out))
+    return out

@@ -168,6 +168,12 @@ int so_pick_unpacker(const so_settings* s)
         if (s->bits_per_sample == 2 && (s->source_channels == 4 || s->source_channels == 8)) return 3;
         return 0;
     }
+    if (s->source_format == SO_FMT_VLBA) {
+        /* VLBAUnpacker::canHandleConfig (:530-533); only 8 tracks, fanout 1,
+         * 2-bit (4 channels) is restated */
+        if (s->bits_per_sample == 2 && s->source_channels == 4) return 4;
+        return 0;
+    }
     return 0;
 }
 
@@ -202,6 +208,7 @@ size_t so_unpack(const so_settings* s, const uint8_t* src, float* dst, size_t co
         }
         return smp;
     }
+    if (kind == 4) return so_vlba_unpack(src, (long)count, count, channel, dst);
     return 0;
 }
 
@@ -682,7 +689,10 @@ int so_domaths(const so_settings* s, int precision,
     while (remain >= s->raw_fullfft_bytes) {
         for (int rs = 0; rs < nsrc; rs++) {
             int ch = rs == 0 ? s->use_channel_file1 : s->use_channel_file2;
-            so_unpack(s, src[rs], xs, (size_t)N, ch);
+            if (so_pick_unpacker(s) == 4)   /* the blanker may read past the FFT's bytes */
+                so_vlba_unpack(src[rs], (long)(len[rs] - (size_t)(src[rs] - src_in[rs])), (size_t)N, ch, xs);
+            else
+                so_unpack(s, src[rs], xs, (size_t)N, ch);
             src[rs] += s->raw_overlap_bytes;
             int do_pcal = s->extract_pcal && out_pcal_v && ((curr_ffts % s->fft_overlap_factor) == 0);
             if (precision == 32) {
@@ -759,6 +769,110 @@ int so_domaths(const so_settings* s, int precision,
     for (int r = 0; r < 2; r++) { free(Xf[r]); free(Xd[r]); }
     plan_free(p);
     return spectra;
+}
+
+/* ------------------------------------------------------------------------- */
+/* VLBA through mark5access 1.5.0 (vendored under mark5access/)              */
+/* ------------------------------------------------------------------------- */
+
+/* initmodulate (mark5access/mark5access/mark5_format_vlba.c:167-196): a
+ * 16-stage shift register starting all ones; each step shifts in
+ * b10 ^ b12 ^ b13 ^ b15 at b0 and that bit is the output; after every 8th
+ * output one further step is taken unrecorded (the track parity bit). */
+void so_vlba_modulate(uint8_t* out)
+{
+    unsigned reg = 0xFFFFu;
+    for (int i = 0; i < SO_VLBA_PAYLOAD; i++) {
+        for (int rep = 0; rep < ((i % 8 == 7) ? 2 : 1); rep++) {
+            const unsigned k = ((reg >> 10) ^ (reg >> 12) ^ (reg >> 13) ^ (reg >> 15)) & 1u;
+            reg = ((reg << 1) | k) & 0xFFFFu;
+            if (rep == 0) out[i] = (uint8_t)k;
+        }
+    }
+}
+
+/* Mark5 fill pattern (blanker_mark5.c:41-46, little-endian host): the 64-bit
+ * word 0x1122334411223344, i.e. bytes 44 33 22 11 repeated.  A word that is
+ * not wholly inside the caller's `avail` bytes counts as data (the reference
+ * reads past its buffer there). */
+static int vl_fill(const uint8_t* p, long off, int bytes, long avail)
+{
+    static const uint8_t pat[4] = {0x44, 0x33, 0x22, 0x11};
+    if (off < 0 || off + bytes > avail) return 0;
+    for (int b = 0; b < bytes; b++)
+        if (p[off + b] != pat[b & 3]) return 0;
+    return 1;
+}
+
+/* blanker_mark5 (blanker_mark5.c:126-200) for one 20000-byte VLBA payload
+ * (8 tracks: 2500 words, one 32 KiB zone) at byte `seg` of p: bytes i with
+ * vs <= i < ve are data, the rest decode to 0.  Both boundary searches
+ * (findfirstinvalid :48-81, findfirstvalid :83-116) are the same bisection
+ * over 64-bit words plus a 32-bit straggler check. */
+void so_vlba_blank(const uint8_t* p, long avail, long seg, int* vs, int* ve)
+{
+    const int nword = SO_VLBA_PAYLOAD / 8;
+    const int start_ok = !vl_fill(p, seg, 8, avail);
+    const int end_ok = !vl_fill(p, seg + 8L * (nword - 1), 8, avail);
+    if (start_ok && end_ok) { *vs = 0; *ve = 1 << 30; return; }
+    if (!start_ok && !end_ok) { *vs = 1 << 30; *ve = 0; return; }
+    /* bisection for the boundary word: `want_fill` = the kind of word that
+     * moves the boundary down */
+    const int want_fill = start_ok;   /* data then fill: find first fill word */
+    int i = nword / 2, r = nword - 1;
+    for (int s = nword / 4; s > 0; s >>= 1) {
+        const int isfill = vl_fill(p, seg + 8L * i, 8, avail);
+        if (isfill == want_fill) { r = i; i -= s; }
+        else i += s;
+    }
+    int byte = 8 * r;
+    if (start_ok) {
+        if (vl_fill(p, seg + byte - 4, 4, avail)) byte -= 4;   /* data32[r/4 - 1] */
+        *vs = 0;
+        *ve = byte;
+    } else {
+        if (vl_fill(p, seg + byte - 8, 4, avail)) byte += 4;   /* data32[r/4 - 2] */
+        *vs = byte;
+        *ve = 1 << 30;
+    }
+}
+
+/* VLBAUnpacker::extract_samples (src/IA-32/DataUnpackers.cpp:500-528) for
+ * VLBA1_1-<Mbps>-4-2 (8 tracks, fanout 1, 2-bit, mark5_format_vlba.c
+ * vlba_decode_2bit_8track_fanout1_decimation1 :4537-4583):
+ *  - the request is cut into chunks of 2*(framebytes - payloadoffset) =
+ *    2*(20160 - 96) = 40128 bytes; each chunk is one mark5_unpack call on an
+ *    unpacker stream without headers (mark5_stream_unpacker.c:112-129), so
+ *    the payload index and the modulation restart at every chunk (and at
+ *    every call, i.e. every FFT);
+ *  - inside a chunk a payload is 20000 bytes; the blanker runs at the start
+ *    of each payload (mark5_stream.c:147-196);
+ *  - byte b of payload index i gives channel c the level
+ *    {-HiMag, +1, -1, +HiMag}[(b >> 2c) & 3] (initluts :90-165), negated when
+ *    modulate[i] is 1, or 0 inside a blank zone. */
+size_t so_vlba_unpack(const uint8_t* src, long avail, size_t count, int channel, float* dst)
+{
+    static uint8_t mod[SO_VLBA_PAYLOAD];
+    static int mod_ready = 0;
+    if (!mod_ready) { so_vlba_modulate(mod); mod_ready = 1; }
+    const float himag = 3.3359f;   /* OPTIMAL_2BIT_HIGH, mark5_stream.h:72 */
+    const float lvl[4] = {-himag, 1.0f, -1.0f, himag};
+    const long chunk = SO_VLBA_CHUNK;
+    for (size_t c0 = 0; c0 < count; c0 += (size_t)chunk) {
+        const size_t n = (count - c0 < (size_t)chunk) ? count - c0 : (size_t)chunk;
+        int vs = 0, ve = 0;
+        for (size_t o = 0; o < n; o++) {
+            const int i = (int)(o % SO_VLBA_PAYLOAD);
+            if (i == 0) so_vlba_blank(src, avail, (long)(c0 + o), &vs, &ve);
+            float v = 0.0f;
+            if (i >= vs && i < ve) {
+                v = lvl[(src[c0 + o] >> (2 * (channel & 3))) & 3u];
+                if (mod[i]) v = -v;
+            }
+            dst[c0 + o] = v;
+        }
+    }
+    return count;
 }
 
 /* ------------------------------------------------------------------------- */

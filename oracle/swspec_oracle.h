@@ -109,7 +109,8 @@ extern const float so_map_rev[4];
 
 /* Which decoder the factory picks (src/IA-32/DataUnpackerFactory.h:38-108):
  * 1 = TwoBitUnpacker, 2 = TwoBitSinglechannelUnpacker, 3 = Mk5BUnpacker (4/8 ch),
- * 0 = none supported here (8/16-bit, MKIV/Mark5B via mark5access, VLBA). */
+ * 4 = VLBAUnpacker for VLBA1_1-*-4-2 (see so_vlba_unpack),
+ * 0 = none supported here (8/16-bit, MKIV/Mark5B via mark5access, other VLBA modes). */
 int so_pick_unpacker(const so_settings* s);
 
 /* Unpack `count` samples of 0-based `channel` starting at src.  Restates
@@ -188,6 +189,26 @@ int so_pcal_ntones(double bandwidth_hz, double spacing_hz, double offset_hz);
  * followed by so_pcal_tones64.  tones_out: ntones complex. */
 void so_pcal_reference64(const double* x, long len, double bandwidth_hz, double spacing_hz,
                          double offset_hz, uint64_t sampleoffset, double* tones_out);
+
+/* ---- VLBA through mark5access (VLBAUnpacker, DataUnpackers.cpp:459-535) ----
+ * Restated from the vendored mark5access 1.5.0 sources for the 8-track,
+ * fanout-1, 2-bit case ("VLBA1_1-<Mbps>-4-2"; 4 channels, one byte per time
+ * sample).  mark5access cannot be built here (its sources include an
+ * autotools-generated config.h), so these functions are pinned by closed
+ * forms and invariants only: the register's period and balance, literal LUT
+ * bytes, blank zones of synthetic fill patterns against a linear scan, and
+ * the per-chunk modulation restart (tests/test_oracle_vlba.py). */
+#define SO_VLBA_PAYLOAD 20000   /* data bytes per frame, 8 tracks */
+#define SO_VLBA_CHUNK 40128     /* 2 * (framebytes 20160 - payloadoffset 96) */
+/* modulate[0..19999] (mark5_format_vlba.c:167-196), values 0/1. */
+void so_vlba_modulate(uint8_t* out);
+/* Blank zone of the 20000-byte payload at byte `seg` of p (blanker_mark5.c:
+ * 126-200): data where vs <= i < ve.  Words beyond `avail` count as data. */
+void so_vlba_blank(const uint8_t* p, long avail, long seg, int* vs, int* ve);
+/* extract_samples of `count` samples of `channel` (0..3) from src; `avail`
+ * bytes are readable from src (the blanker looks up to 20000 bytes past the
+ * last chunk's payload start). */
+size_t so_vlba_unpack(const uint8_t* src, long avail, size_t count, int channel, float* dst);
 
 /* ---- Mk5B framing -----------------------------------------------------------
  * FileSource::read framed mode (src/FileSource.cpp:184-215): strip the 16-byte

@@ -9,7 +9,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libswspec_hip.so")
 
-SWSPEC_HIP_ABI_VERSION = 1
+SWSPEC_HIP_ABI_VERSION = 2
 FLAG_NO_KAHAN = 0x1
 
 WINDOWS = {"None": 0, "Cosine": 1, "Cosine2": 2, "Hamming": 3, "Hann": 4, "Blackman": 5}
@@ -53,7 +53,7 @@ class Cfg(C.Structure):
         ("extract_pcal", C.c_int32), ("pcal_tonebins", C.c_int32), ("pcal_rotatorlen", C.c_int32),
         ("device", C.c_int32), ("samplingfreq", C.c_double), ("pcaloffsethz", C.c_double),
         ("pcalharmonicshz", C.c_double), ("max_rawbuf_bytes", C.c_int64), ("flags", C.c_uint32),
-        ("ffts_per_chunk", C.c_uint32),
+        ("ffts_per_chunk", C.c_uint32), ("mark5_format", C.c_char * 32),
     ]
 
 

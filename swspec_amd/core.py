@@ -49,7 +49,25 @@ class Settings:
         c.device = device
         c.flags = flags
         c.ffts_per_chunk = ffts_per_chunk
+        f = self.keys["SourceFormat"]
+        if isinstance(f, str):
+            c.mark5_format = f.encode()[:31]
         return c
+
+
+def source_format_enum(name):
+    """FileSource::open's format selection (src/FileSource.cpp:41-106): exact
+    names, or the mark5access prefixes MKIV / VLBA / Mark5B (case-insensitive)."""
+    if not isinstance(name, str):
+        return int(name)
+    low = name.lower()
+    for k, v in FORMATS.items():
+        if k.lower() == low:
+            return v
+    for prefix in ("MKIV", "VLBA", "Mark5B"):
+        if low.startswith(prefix.lower()):
+            return FORMATS[prefix]
+    return -1
 
 
 def derive_settings(num_sources=1, **keys):
@@ -68,8 +86,7 @@ def derive_settings(num_sources=1, **keys):
     ini.bandwidth_hz = float(k["BandwidthHz"])
     ini.pcal_offset_hz = float(k["PCalOffsetHz"])
     ini.pcal_harmonics_hz = PCAL_HARMONICS_HZ
-    f = k["SourceFormat"]
-    ini.source_format = FORMATS.get(f, -1) if isinstance(f, str) else int(f)
+    ini.source_format = source_format_enum(k["SourceFormat"])
     ini.bits_per_sample = int(k["BitsPerSample"])
     ini.channelorder_increasing = 1 if k["ChannelOrderIncreasing"] else 0
     ini.source_channels = int(k["SourceChannels"])

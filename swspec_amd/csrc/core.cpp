@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <strings.h>
 #include <vector>
 
 using namespace swspec;
@@ -53,6 +54,9 @@ struct swspec_hip_core {
     double* d_pcal_part = nullptr;
     double* d_pcal_acc = nullptr;
     int ntones = 0, tone_step = 0;
+    bool vlba = false;             // some job decodes VLBA
+    uint32_t* d_vmod = nullptr;    // modulation bits
+    int2* d_vblank = nullptr;      // [job][FFT of the chunk][segment] blank zones
     // host-mode staging
     uint8_t* d_in[2] = {nullptr, nullptr};
     size_t d_in_cap = 0;
@@ -290,6 +294,26 @@ static int pick_unpack(const swspec_hip_cfg* c, int channel, Unpack& up, std::st
         up.shift = 2 * (channel % 4);
         return 0;
     }
+    if (c->source_format == SWSPEC_FMT_VLBA) {
+        // VLBAUnpacker (:459-535) over mark5access's format string
+        // "VLBA1_<fanout>-<Mbps>-<nchan>-<nbit>[/<decimation>]" (mark5_stream.c:556-571)
+        char fmt[sizeof(c->mark5_format) + 1];
+        std::memcpy(fmt, c->mark5_format, sizeof(c->mark5_format));
+        fmt[sizeof(c->mark5_format)] = 0;
+        int fan = 0, mbps = 0, nch = 0, nbit = 0, dec = 1;
+        const bool prefix = strncasecmp(fmt, "VLBA1_", 6) == 0;
+        const int r = prefix ? sscanf(fmt + 6, "%d-%d-%d-%d/%d", &fan, &mbps, &nch, &nbit, &dec) : 0;
+        if (r < 4) { why = "VLBA format string not understood: '" + std::string(fmt) + "'"; return -1; }
+        if (fan != 1 || nch != 4 || nbit != 2 || dec != 1) {
+            why = "VLBA decode implemented for VLBA1_1-<Mbps>-4-2 (8 tracks, fanout 1, 2-bit) only";
+            return -1;
+        }
+        if (c->source_channels != 4) { why = "VLBA1_1-*-4-2 needs SourceChannels = 4"; return -1; }
+        up.kind = kUnpackVLBA;
+        up.step = 1;
+        up.shift = 2 * (channel % 4);   // allchannels[channel % nchan] (:518)
+        return 0;
+    }
     why = "decoder not implemented for this format/channel count";
     return -1;
 }
@@ -317,6 +341,20 @@ static int core_alloc(swspec_hip_core* c)
         HIPCHK(c, (hipError_t)launch_zero(c->d_pcal_acc, (size_t)c->njobs * c->pg.T * 2 * sizeof(double),
                                           c->stream));
     }
+    if (c->vlba) {
+        uint32_t bits[kVlbaModWords];
+        vlba_modulation_bits(bits);
+        const int nseg = vlba_segments(c->cfg.fft_points);
+        HIPCHK(c, hipMalloc(&c->d_vmod, sizeof(bits)));
+        HIPCHK(c, hipMemcpy(c->d_vmod, bits, sizeof(bits), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMalloc(&c->d_vblank, (size_t)c->njobs * c->G * nseg * sizeof(int2)));
+        for (int j = 0; j < c->njobs; j++) {
+            if (c->up[j].kind != kUnpackVLBA) continue;
+            c->up[j].vmod = c->d_vmod;
+            c->up[j].vnseg = nseg;
+            c->up[j].vblank = c->d_vblank + (size_t)j * c->G * nseg;
+        }
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SWSPEC_OK;
 }
@@ -339,6 +377,7 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
         std::string why;
         c->job_src[j] = job_src[j];
         if (pick_unpack(cfg, job_ch[j], c->up[j], why) != 0) return fail(c, SWSPEC_EUNSUPPORTED, why);
+        c->vlba |= c->up[j].kind == kUnpackVLBA;
     }
     // bytes consumed per FFT must match the decoder layout
     {
@@ -375,6 +414,9 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
         c->pg.T = T;
         c->pg.L = L;
         pcal_geometry(c->pg, cfg->fft_points);
+        // the integer-count fold assumes the 2-bit field levels; VLBA (modulated,
+        // blanked samples) takes the generic fold
+        if (c->vlba) c->pg.fast = 0;
         std::vector<float2> rot(L);
         const double dphi = 2 * M_PI * (-cfg->pcaloffsethz / cfg->samplingfreq);   // :130-135
         for (int i = 0; i < L; i++) {
@@ -442,7 +484,8 @@ extern "C" int swspec_hip_finalize(swspec_hip_core* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     plan_destroy(c->plan);
     for (void* p : {(void*)c->d_rot, (void*)c->d_Y, (void*)c->d_acc, (void*)c->d_xacc, (void*)c->d_pcal_part,
-                    (void*)c->d_pcal_acc, (void*)c->d_in[0], (void*)c->d_in[1], (void*)c->d_out, (void*)c->d_tones})
+                    (void*)c->d_pcal_acc, (void*)c->d_in[0], (void*)c->d_in[1], (void*)c->d_out, (void*)c->d_tones,
+                    (void*)c->d_vmod, (void*)c->d_vblank})
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -508,7 +551,9 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
             bool any_pcal = false;
             for (int j = 0; j < c->njobs; j++) {
                 JobDev& J = jobs[j];
-                J.raw = d_in[c->job_src[j]] + w.start + (long long)f0 * g.raw_overlap_bytes;
+                const long long off = w.start + (long long)f0 * g.raw_overlap_bytes;
+                J.raw = d_in[c->job_src[j]] + off;
+                J.raw_len = (long long)len - off;
                 J.fft_stride = g.raw_overlap_bytes;
                 J.nf = nf;
                 J.up = c->up[j];
@@ -526,6 +571,7 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
             }
             {
                 ProfScope ps(c, SWSPEC_K_PASS1);
+                if (c->vlba) HIPCHK(c, launch_vlba_blank(jobs, c->njobs, nf, g.fft_points, c->stream));
                 HIPCHK(c, launch_pass1(p, jobs, c->njobs, nf, c->stream));
             }
             if (any_pcal) {
@@ -793,8 +839,30 @@ extern "C" int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d
     Unpack up;
     std::string why;
     if (pick_unpack(cfg, channel, up, why) != 0) return fail(nullptr, SWSPEC_EUNSUPPORTED, why);
-    hipError_t e = launch_unpack((const uint8_t*)d_raw, up, count, d_out, nullptr);
+    hipError_t e = hipSuccess;
+    uint32_t* d_mod = nullptr;
+    int2* d_blank = nullptr;
+    if (up.kind == kUnpackVLBA) {
+        // one extract_samples call over `count` bytes: its blank zones see only these bytes
+        uint32_t bits[kVlbaModWords];
+        vlba_modulation_bits(bits);
+        up.vnseg = vlba_segments(count);
+        e = hipMalloc(&d_mod, sizeof(bits));
+        if (e == hipSuccess) e = hipMemcpy(d_mod, bits, sizeof(bits), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&d_blank, (size_t)std::max(up.vnseg, 1) * sizeof(int2));
+        up.vmod = d_mod;
+        up.vblank = d_blank;
+        JobDev J;
+        J.raw = (const uint8_t*)d_raw;
+        J.raw_len = count;
+        J.nf = 1;
+        J.up = up;
+        if (e == hipSuccess && count > 0) e = launch_vlba_blank(&J, 1, 1, count, nullptr);
+    }
+    if (e == hipSuccess) e = launch_unpack((const uint8_t*)d_raw, up, count, d_out, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (d_mod) (void)hipFree(d_mod);
+    if (d_blank) (void)hipFree(d_blank);
     if (e != hipSuccess) return fail(nullptr, SWSPEC_EDEVICE, hipGetErrorString(e));
     return SWSPEC_OK;
 }

@@ -24,14 +24,36 @@ enum UnpackKind : int {
     kUnpackNone = 0,
     kUnpackMultiChannel = 1,   // TwoBitUnpacker / Mk5BUnpacker: one byte group per time sample
     kUnpackSingleChannel = 2,  // TwoBitSinglechannelUnpacker: 4 samples per byte, oldest in MSB
+    kUnpackVLBA = 3,           // VLBAUnpacker, VLBA1_1-*-4-2: 8 tracks, fanout 1, 2-bit, 1 byte per sample
 };
+
+// VLBA framing seen by VLBAUnpacker::extract_samples (DataUnpackers.cpp:500-528):
+// every call (one FFT) is cut into chunks of 2*(20160 - 96) bytes, each an
+// unpacker stream without headers whose 20000-byte payloads restart the
+// modulation index and get their own blank zone.
+constexpr int kVlbaPayload = 20000;
+constexpr int kVlbaChunk = 40128;
+constexpr int kVlbaSegPerChunk = 3;    // payloads at 0, 20000 and the 128-byte tail at 40000
+constexpr int kVlbaModWords = (kVlbaPayload + 31) / 32;
 
 struct Unpack {
     int kind = kUnpackNone;
     int step = 1;      // bytes per time sample (multi-channel)
     int cbyte = 0;     // byte offset of the channel inside a time sample
     int shift = 0;     // bit shift of the 2-bit field
+    // VLBA only
+    int vnseg = 0;                       // payload segments per FFT
+    const uint32_t* vmod = nullptr;      // modulation bits, payload index i at bit i
+    int2* vblank = nullptr;              // [FFT of the chunk][segment] data zone (start, end)
 };
+
+// Segments of an N-sample VLBA FFT.
+inline int vlba_segments(long long N)
+{
+    return (int)(((N + kVlbaChunk - 1) / kVlbaChunk) * kVlbaSegPerChunk);
+}
+// modulate[] of mark5_format_vlba.c:167-196 packed as bits.
+void vlba_modulation_bits(uint32_t* words /* kVlbaModWords */);
 
 // Stockham sub-FFT executed in LDS by one workgroup slice.
 struct SubPlan {
@@ -108,6 +130,7 @@ struct JobDev {
     double* pcal_acc = nullptr;    // pcal accumulator [T] double2
     unsigned long long pcal_mask = 0;  // bit f: FFT f of the chunk gets phase-cal
     int nf = 0;                    // FFTs of this job in the chunk (<= launch nfft)
+    long long raw_len = 0;         // bytes readable from raw (VLBA blank-zone probes past an FFT)
 };
 
 struct XpolDev {
@@ -157,5 +180,8 @@ hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int n
 hipError_t launch_pcal_emit(const PcalGeom& g, double* acc, float* out, double* tones, int ntones,
                             int tone_step, hipStream_t s);
 hipError_t launch_unpack(const uint8_t* raw, Unpack up, long long count, float* out, hipStream_t s);
+// VLBA jobs: blank zones of every payload segment of the chunk's FFTs
+// (blanker_mark5.c:126-200), written to each job's up.vblank.
+hipError_t launch_vlba_blank(const JobDev* jobs, int njobs, int nfft, long long N, hipStream_t s);
 
 }  // namespace swspec

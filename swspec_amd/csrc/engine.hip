@@ -328,6 +328,41 @@ __device__ __forceinline__ unsigned field2(const uint8_t* __restrict__ raw, cons
     return (raw[j * up.step + up.cbyte] >> up.shift) & 3u;
 }
 
+// VLBA level of 2-bit code l = (byte >> 2c) & 3 (mark5access initluts,
+// mark5_format_vlba.c:90-165): {-HiMag, +1, -1, +HiMag}[l], HiMag = 3.3359f
+// (mark5_stream.h:72).  Note the order differs from swspec's own map above.
+__device__ __forceinline__ float vlba_level(unsigned l)
+{
+    const float mag = ((l ^ (l >> 1)) & 1u) ? 1.0f : 3.3359f;
+    return (l & 1u) ? mag : -mag;
+}
+
+// Sample j of one extract_samples call (one FFT) for VLBA1_1-*-4-2: chunk
+// c = j / 40128, payload segment s (0, 1, or the tail 2) of the chunk, payload
+// index i; negated where modulate[i] is set, zero outside the segment's data
+// zone (vlba_decode_2bit_8track_fanout1_decimation1, mark5_format_vlba.c:
+// 4537-4583).  brow: the FFT's row of segment zones.
+__device__ __forceinline__ float vlba_sample(const uint8_t* __restrict__ raw, const Unpack& up,
+                                             const int2* __restrict__ brow, unsigned j)
+{
+    const unsigned c = j / (unsigned)kVlbaChunk;
+    const unsigned r = j - c * (unsigned)kVlbaChunk;
+    const unsigned s = r >= 2u * kVlbaPayload ? 2u : (r >= (unsigned)kVlbaPayload ? 1u : 0u);
+    const int i = (int)(r - s * (unsigned)kVlbaPayload);
+    const int2 z = brow[c * kVlbaSegPerChunk + s];
+    const float lv = vlba_level((raw[j] >> up.shift) & 3u);
+    const bool flip = (up.vmod[i >> 5] >> (i & 31)) & 1u;
+    return (i >= z.x && i < z.y) ? (flip ? -lv : lv) : 0.0f;
+}
+
+// Any decoder, sample j of the FFT starting at raw (brow: VLBA only).
+__device__ __forceinline__ float decode_sample(const uint8_t* __restrict__ raw, const Unpack& up,
+                                               const int2* __restrict__ brow, long long j)
+{
+    if (up.kind == kUnpackVLBA) return vlba_sample(raw, up, brow, (unsigned)j);
+    return map2bit(field2(raw, up, j));
+}
+
 // ---------------------------------------------------------------------------
 // pass 1: unpack + window + column FFTs + four-step twiddle
 // ---------------------------------------------------------------------------
@@ -441,18 +476,31 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     const int total = N1 * B;
     const int cb = c0 + bcol;
     const int estep = nth >> lgB;
+    // MODE 3 (VLBA): samples decoded through the FFT's row of blank zones
+    const int2* __restrict__ brow =
+        up.kind == kUnpackVLBA ? up.vblank + (size_t)(active ? f : 0) * up.vnseg : nullptr;
     auto fill = [&](auto mode) {
         constexpr int MODE = decltype(mode)::value;
-        constexpr int U = 8;
+        // VLBA decode holds more per sample (segment zone, modulation word):
+        // a smaller batch keeps it under the register budget of the 2-bit modes
+        constexpr int U = MODE == 3 ? 2 : 8;
         for (int base = tid; base < total; base += U * nth) {
-            unsigned f0[U], f1[U];
+            float x0[U], x1[U];
             float2 ph[U];
             const int e0 = base >> lgB;
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int e = min(e0 + u * estep, N1 - 1);
-                fields_pair<MODE>(raw, up, (long long)(__umul24((unsigned)N2, (unsigned)e) + (unsigned)cb), f0[u],
-                                  f1[u]);
+                const unsigned n = __umul24((unsigned)N2, (unsigned)e) + (unsigned)cb;
+                if constexpr (MODE == 3) {
+                    x0[u] = vlba_sample(raw, up, brow, 2u * n);
+                    x1[u] = vlba_sample(raw, up, brow, 2u * n + 1u);
+                } else {
+                    unsigned f0, f1;
+                    fields_pair<MODE>(raw, up, (long long)n, f0, f1);
+                    x0[u] = map2bit(f0);
+                    x1[u] = map2bit(f1);
+                }
                 ph[u] = wg.phi[e];
             }
 #pragma unroll
@@ -464,13 +512,14 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
                     const float cs1 = p.x * psi1.x - p.y * psi1.y, sn1 = p.y * psi1.x + p.x * psi1.y;
                     const float w0 = wa0 + wa1 * cs0 + wa2 * sn0 + wa3 * (2.f * cs0 * cs0 - 1.f);
                     const float w1 = wa0 + wa1 * cs1 + wa2 * sn1 + wa3 * (2.f * cs1 * cs1 - 1.f);
-                    lp[idx] = make_float2(map2bit(f0[u]) * w0, map2bit(f1[u]) * w1);
+                    lp[idx] = make_float2(x0[u] * w0, x1[u] * w1);
                 }
             }
         }
     };
     if (active) {
         if (up.kind == kUnpackSingleChannel) fill(std::integral_constant<int, 0>());
+        else if (up.kind == kUnpackVLBA) fill(std::integral_constant<int, 3>());
         else if (up.step == 1 && even) fill(std::integral_constant<int, 1>());
         else fill(std::integral_constant<int, 2>());
     }
@@ -792,11 +841,12 @@ __global__ void __launch_bounds__(1024) k_pcal(PcArgs a)
     if ((J.pcal_mask >> f) & 1ull) {
         const uint8_t* __restrict__ raw = job_fft_ptr(J, f);
         const Unpack up = J.up;
+        const int2* __restrict__ brow = up.kind == kUnpackVLBA ? up.vblank + (size_t)f * up.vnseg : nullptr;
         const long long s0 = (long long)blk * a.g.chunk;
         const long long s1 = min(s0 + (long long)a.g.chunk, a.N);
         int ri = (int)((s0 + tid) % L);
         for (long long j = s0 + tid; j < s1; j += nth) {
-            const float x = map2bit(field2(raw, up, j));
+            const float x = decode_sample(raw, up, brow, j);
             const float2 r = a.g.d_rot[ri];
             ar += (double)x * (double)r.x;
             ai += (double)x * (double)r.y;
@@ -959,11 +1009,98 @@ __global__ void k_pcal_emit(double* __restrict__ acc, float* __restrict__ out, d
     }
 }
 
+// Decoder only: samples [0, count) of one extract_samples call (VLBA: the
+// first row of up.vblank holds its blank zones).
 __global__ void k_unpack(const uint8_t* __restrict__ raw, Unpack up, long long count, float* __restrict__ out)
 {
     for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < count;
          j += (long long)gridDim.x * blockDim.x)
-        out[j] = map2bit(field2(raw, up, j));
+        out[j] = decode_sample(raw, up, up.vblank, j);
+}
+
+// ---------------------------------------------------------------------------
+// VLBA blank zones (mark5access blanker_mark5.c:126-200) for every 20000-byte
+// payload segment of the chunk's FFTs.  One thread per segment: the first and
+// last 64-bit words decide; when exactly one is the fill word the boundary is
+// found by the reference's bisection over words (findfirstinvalid :48-81,
+// findfirstvalid :83-116) with its 32-bit straggler adjustment, so results
+// match the reference where that bisection misses the boundary too.  Words
+// not wholly inside the bytes the caller supplied count as data.
+// ---------------------------------------------------------------------------
+
+struct VbArgs {
+    int nseg;
+    JobDev jobs[kMaxJobs];
+};
+
+__device__ __forceinline__ bool vl_fill(const uint8_t* __restrict__ p, long long off, int bytes, long long avail)
+{
+    if (off < 0 || off + bytes > avail) return false;
+    bool f = true;
+    for (int b = 0; b < bytes; b++) {
+        const unsigned want = 0x44u - 0x11u * (unsigned)(b & 3);   // 44 33 22 11: 0x11223344 little-endian
+        f = f && (p[off + b] == want);
+    }
+    return f;
+}
+
+__global__ void k_vlba_blank(VbArgs a)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    const JobDev& J = a.jobs[blockIdx.z];
+    if (g >= a.nseg || f >= J.nf || J.up.kind != kUnpackVLBA) return;
+    const uint8_t* __restrict__ p = job_fft_ptr(J, f);
+    const long long avail = J.raw_len - (long long)(p - J.raw);
+    const long long seg = (long long)(g / kVlbaSegPerChunk) * kVlbaChunk + (long long)(g % kVlbaSegPerChunk) * kVlbaPayload;
+    constexpr int nword = kVlbaPayload / 8;
+    const bool start_ok = !vl_fill(p, seg, 8, avail);
+    const bool end_ok = !vl_fill(p, seg + 8LL * (nword - 1), 8, avail);
+    int vs, ve;
+    if (start_ok && end_ok) {
+        vs = 0;
+        ve = 1 << 30;
+    } else if (!start_ok && !end_ok) {
+        vs = 1 << 30;
+        ve = 0;
+    } else {
+        int i = nword / 2, r = nword - 1;
+        for (int s = nword / 4; s > 0; s >>= 1) {
+            if (vl_fill(p, seg + 8LL * i, 8, avail) == start_ok) {
+                r = i;
+                i -= s;
+            } else {
+                i += s;
+            }
+        }
+        int byte = 8 * r;
+        if (start_ok) {
+            if (vl_fill(p, seg + byte - 4, 4, avail)) byte -= 4;
+            vs = 0;
+            ve = byte;
+        } else {
+            if (vl_fill(p, seg + byte - 8, 4, avail)) byte += 4;
+            vs = byte;
+            ve = 1 << 30;
+        }
+    }
+    J.up.vblank[(size_t)f * a.nseg + g] = make_int2(vs, ve);
+}
+
+void vlba_modulation_bits(uint32_t* words)
+{
+    // mark5_format_vlba.c:167-196: register of 16 ones; each step shifts in
+    // b10 ^ b12 ^ b13 ^ b15; after every 8th output one unrecorded step
+    for (int w = 0; w < kVlbaModWords; w++) words[w] = 0u;
+    unsigned reg = 0xFFFFu;
+    for (int i = 0; i < kVlbaPayload; i++) {
+        const int steps = (i % 8 == 7) ? 2 : 1;
+        for (int st = 0; st < steps; st++) {
+            const unsigned k = ((reg >> 10) ^ (reg >> 12) ^ (reg >> 13) ^ (reg >> 15)) & 1u;
+            reg = ((reg << 1) | k) & 0xFFFFu;
+            if (st == 0 && k) words[i >> 5] |= 1u << (i & 31);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1121,7 +1258,8 @@ hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int n
         a.jobs[j] = jobs[j];
         aligned &= ((uintptr_t)jobs[j].raw % 4 == 0) && (jobs[j].fft_stride % 4 == 0);
         const int st = jobs[j].up.kind == kUnpackSingleChannel ? 0 : jobs[j].up.step;
-        if (step < 0) step = st;
+        if (jobs[j].up.kind == kUnpackVLBA) step = -2;   // level counts do not apply
+        else if (step == -1) step = st;
         else if (step != st) step = -2;
     }
     const bool fast = g.fast && aligned && (step == 0 || step == 1 || step == 2 || step == 4);
@@ -1160,6 +1298,24 @@ hipError_t launch_pcal_emit(const PcalGeom& g, double* acc, float* out, double* 
     const int th = ((std::max(g.T, ntones) + 63) / 64) * 64;
     if (th > 1024) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_pcal_emit, dim3(1), dim3(th), 0, s, acc, out, tones, g.T, ntones, tone_step);
+    return hipGetLastError();
+}
+
+hipError_t launch_vlba_blank(const JobDev* jobs, int njobs, int nfft, long long N, hipStream_t s)
+{
+    if (njobs < 1 || njobs > kMaxJobs || nfft < 1) return hipErrorInvalidValue;
+    VbArgs a;
+    a.nseg = vlba_segments(N);
+    bool any = false;
+    for (int j = 0; j < njobs; j++) {
+        a.jobs[j] = jobs[j];
+        if (jobs[j].up.kind == kUnpackVLBA) {
+            if (!jobs[j].up.vblank || jobs[j].up.vnseg != a.nseg) return hipErrorInvalidValue;
+            any = true;
+        }
+    }
+    if (!any) return hipSuccess;
+    hipLaunchKernelGGL(k_vlba_blank, dim3((a.nseg + 63) / 64, nfft, njobs), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

@@ -6,6 +6,7 @@
 #include "TaskCoreHIP.h"
 
 #include <cstdlib>
+#include <cstring>
 #include <iostream>
 
 using std::endl;
@@ -53,6 +54,7 @@ void TaskCoreHIP::prepare(swspect_settings_t* settings)
     c.pcaloffsethz = settings->pcaloffsethz;
     c.pcalharmonicshz = settings->pcalharmonicshz;
     c.max_rawbuf_bytes = (int64_t)settings->rawbuf_size;
+    std::strncpy(c.mark5_format, settings->sourceformat_str.c_str(), sizeof(c.mark5_format) - 1);
     const char* dev = getenv("SWSPEC_HIP_DEVICE");
     c.device = dev ? atoi(dev) : 0;
 

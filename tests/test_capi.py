@@ -33,7 +33,7 @@ def test_library_exports_all_declared_symbols():
 def test_cfg_struct_layout_matches_header():
     # sizeof(swspec_hip_cfg) as the C compiler lays it out
     from swspec_amd import _lib
-    assert C.sizeof(_lib.Cfg) == 144
+    assert C.sizeof(_lib.Cfg) == 176
     assert C.sizeof(_lib.Ini) == 96
     assert C.sizeof(_lib.Derived) == 112
 

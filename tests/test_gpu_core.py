@@ -235,3 +235,61 @@ def test_parseval_full_cfg2_length(hip):
     say("  parseval rel %.3e" % (abs(lhs - rhs) / rhs))
     assert abs(lhs - rhs) <= 1e-5 * rhs
     core.finalize()
+
+
+# ---------------------------------------------------------------- VLBA (mark5access decode)
+
+def _vlba_bytes(rng, n, fills):
+    """Random 8-track bytes with Mark5 fill-pattern zones [a, b)."""
+    raw = rng.integers(0, 256, n, dtype=np.uint8)
+    raw[::4] = np.where(raw[::4] == 0x44, 0x45, raw[::4])
+    pat = np.frombuffer(bytes([0x44, 0x33, 0x22, 0x11]), dtype=np.uint8)
+    for a, b in fills:
+        for o in range(a, b, 4):
+            raw[o:o + 4] = pat
+    return raw
+
+
+VLBA_FILLS = [(12000, 20000), (40000 + 40128, 40128 + 40128 + 8),   # tail of a payload; the 128-byte chunk tail
+              (2 * 40128, 2 * 40128 + 4096), (3 * 40128 + 20000 + 56, 3 * 40128 + 40000)]
+
+
+@pytest.mark.parametrize("ch", [0, 1, 2, 3])
+def test_unpack_vlba_bitexact(hip, oracle, ch):
+    """VLBA1_1-128-4-2 decode (modulation, blank zones, per-chunk restart)
+    bit-exact against the oracle's restatement of VLBAUnpacker."""
+    import torch
+    say("unpack vlba", ch)
+    count = 4 * 40128 + 5000
+    raw = _vlba_bytes(np.random.default_rng(11 + ch), count, VLBA_FILLS)
+    s = hip.derive_settings(FFTpoints=40000, FFTIntegrationTimeSec=0.01, FFToverlapFactor=1, BandwidthHz=8e6,
+                            BitsPerSample=2, SourceChannels=4, SourceFormat="VLBA1_1-128-4-2")
+    d_raw = torch.from_numpy(raw).cuda()
+    d_out = torch.empty(count, dtype=torch.float32, device="cuda")
+    hip.unpack_device(s, d_raw, ch, count, d_out)
+    ref = oracle.vlba_unpack(raw, count, ch, avail=count)
+    got = d_out.cpu().numpy()
+    assert np.array_equal(got, ref), int(np.argmax(got != ref))
+    assert np.count_nonzero(ref == 0) > 0   # the fill zones were blanked
+
+
+def test_vlba_spectrum_and_pcal_vs_oracle(hip, oracle):
+    """doMaths over VLBA bytes with overlapping FFTs (every FFT restarts the
+    chunking), phase-cal on (generic fold) and blank zones inside FFTs."""
+    n, nf, fs = 96000, 3, 32e6     # 96000 = 30 rotator periods of 3200
+    say("vlba spectrum", n, nf)
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=2, WindowType="Hann",
+                            BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=4, SourceFormat="VLBA1_1-256-4-2",
+                            ExtractPCal=True, MaxSourceBufferMB=4096, UseFile1Channel=3)
+    assert s.extract_pcal == 1
+    nbytes = n * (nf + 2)
+    raw = _vlba_bytes(np.random.default_rng(21), nbytes, [(30000, 36000), (100000, 140000), (200008, 210000)])
+    m, auto, _, pc, core = run_device(s, [raw])
+    os_ = oracle_settings(oracle, s)
+    m64, a64, _, p64 = oracle.domaths(os_, [raw], 64)
+    assert m == m64
+    e = relerr(auto[0], a64[0])
+    ep = relerr(pc[0], p64[0])
+    say("  spectra %d err %.3e  pcal err %.3e" % (m, e, ep))
+    assert e <= TOL and ep <= 1e-5
+    core.finalize()
