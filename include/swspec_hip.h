@@ -39,6 +39,12 @@ enum swspec_format { SWSPEC_FMT_UNKNOWN = -1, SWSPEC_FMT_RAWSIGNED = 0, SWSPEC_F
 
 /* flags */
 #define SWSPEC_FLAG_NO_KAHAN 0x1u   /* plain FP32 accumulation (accuracy comparisons only) */
+/* Mk5B input as stored on disk: whole 10016-byte frames (16-byte header +
+ * 10000-byte payload).  The core skips the headers on the device instead of
+ * FileSource::read stripping them on the host (src/FileSource.cpp:184-215);
+ * every length passed to run()/run_device() is in frame bytes, and a
+ * trailing partial frame is ignored.  Sync words are not checked. */
+#define SWSPEC_FLAG_MK5B_FRAMED 0x2u
 
 /*
  * Flat copy of the derived settings a core reads (src/Settings.h:59-159,

@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(HERE, "libswspec_hip.so")
 
 SWSPEC_HIP_ABI_VERSION = 2
 FLAG_NO_KAHAN = 0x1
+FLAG_MK5B_FRAMED = 0x2   # Mk5B input as whole 10016-byte frames; headers skipped on the device
 
 WINDOWS = {"None": 0, "Cosine": 1, "Cosine2": 2, "Hamming": 3, "Hann": 4, "Blackman": 5}
 FORMATS = {"RawSigned": 0, "RawUnsigned": 1, "Mk5B": 2, "iBOB": 3, "VDIF": 4, "VLBA": 5,

@@ -275,9 +275,10 @@ class TaskCoreHIP:
             pass
 
 
-def unpack_device(settings, d_raw, channel, count, d_out, device=0):
-    """DataUnpacker::extract_samples on the GPU (device pointers or tensors)."""
-    cfg = settings.cfg(device=device)
+def unpack_device(settings, d_raw, channel, count, d_out, device=0, flags=0):
+    """DataUnpacker::extract_samples on the GPU (device pointers or tensors).
+    flags=FLAG_MK5B_FRAMED: d_raw holds whole Mk5B frames (headers included)."""
+    cfg = settings.cfg(device=device, flags=flags)
     raw = d_raw if isinstance(d_raw, int) else d_raw.data_ptr()
     out = d_out if isinstance(d_out, int) else d_out.data_ptr()
     check(load().swspec_hip_unpack_device(C.byref(cfg), raw, channel, count, out), None,

@@ -378,6 +378,11 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
         c->job_src[j] = job_src[j];
         if (pick_unpack(cfg, job_ch[j], c->up[j], why) != 0) return fail(c, SWSPEC_EUNSUPPORTED, why);
         c->vlba |= c->up[j].kind == kUnpackVLBA;
+        if (cfg->flags & SWSPEC_FLAG_MK5B_FRAMED) {
+            if (cfg->source_format != SWSPEC_FMT_MK5B || c->up[j].kind != kUnpackMultiChannel)
+                return fail(c, SWSPEC_EUNSUPPORTED, "SWSPEC_FLAG_MK5B_FRAMED needs Mk5B input");
+            c->up[j].framed = 1;
+        }
     }
     // bytes consumed per FFT must match the decoder layout
     {
@@ -414,9 +419,9 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
         c->pg.T = T;
         c->pg.L = L;
         pcal_geometry(c->pg, cfg->fft_points);
-        // the integer-count fold assumes the 2-bit field levels; VLBA (modulated,
-        // blanked samples) takes the generic fold
-        if (c->vlba) c->pg.fast = 0;
+        // the integer-count fold reads whole words of contiguous samples: VLBA
+        // (modulated, blanked samples) and framed input take the generic fold
+        if (c->vlba || (cfg->flags & SWSPEC_FLAG_MK5B_FRAMED)) c->pg.fast = 0;
         std::vector<float2> rot(L);
         const double dphi = 2 * M_PI * (-cfg->pcaloffsethz / cfg->samplingfreq);   // :130-135
         for (int i = 0; i < L; i++) {
@@ -539,7 +544,10 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
 {
     const Plan& p = c->plan;
     const swspec_hip_cfg& g = c->cfg;
-    std::vector<SpecWork> sched = schedule(c, len);
+    const bool framed = (g.flags & SWSPEC_FLAG_MK5B_FRAMED) != 0;
+    // framed input: the doMaths loop runs over the payload bytes of whole frames
+    const size_t plen = framed ? (len / (size_t)kMk5bFrame) * (size_t)kMk5bPayload : len;
+    std::vector<SpecWork> sched = schedule(c, plen);
     const size_t M1 = (size_t)p.M + 1;
     const int ssb = g.fft_ssb_points;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -552,8 +560,14 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
             for (int j = 0; j < c->njobs; j++) {
                 JobDev& J = jobs[j];
                 const long long off = w.start + (long long)f0 * g.raw_overlap_bytes;
-                J.raw = d_in[c->job_src[j]] + off;
-                J.raw_len = (long long)len - off;
+                if (framed) {
+                    J.raw = d_in[c->job_src[j]];
+                    J.pay0 = off;
+                    J.raw_len = (long long)len;
+                } else {
+                    J.raw = d_in[c->job_src[j]] + off;
+                    J.raw_len = (long long)len - off;
+                }
                 J.fft_stride = g.raw_overlap_bytes;
                 J.nf = nf;
                 J.up = c->up[j];
@@ -839,6 +853,11 @@ extern "C" int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d
     Unpack up;
     std::string why;
     if (pick_unpack(cfg, channel, up, why) != 0) return fail(nullptr, SWSPEC_EUNSUPPORTED, why);
+    if (cfg->flags & SWSPEC_FLAG_MK5B_FRAMED) {
+        if (cfg->source_format != SWSPEC_FMT_MK5B || up.kind != kUnpackMultiChannel)
+            return fail(nullptr, SWSPEC_EUNSUPPORTED, "SWSPEC_FLAG_MK5B_FRAMED needs Mk5B input");
+        up.framed = 1;   // count samples from payload byte 0 of the frames at d_raw
+    }
     hipError_t e = hipSuccess;
     uint32_t* d_mod = nullptr;
     int2* d_blank = nullptr;

@@ -41,11 +41,23 @@ struct Unpack {
     int step = 1;      // bytes per time sample (multi-channel)
     int cbyte = 0;     // byte offset of the channel inside a time sample
     int shift = 0;     // bit shift of the 2-bit field
+    // Mk5B frames as stored on disk (multi-channel layout only): payload byte P
+    // is at mk5b_frame_addr(P) of the input
+    int framed = 0;
     // VLBA only
     int vnseg = 0;                       // payload segments per FFT
     const uint32_t* vmod = nullptr;      // modulation bits, payload index i at bit i
     int2* vblank = nullptr;              // [FFT of the chunk][segment] data zone (start, end)
 };
+
+// Mk5B framing (src/FileSource.cpp:184-215: 10016-byte frames, 16-byte header,
+// stripped on the host by the reference): address of payload byte P.
+constexpr long long kMk5bFrame = 10016, kMk5bHeader = 16, kMk5bPayload = 10000;
+__host__ __device__ inline long long mk5b_frame_addr(long long P)
+{
+    const long long fr = P / kMk5bPayload;
+    return fr * kMk5bFrame + kMk5bHeader + (P - fr * kMk5bPayload);
+}
 
 // Segments of an N-sample VLBA FFT.
 inline int vlba_segments(long long N)
@@ -131,6 +143,8 @@ struct JobDev {
     unsigned long long pcal_mask = 0;  // bit f: FFT f of the chunk gets phase-cal
     int nf = 0;                    // FFTs of this job in the chunk (<= launch nfft)
     long long raw_len = 0;         // bytes readable from raw (VLBA blank-zone probes past an FFT)
+    long long pay0 = 0;            // framed Mk5B: payload offset of FFT 0 (raw = first frame; FFT f
+                                   // starts at payload byte pay0 + f*fft_stride)
 };
 
 struct XpolDev {

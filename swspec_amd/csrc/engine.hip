@@ -355,11 +355,21 @@ __device__ __forceinline__ float vlba_sample(const uint8_t* __restrict__ raw, co
     return (i >= z.x && i < z.y) ? (flip ? -lv : lv) : 0.0f;
 }
 
-// Any decoder, sample j of the FFT starting at raw (brow: VLBA only).
+// Framed Mk5B: sample j of the FFT whose payload starts at byte `pay` of the
+// header-stripped stream; raw is the first frame.
+__device__ __forceinline__ unsigned field2_framed(const uint8_t* __restrict__ raw, const Unpack& up, long long pay,
+                                                  long long j)
+{
+    return (raw[mk5b_frame_addr(pay + j * up.step + up.cbyte)] >> up.shift) & 3u;
+}
+
+// Any decoder, sample j of one FFT: raw is the FFT start (framed: the first
+// frame, with the FFT's payload offset `pay`); brow: VLBA only.
 __device__ __forceinline__ float decode_sample(const uint8_t* __restrict__ raw, const Unpack& up,
-                                               const int2* __restrict__ brow, long long j)
+                                               const int2* __restrict__ brow, long long pay, long long j)
 {
     if (up.kind == kUnpackVLBA) return vlba_sample(raw, up, brow, (unsigned)j);
+    if (up.framed) return map2bit(field2_framed(raw, up, pay, j));
     return map2bit(field2(raw, up, j));
 }
 
@@ -447,8 +457,9 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     const int c0 = (int)bx * B;
     const int tid = threadIdx.x - pipe * pth, nth = pth;
     float2* __restrict__ lp = lds + pipe * (N1 * B);
-    const uint8_t* __restrict__ raw = job_fft_ptr(J, active ? f : 0);
     const Unpack up = J.up;
+    const uint8_t* __restrict__ raw = up.framed ? J.raw : job_fft_ptr(J, active ? f : 0);
+    const long long pay = J.pay0 + (long long)(active ? f : 0) * J.fft_stride;   // framed only
     const bool even = ((uintptr_t)raw & 1) == 0;
 
     // Window on the fly: sample j = 2(N2 e + c) + r has angle phi_e + psi_{c,r}
@@ -481,9 +492,10 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
         up.kind == kUnpackVLBA ? up.vblank + (size_t)(active ? f : 0) * up.vnseg : nullptr;
     auto fill = [&](auto mode) {
         constexpr int MODE = decltype(mode)::value;
-        // VLBA decode holds more per sample (segment zone, modulation word):
-        // a smaller batch keeps it under the register budget of the 2-bit modes
-        constexpr int U = MODE == 3 ? 2 : 8;
+        // VLBA decode holds more per sample (segment zone, modulation word), and
+        // framed Mk5B 64-bit frame addresses: smaller batches keep them under
+        // the register budget of the plain 2-bit modes
+        constexpr int U = MODE >= 3 ? 2 : 8;
         for (int base = tid; base < total; base += U * nth) {
             float x0[U], x1[U];
             float2 ph[U];
@@ -495,6 +507,9 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
                 if constexpr (MODE == 3) {
                     x0[u] = vlba_sample(raw, up, brow, 2u * n);
                     x1[u] = vlba_sample(raw, up, brow, 2u * n + 1u);
+                } else if constexpr (MODE == 4) {
+                    x0[u] = map2bit(field2_framed(raw, up, pay, 2LL * n));
+                    x1[u] = map2bit(field2_framed(raw, up, pay, 2LL * n + 1));
                 } else {
                     unsigned f0, f1;
                     fields_pair<MODE>(raw, up, (long long)n, f0, f1);
@@ -520,6 +535,7 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     if (active) {
         if (up.kind == kUnpackSingleChannel) fill(std::integral_constant<int, 0>());
         else if (up.kind == kUnpackVLBA) fill(std::integral_constant<int, 3>());
+        else if (up.framed) fill(std::integral_constant<int, 4>());
         else if (up.step == 1 && even) fill(std::integral_constant<int, 1>());
         else fill(std::integral_constant<int, 2>());
     }
@@ -839,14 +855,15 @@ __global__ void __launch_bounds__(1024) k_pcal(PcArgs a)
     const int T = a.g.T, L = a.g.L;
     double ar = 0.0, ai = 0.0;
     if ((J.pcal_mask >> f) & 1ull) {
-        const uint8_t* __restrict__ raw = job_fft_ptr(J, f);
         const Unpack up = J.up;
+        const uint8_t* __restrict__ raw = up.framed ? J.raw : job_fft_ptr(J, f);
+        const long long pay = J.pay0 + (long long)f * J.fft_stride;   // framed only
         const int2* __restrict__ brow = up.kind == kUnpackVLBA ? up.vblank + (size_t)f * up.vnseg : nullptr;
         const long long s0 = (long long)blk * a.g.chunk;
         const long long s1 = min(s0 + (long long)a.g.chunk, a.N);
         int ri = (int)((s0 + tid) % L);
         for (long long j = s0 + tid; j < s1; j += nth) {
-            const float x = decode_sample(raw, up, brow, j);
+            const float x = decode_sample(raw, up, brow, pay, j);
             const float2 r = a.g.d_rot[ri];
             ar += (double)x * (double)r.x;
             ai += (double)x * (double)r.y;
@@ -1015,7 +1032,7 @@ __global__ void k_unpack(const uint8_t* __restrict__ raw, Unpack up, long long c
 {
     for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < count;
          j += (long long)gridDim.x * blockDim.x)
-        out[j] = decode_sample(raw, up, up.vblank, j);
+        out[j] = decode_sample(raw, up, up.vblank, 0, j);
 }
 
 // ---------------------------------------------------------------------------
@@ -1258,7 +1275,7 @@ hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int n
         a.jobs[j] = jobs[j];
         aligned &= ((uintptr_t)jobs[j].raw % 4 == 0) && (jobs[j].fft_stride % 4 == 0);
         const int st = jobs[j].up.kind == kUnpackSingleChannel ? 0 : jobs[j].up.step;
-        if (jobs[j].up.kind == kUnpackVLBA) step = -2;   // level counts do not apply
+        if (jobs[j].up.kind == kUnpackVLBA || jobs[j].up.framed) step = -2;   // generic fold
         else if (step == -1) step = st;
         else if (step != st) step = -2;
     }

@@ -293,3 +293,68 @@ def test_vlba_spectrum_and_pcal_vs_oracle(hip, oracle):
     say("  spectra %d err %.3e  pcal err %.3e" % (m, e, ep))
     assert e <= TOL and ep <= 1e-5
     core.finalize()
+
+
+# ---------------------------------------------------------------- Mk5B frames on the device
+
+def test_unpack_mk5b_framed_bitexact(hip, oracle):
+    """Whole Mk5B frames in, headers skipped on the device: equals the oracle's
+    FileSource header strip followed by the Mk5B decoder."""
+    import torch
+    from swspec_amd._lib import FLAG_MK5B_FRAMED
+    say("unpack mk5b framed")
+    for nch in (4, 8):
+        payload = np.random.default_rng(31 + nch).integers(0, 256, 7 * 10000, dtype=np.uint8)
+        frames = synth.mk5b_frames(payload)
+        count = len(payload) // (nch // 4)
+        s = hip.derive_settings(FFTpoints=4000, FFTIntegrationTimeSec=0.01, FFToverlapFactor=1, BandwidthHz=8e6,
+                                BitsPerSample=2, SourceChannels=nch, SourceFormat="Mk5B")
+        os_ = oracle_settings(oracle, s)
+        stripped = oracle.mk5b_strip(frames)
+        d_raw = torch.from_numpy(frames).cuda()
+        for ch in range(nch):
+            d_out = torch.empty(count, dtype=torch.float32, device="cuda")
+            hip.unpack_device(s, d_raw, ch, count, d_out, flags=FLAG_MK5B_FRAMED)
+            ref = oracle.unpack(os_, stripped, count, ch)[:count]
+            assert np.array_equal(d_out.cpu().numpy(), ref), (nch, ch)
+
+
+def test_mk5b_framed_run_matches_stripped(hip, oracle):
+    """A core fed the on-disk frames gives the spectra of the core fed the
+    stripped payload bit for bit (same decoded samples, same FFT), and both
+    match the oracle; phase-cal agrees to FP64 summation order."""
+    from swspec_amd._lib import FLAG_MK5B_FRAMED
+    n, nf, fs = 320000, 3, 32e6
+    say("mk5b framed run", n, nf)
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=2, WindowType="Cosine2",
+                            BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=4, SourceFormat="Mk5B",
+                            ExtractPCal=True, MaxSourceBufferMB=4096, UseFile1Channel=2)
+    rng = np.random.default_rng(33)
+    comb = synth.pcal_comb(fs, fs / 2, amp=0.2, rng=rng)
+    payload = synth.raw_multi(7, 4, n * (nf + 2), tones_per_ch=[comb] * 4, fs=fs)
+    payload = payload[: (len(payload) // 10000) * 10000]
+    frames = synth.mk5b_frames(payload)
+    m1, a1, _, p1, c1 = run_device(s, [payload])
+    c1.finalize()
+    from swspec_amd import TaskCoreHIP
+    import torch
+    core = TaskCoreHIP(0)
+    core.prepare(s, flags=FLAG_MK5B_FRAMED)
+    ssb, T = s.fft_ssb_points, s.pcal_tonebins
+    d_in = torch.from_numpy(frames).cuda()
+    d_a = torch.full((4 * ssb,), float("nan"), device="cuda")
+    d_p = torch.full((8 * T,), float("nan"), device="cuda")
+    core.run_device([d_in], [len(frames)], [d_a], None, [d_p])
+    m2 = core.join()
+    core.finalize()
+    a2 = d_a.cpu().numpy().reshape(4, ssb)[:m2]
+    pv = d_p.cpu().numpy()
+    p2 = (pv[0::2] + 1j * pv[1::2]).reshape(4, T)[:m2]
+    assert m1 == m2 >= 1
+    assert np.array_equal(a1[0], a2)
+    ep = relerr(p2, p1[0])
+    os_ = oracle_settings(oracle, s)
+    _, a64, _, _ = oracle.domaths(os_, [payload], 64)
+    e = relerr(a2, a64[0])
+    say("  spectra %d err vs fp64 %.3e pcal framed vs stripped %.3e" % (m2, e, ep))
+    assert e <= TOL and ep <= 1e-9
