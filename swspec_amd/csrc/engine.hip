@@ -640,12 +640,19 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
         accp[s] = a.jobs[grp * NSRC + s].acc + (sact ? sl_i : 0) * a.accstride;
     }
     const int nbins = (self ? 1 : 2) * N2;
-    // chunk sums in LDS after the FFT rows: |X|^2 per source, then X0*conj(X1)
-    float* __restrict__ accl = (float*)(lds + NSL * N2);
-    float2* __restrict__ accxl = (float2*)(accl + NSRC * 2 * N2);
-    for (int i = tid; i < NSRC * 2 * N2; i += nth) accl[i] = 0.f;
-    if (NSRC == 2)
-        for (int i = tid; i < 2 * N2; i += nth) accxl[i] = make_float2(0.f, 0.f);
+    // Chunk sums in registers: thread bins idx = tid + m*nth, m < MB, the same
+    // bins for every FFT of the slice (nbins <= 2*N2 <= MB*nth because a row
+    // FFT gives each thread at most MAXPTS points; the launcher checks it).
+    // Keeping them out of LDS leaves the workgroup only its rows there.
+    constexpr int MB = MAXPTS / NSRC;
+    float accr[NSRC][MB];
+    float2 accx[MB];   // cross-power sums (NSRC == 2; dead otherwise)
+#pragma unroll
+    for (int m = 0; m < MB; m++) {
+#pragma unroll
+        for (int s = 0; s < NSRC; s++) accr[s][m] = 0.f;
+        if (NSRC == 2) accx[m] = make_float2(0.f, 0.f);
+    }
     float nyqP[NSRC];
     float nyqX = 0.f;
 #pragma unroll
@@ -681,7 +688,13 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
         fft_lds<MAXPTS, CX>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * N2, 0, q0, T);
 
         // each thread keeps the same bins for every FFT of the slice
-        for (int idx = tid; act && idx < nbins; idx += nth) {
+#pragma unroll
+        for (int m = 0; m < MB; m++) {
+            // opaque per bin: keeps the unrolled bins' loads and index math
+            // from being hoisted and held live together
+            int idx = tid + m * nth;
+            asm volatile("" : "+v"(idx));
+            if (!act || idx >= nbins) continue;
             const int which = idx >= N2 ? 1 : 0, k2 = idx - which * N2;
             const int row = which ? rowB : rowA;
             int msl, mk2;
@@ -694,14 +707,14 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
                 const float2 Z = lds[(2 * s + which) * N2 + k2];
                 const float2 Zm = lds[(2 * s + msl) * N2 + mk2];
                 X[s] = r2c_bin(Z, Zm, w);
-                accl[s * 2 * N2 + idx] += X[s].x * X[s].x + X[s].y * X[s].y;
+                accr[s][m] += X[s].x * X[s].x + X[s].y * X[s].y;
             }
             if (NSRC == 2) {
                 // X0 * conj(X1)
-                float2 c = accxl[idx];
+                float2 c = accx[m];
                 c.x += X[0].x * X[NSRC - 1].x + X[0].y * X[NSRC - 1].y;
                 c.y += X[0].y * X[NSRC - 1].x - X[0].x * X[NSRC - 1].y;
-                accxl[idx] = c;
+                accx[m] = c;
             }
             if (row == 0 && k2 == 0) {
                 float xn[NSRC];
@@ -720,12 +733,14 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
     // accumulator (sum, compensation) pairs: loads batched like the row loads
     float* __restrict__ xbase = NSRC == 2 ? a.xp[grp].acc + sl_i * a.xaccstride : nullptr;
     constexpr int UA = 4;
-    for (int base = tid; sact && base < nbins; base += UA * nth) {
+#pragma unroll
+    for (int m0 = 0; m0 < MB; m0 += UA) {
+        if (!sact || tid + m0 * nth >= nbins) break;
         float2 cur[NSRC][UA];
         float4 xcur[UA];
 #pragma unroll
         for (int u = 0; u < UA; u++) {
-            const int idx = batch_idx<UA>(base, u, nth, nbins);
+            const int idx = min(tid + (m0 + u) * nth, nbins - 1);
             const int which = idx >= N2 ? 1 : 0;
             const long long o = (long long)(which ? rowB : rowA) * N2 + (idx - which * N2);
 #pragma unroll
@@ -734,7 +749,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
         }
 #pragma unroll
         for (int u = 0; u < UA; u++) {
-            const int idx = base + u * nth;
+            const int idx = tid + (m0 + u) * nth;
             if (idx >= nbins) continue;
             const int which = idx >= N2 ? 1 : 0, k2 = idx - which * N2;
             const int row = which ? rowB : rowA;
@@ -742,7 +757,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
 #pragma unroll
             for (int s = 0; s < NSRC; s++) {
                 float2 sc = cur[s][u];
-                const float v = accl[s * 2 * N2 + idx];
+                const float v = accr[s][m0 + u];
                 if (a.kahan) {
                     const float y = v - sc.y, t = sc.x + y;
                     sc.y = (t - sc.x) - y;
@@ -754,7 +769,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
             }
             if (NSRC == 2) {
                 float4 x = xcur[u];
-                const float2 v = accxl[idx];
+                const float2 v = accx[m0 + u];
                 if (a.kahan) {
                     const float y0 = v.x - x.z, y1 = v.y - x.w;
                     const float t0 = x.x + y0, t1 = x.y + y1;
@@ -1125,11 +1140,10 @@ void vlba_modulation_bits(uint32_t* words)
 // ---------------------------------------------------------------------------
 
 static size_t p1_lds(const Plan& p) { return (size_t)p.N1 * p.B * sizeof(float2); }
-// FFT rows + |X|^2 chunk sums (+ cross-power chunk sums)
+// FFT rows only (the chunk sums live in registers)
 static size_t p2_lds(const Plan& p, int nsrc)
 {
-    return (size_t)2 * nsrc * p.N2 * sizeof(float2) + (size_t)nsrc * 2 * p.N2 * sizeof(float) +
-           (nsrc == 2 ? (size_t)2 * p.N2 * sizeof(float2) : 0);
+    return (size_t)2 * nsrc * p.N2 * sizeof(float2);
 }
 
 hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, hipStream_t s)
@@ -1175,6 +1189,8 @@ hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const Xpol
     if (xp) for (int g = 0; g < njobs / 2; g++) a.xp[g] = xp[g];
     a.S = pass2_slices(p, njobs / nsrc, nfft, smax);
     a.pth = nsrc * 2 * p.sp2.T;
+    // register chunk sums: each thread owns <= p2_maxpts / nsrc bins
+    if ((long long)2 * p.N2 > (long long)(p.p2_maxpts / nsrc) * a.pth) return hipErrorInvalidValue;
     const size_t lds1 = p2_lds(p, nsrc);
     a.pipe_lds = (int)(lds1 / sizeof(float2));
     // pack short rows: up to 256 threads (SWSPEC_P2_WG_THREADS) and 64 KB of LDS per workgroup
