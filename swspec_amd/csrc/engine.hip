@@ -546,24 +546,24 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     float2* __restrict__ Y = J.Y + (long long)(active ? f : 0) * a.M;
     const unsigned mask = (1u << a.twshift) - 1u;
     constexpr int U = 8;
-    // Y[k1*N2 + n2] = lp[k1*B + bcol] * W_M^{n2 k1}, n2 = cb fixed per thread
+    // Y[k1*N2 + n2] = lp[k1*B + bcol] * W_M^{n2 k1}, n2 = cb fixed per thread.
+    // Along a thread's rows k1 = k0 + u*estep the twiddle advances by the fixed
+    // factor W_M^{cb*estep}: each batch of U rows takes W_M^{cb*k0} from the
+    // split tables (FP64-derived, one product) and the other U-1 by complex
+    // multiplication, so the relative error stays within ~U ulp.
+    auto tw_at = [&](unsigned e) { return cmul(a.twA[e >> a.twshift], a.twB[e & mask]); };
+    const float2 tstep = tw_at(__umul24((unsigned)cb, (unsigned)estep));   // cb*estep < N2*N1 = M
     for (int base = tid; active && base < total; base += U * nth) {
-        float2 tA[U], tB[U];
         const int k0 = base >> lgB;
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int k1 = min(k0 + u * estep, N1 - 1);
-            const unsigned e = __umul24((unsigned)cb, (unsigned)k1);   // n2 k1 < M
-            tA[u] = a.twA[e >> a.twshift];
-            tB[u] = a.twB[e & mask];
-        }
+        float2 w = tw_at(__umul24((unsigned)cb, (unsigned)k0));              // n2 k1 < M
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int k1 = k0 + u * estep;
             if (k1 < N1) {
                 const unsigned o = __umul24((unsigned)k1, (unsigned)N2) + (unsigned)cb;
-                Y[o] = cmul(lp[base + u * nth], cmul(tA[u], tB[u]));
+                Y[o] = cmul(lp[base + u * nth], w);
             }
+            w = cmul(w, tstep);
         }
     }
 }
