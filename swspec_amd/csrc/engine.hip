@@ -591,24 +591,6 @@ struct P2Args {
     XpolDev xp[kMaxJobs / 2];
 };
 
-__device__ __forceinline__ float2 r2c_bin(float2 Z, float2 Zm, float2 w)
-{
-    const float2 E = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
-    const float2 O = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
-    const float2 wO = cmul(w, O);
-    return make_float2(E.x + wO.x, E.y + wO.y);
-}
-
-__device__ __forceinline__ void kahan_add(float* __restrict__ p, float v)
-{
-    float s = p[0], c = p[1];
-    const float y = v - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    p[0] = t;
-    p[1] = c;
-}
-
 template <int MAXPTS, int NSRC, bool CX>
 __global__ void __launch_bounds__(512) k_pass2(P2Args a)
 {
@@ -639,24 +621,31 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
         Yp[s] = a.jobs[grp * NSRC + s].Y;
         accp[s] = a.jobs[grp * NSRC + s].acc + (sact ? sl_i : 0) * a.accstride;
     }
-    const int nbins = (self ? 1 : 2) * N2;
-    // Chunk sums in registers: thread bins idx = tid + m*nth, m < MB, the same
-    // bins for every FFT of the slice (nbins <= 2*N2 <= MB*nth because a row
-    // FFT gives each thread at most MAXPTS points; the launcher checks it).
-    // Keeping them out of LDS leaves the workgroup only its rows there.
-    constexpr int MB = MAXPTS / NSRC;
-    float accr[NSRC][MB];
-    float2 accx[MB];   // cross-power sums (NSRC == 2; dead otherwise)
+    // Bins in conjugate pairs: with E = (Z[k] + conj Z[M-k])/2 and
+    // O = (Z[k] - conj Z[M-k])/(2i), X[k] = E + W_N^k O and X[M-k] =
+    // conj(E - W_N^k O), so one pair of row values gives both bins.  Pair pi
+    // of the workgroup: bin (rowA, pi) with its mirror in rowB (column
+    // N2-1-pi), or within the row for a self-mirrored row; row 0's pair 0 is
+    // DC with Nyquist (accumulator index M).  Pairs pi = tid + m*nth, m < MP,
+    // are the same for every FFT of the slice (npair <= N2 <= MP*nth: a row FFT
+    // gives a thread at most MAXPTS points; the launcher checks it).  The chunk
+    // sums live in registers, so LDS holds only the rows.
+    const int npair = !self ? N2 : (rowA == 0 ? N2 / 2 + 1 : (N2 + 1) / 2);
+    constexpr int MP = MAXPTS / NSRC / 2;
+    float acc1[NSRC][MP], acc2[NSRC][MP];
+    float2 xacc1[MP], xacc2[MP];   // cross-power sums (NSRC == 2; dead otherwise)
 #pragma unroll
-    for (int m = 0; m < MB; m++) {
+    for (int m = 0; m < MP; m++) {
 #pragma unroll
-        for (int s = 0; s < NSRC; s++) accr[s][m] = 0.f;
-        if (NSRC == 2) accx[m] = make_float2(0.f, 0.f);
+        for (int s = 0; s < NSRC; s++) acc1[s][m] = acc2[s][m] = 0.f;
+        if (NSRC == 2) xacc1[m] = xacc2[m] = make_float2(0.f, 0.f);
     }
-    float nyqP[NSRC];
-    float nyqX = 0.f;
-#pragma unroll
-    for (int s = 0; s < NSRC; s++) nyqP[s] = 0.f;
+    // LDS column of the mirror of (rowA, pi) and whether the pair is one bin
+    auto mirror_col = [&](int pi) { return (self && rowA == 0) ? (N2 - pi) % N2 : N2 - 1 - pi; };
+    auto mirror_row = [&]() { return self ? 0 : 1; };   // row slot of the mirror (0 = rowA's slot)
+    auto single = [&](int pi) {
+        return self && (rowA == 0 ? (pi != 0 && 2 * pi == N2) : (2 * pi == N2 - 1));
+    };
 
     const int nfj = a.jobs[grp * NSRC].nf;   // FFTs of this job (pair) in the chunk
     const int per = (nfj + a.S - 1) / a.S;   // FFTs per slice (uniform trip count)
@@ -687,119 +676,111 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
         __syncthreads();
         fft_lds<MAXPTS, CX>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * N2, 0, q0, T);
 
-        // each thread keeps the same bins for every FFT of the slice
+        // each thread keeps the same bin pairs for every FFT of the slice
 #pragma unroll
-        for (int m = 0; m < MB; m++) {
-            // opaque per bin: keeps the unrolled bins' loads and index math
+        for (int m = 0; m < MP; m++) {
+            // opaque per pair: keeps the unrolled pairs' loads and index math
             // from being hoisted and held live together
-            int idx = tid + m * nth;
-            asm volatile("" : "+v"(idx));
-            if (!act || idx >= nbins) continue;
-            const int which = idx >= N2 ? 1 : 0, k2 = idx - which * N2;
-            const int row = which ? rowB : rowA;
-            int msl, mk2;
-            if (row == 0) { msl = which; mk2 = (k2 == 0) ? 0 : N2 - k2; }
-            else { msl = self ? which : 1 - which; mk2 = N2 - 1 - k2; }
-            const float2 w = cmul(a.rA[row], a.rB[k2]);
-            float2 X[NSRC];
+            int pi = tid + m * nth;
+            asm volatile("" : "+v"(pi));
+            if (!act || pi >= npair) continue;
+            const int mc = mirror_col(pi), ms = mirror_row();
+            const float2 w = cmul(a.rA[rowA], a.rB[pi]);
+            float2 X1[NSRC], X2[NSRC];
 #pragma unroll
             for (int s = 0; s < NSRC; s++) {
-                const float2 Z = lds[(2 * s + which) * N2 + k2];
-                const float2 Zm = lds[(2 * s + msl) * N2 + mk2];
-                X[s] = r2c_bin(Z, Zm, w);
-                accr[s][m] += X[s].x * X[s].x + X[s].y * X[s].y;
+                const float2 Z = lds[(2 * s) * N2 + pi];
+                const float2 Zm = lds[(2 * s + ms) * N2 + mc];
+                const float2 E = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+                const float2 O = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
+                const float2 wO = cmul(w, O);
+                X1[s] = make_float2(E.x + wO.x, E.y + wO.y);
+                X2[s] = make_float2(E.x - wO.x, wO.y - E.y);   // conj(E - wO)
+                acc1[s][m] += X1[s].x * X1[s].x + X1[s].y * X1[s].y;
+                acc2[s][m] += X2[s].x * X2[s].x + X2[s].y * X2[s].y;
             }
             if (NSRC == 2) {
-                // X0 * conj(X1)
-                float2 c = accx[m];
-                c.x += X[0].x * X[NSRC - 1].x + X[0].y * X[NSRC - 1].y;
-                c.y += X[0].y * X[NSRC - 1].x - X[0].x * X[NSRC - 1].y;
-                accx[m] = c;
-            }
-            if (row == 0 && k2 == 0) {
-                float xn[NSRC];
-#pragma unroll
-                for (int s = 0; s < NSRC; s++) {
-                    const float2 Z = lds[(2 * s) * N2];
-                    xn[s] = Z.x - Z.y;   // X[M] = Re Z0 - Im Z0
-                    nyqP[s] += xn[s] * xn[s];
-                }
-                if (NSRC == 2) nyqX += xn[0] * xn[NSRC - 1];
+                // X0 * conj(X1) for both bins of the pair
+                float2 c = xacc1[m];
+                c.x += X1[0].x * X1[NSRC - 1].x + X1[0].y * X1[NSRC - 1].y;
+                c.y += X1[0].y * X1[NSRC - 1].x - X1[0].x * X1[NSRC - 1].y;
+                xacc1[m] = c;
+                c = xacc2[m];
+                c.x += X2[0].x * X2[NSRC - 1].x + X2[0].y * X2[NSRC - 1].y;
+                c.y += X2[0].y * X2[NSRC - 1].x - X2[0].x * X2[NSRC - 1].y;
+                xacc2[m] = c;
             }
         }
         __syncthreads();
     }
 
-    // accumulator (sum, compensation) pairs: loads batched like the row loads
+    // accumulator (sum, compensation) updates, both bins of each pair; the
+    // accumulator reads of UA pairs are issued together
     float* __restrict__ xbase = NSRC == 2 ? a.xp[grp].acc + sl_i * a.xaccstride : nullptr;
-    constexpr int UA = 4;
+    auto acc_index = [&](int pi, bool second) -> long long {
+        if (!second) return (long long)rowA * N2 + pi;
+        if (self && rowA == 0 && pi == 0) return a.M;               // Nyquist
+        return (long long)(self ? rowA : rowB) * N2 + mirror_col(pi);
+    };
+    auto kahan2 = [&](float2 sc, float v) {
+        if (a.kahan) {
+            const float y = v - sc.y, t = sc.x + y;
+            sc.y = (t - sc.x) - y;
+            sc.x = t;
+        } else {
+            sc.x += v;
+        }
+        return sc;
+    };
+    auto kahan4 = [&](float4 x, float2 v) {
+        if (a.kahan) {
+            const float y0 = v.x - x.z, y1 = v.y - x.w;
+            const float t0 = x.x + y0, t1 = x.y + y1;
+            x.z = (t0 - x.x) - y0;
+            x.w = (t1 - x.y) - y1;
+            x.x = t0;
+            x.y = t1;
+        } else {
+            x.x += v.x;
+            x.y += v.y;
+        }
+        return x;
+    };
+    constexpr int UA = MP < 4 ? MP : 4;
 #pragma unroll
-    for (int m0 = 0; m0 < MB; m0 += UA) {
-        if (!sact || tid + m0 * nth >= nbins) break;
-        float2 cur[NSRC][UA];
-        float4 xcur[UA];
+    for (int m0 = 0; m0 < MP; m0 += UA) {
+        if (!sact || tid + m0 * nth >= npair) break;
+        float2 cur1[NSRC][UA], cur2[NSRC][UA];
+        float4 xc1[UA], xc2[UA];
+        long long o1[UA], o2[UA];
 #pragma unroll
         for (int u = 0; u < UA; u++) {
-            const int idx = min(tid + (m0 + u) * nth, nbins - 1);
-            const int which = idx >= N2 ? 1 : 0;
-            const long long o = (long long)(which ? rowB : rowA) * N2 + (idx - which * N2);
+            const int pi = min(tid + (m0 + u) * nth, npair - 1);
+            o1[u] = acc_index(pi, false);
+            o2[u] = acc_index(pi, true);
 #pragma unroll
-            for (int s = 0; s < NSRC; s++) cur[s][u] = *(const float2*)(accp[s] + 2 * o);
-            if (NSRC == 2) xcur[u] = *(const float4*)(xbase + 4 * o);
+            for (int s = 0; s < NSRC; s++) {
+                cur1[s][u] = *(const float2*)(accp[s] + 2 * o1[u]);
+                cur2[s][u] = *(const float2*)(accp[s] + 2 * o2[u]);
+            }
+            if (NSRC == 2) {
+                xc1[u] = *(const float4*)(xbase + 4 * o1[u]);
+                xc2[u] = *(const float4*)(xbase + 4 * o2[u]);
+            }
         }
 #pragma unroll
         for (int u = 0; u < UA; u++) {
-            const int idx = tid + (m0 + u) * nth;
-            if (idx >= nbins) continue;
-            const int which = idx >= N2 ? 1 : 0, k2 = idx - which * N2;
-            const int row = which ? rowB : rowA;
-            const long long o = (long long)row * N2 + k2;
+            const int pi = tid + (m0 + u) * nth;
+            if (pi >= npair) continue;
+            const bool two = !single(pi);
 #pragma unroll
             for (int s = 0; s < NSRC; s++) {
-                float2 sc = cur[s][u];
-                const float v = accr[s][m0 + u];
-                if (a.kahan) {
-                    const float y = v - sc.y, t = sc.x + y;
-                    sc.y = (t - sc.x) - y;
-                    sc.x = t;
-                } else {
-                    sc.x += v;
-                }
-                *(float2*)(accp[s] + 2 * o) = sc;
+                *(float2*)(accp[s] + 2 * o1[u]) = kahan2(cur1[s][u], acc1[s][m0 + u]);
+                if (two) *(float2*)(accp[s] + 2 * o2[u]) = kahan2(cur2[s][u], acc2[s][m0 + u]);
             }
             if (NSRC == 2) {
-                float4 x = xcur[u];
-                const float2 v = accx[m0 + u];
-                if (a.kahan) {
-                    const float y0 = v.x - x.z, y1 = v.y - x.w;
-                    const float t0 = x.x + y0, t1 = x.y + y1;
-                    x.z = (t0 - x.x) - y0;
-                    x.w = (t1 - x.y) - y1;
-                    x.x = t0;
-                    x.y = t1;
-                } else {
-                    x.x += v.x;
-                    x.y += v.y;
-                }
-                *(float4*)(xbase + 4 * o) = x;
-            }
-            if (row == 0 && k2 == 0) {
-#pragma unroll
-                for (int s = 0; s < NSRC; s++) {
-                    float* acc = accp[s] + 2 * a.M;
-                    if (a.kahan) kahan_add(acc, nyqP[s]);
-                    else acc[0] += nyqP[s];
-                }
-                if (NSRC == 2) {
-                    float* xa = a.xp[grp].acc + sl_i * a.xaccstride + 4 * a.M;
-                    if (a.kahan) {
-                        float s0 = xa[0], c0 = xa[2];
-                        float y0 = nyqX - c0, t0 = s0 + y0;
-                        xa[2] = (t0 - s0) - y0; xa[0] = t0;
-                    } else {
-                        xa[0] += nyqX;
-                    }
-                }
+                *(float4*)(xbase + 4 * o1[u]) = kahan4(xc1[u], xacc1[m0 + u]);
+                if (two) *(float4*)(xbase + 4 * o2[u]) = kahan4(xc2[u], xacc2[m0 + u]);
             }
         }
     }

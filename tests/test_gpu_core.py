@@ -45,6 +45,9 @@ def test_unpack_bitexact(hip, oracle, fmt, nch, incr):
 SIZES = [
     (4096, "Hann", 5), (20000, "Cosine2", 3), (32000, "Hann", 4), (65536, "Hamming", 3),
     (320000, "Blackman", 2), (1 << 20, "Cosine", 2), (1000000, "None", 2), (3200000, "Cosine2", 2),
+    # odd row lengths: 2700 = 2 * 30 x 45 (even column count: a self-mirrored
+    # middle row with a self-paired bin), 4374 = 2 * 27 x 81, 156250 = 2 * 125 x 625
+    (2700, "Hann", 5), (4374, "Cosine2", 4), (156250, "Hamming", 2),
 ]
 
 
