@@ -419,6 +419,7 @@ struct P1Args {
     const float2* twA;
     const float2* twB;
     WindowGen win;
+    int zfast;        // jobs share raw bytes: job index fastest in the workgroup order
     JobDev jobs[kMaxJobs];
 };
 
@@ -426,13 +427,23 @@ struct P1Args {
 // Workgroups are dispatched round-robin over the 8 XCDs by linear id; when the
 // grid size is a multiple of 8, linear id L runs on XCD L % 8, so logical id
 // (L % 8) * (total / 8) + L / 8 keeps neighbouring column blocks (which share
-// cache lines of the input and of Y) in one L2.
-__device__ __forceinline__ void xcd_remap(unsigned& x, unsigned& y, unsigned& z)
+// cache lines of the input and of Y) in one L2.  With zfast the job index z
+// varies fastest: the channels of one multi-channel source read the same raw
+// bytes, and adjacent logical ids find them in the same L2 at nearly the same
+// time instead of re-reading them from HBM once per channel.
+__device__ __forceinline__ void xcd_remap(unsigned& x, unsigned& y, unsigned& z, bool zfast = false)
 {
-    const unsigned gx = gridDim.x, gy = gridDim.y;
-    const unsigned total = gx * gy * gridDim.z;
+    const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const unsigned total = gx * gy * gz;
     unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
     if ((total & 7u) == 0) L = (L & 7u) * (total >> 3) + (L >> 3);
+    if (zfast) {
+        z = L % gz;
+        const unsigned xy = L / gz;
+        x = xy % gx;
+        y = xy / gx;
+        return;
+    }
     x = L % gx;
     const unsigned yz = L / gx;
     y = yz % gy;
@@ -444,7 +455,7 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
 {
     extern __shared__ float2 lds[];
     unsigned bx, by, bz;
-    xcd_remap(bx, by, bz);
+    xcd_remap(bx, by, bz, a.zfast);
     const JobDev J = a.jobs[bz];
     // a workgroup runs `fpw` independent pipelines, one FFT each (short columns)
     const int fpw = a.fpw, pth = a.pth;
@@ -1147,6 +1158,10 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
     a.N1 = p.N1; a.N2 = p.N2; a.M = p.M; a.B = p.B; a.lgB = p.lgB; a.twshift = p.twshift;
     a.sp = p.sp1; a.stw = p.d_stw1; a.twA = p.d_twA; a.twB = p.d_twB; a.win = p.win;
     a.fpw = std::max(1, std::min(p.p1_fpw, nfft));
+    a.zfast = 0;
+    for (int j = 1; j < njobs; j++)
+        if (jobs[j].raw == jobs[0].raw && jobs[j].fft_stride == jobs[0].fft_stride) a.zfast = 1;
+    if (const char* e = getenv("SWSPEC_P1_ZFAST")) a.zfast = atoi(e) != 0;
     a.pth = p.p1_threads;
     for (int j = 0; j < njobs; j++) a.jobs[j] = jobs[j];
     dim3 grid(p.N2 / p.B, (nfft + a.fpw - 1) / a.fpw, njobs);
