@@ -901,6 +901,8 @@ template <int STEP, int GPT>
 __global__ void __launch_bounds__(1024) k_pcal_fast(PcArgs a)
 {
     extern __shared__ double2 red4[];   // [nth][4]
+    // (job-fastest ordering as in pass 1 cut this kernel's HBM reads 4x but
+    // made it slower: it is not bound by them)
     const JobDev J = a.jobs[blockIdx.z];
     const int f = blockIdx.y, blk = blockIdx.x;
     const int tid = threadIdx.x, nth = blockDim.x;
