@@ -41,6 +41,10 @@ struct swspec_hip_core {
     int rank = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // phase-cal fold on its own stream (it reads only the raw input), so it
+    // runs beside the FFT passes; events order it against input and emission
+    hipStream_t pstream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_pc = nullptr, ev_em = nullptr;
     Plan plan;
     PcalGeom pg;
     float2* d_rot = nullptr;
@@ -89,8 +93,10 @@ struct swspec_hip_core {
 // Records an event pair around the launches issued inside its scope.
 struct ProfScope {
     swspec_hip_core* c;
+    hipStream_t s;
     int idx = -1;
-    ProfScope(swspec_hip_core* core, int kind) : c(core)
+    ProfScope(swspec_hip_core* core, int kind, hipStream_t stream = nullptr)
+        : c(core), s(stream ? stream : core->stream)
     {
         if (!c->profiling) return;
         const size_t need = 2 * (c->ev_used + 1);
@@ -102,11 +108,11 @@ struct ProfScope {
         idx = (int)c->ev_used++;
         c->ev_kind.resize(c->ev_used);
         c->ev_kind[idx] = kind;
-        (void)hipEventRecord(c->ev_pool[2 * idx], c->stream);
+        (void)hipEventRecord(c->ev_pool[2 * idx], s);
     }
     ~ProfScope()
     {
-        if (idx >= 0) (void)hipEventRecord(c->ev_pool[2 * idx + 1], c->stream);
+        if (idx >= 0) (void)hipEventRecord(c->ev_pool[2 * idx + 1], s);
     }
 };
 
@@ -396,6 +402,13 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(c, hipEventCreate(&c->ev0));
     HIPCHK(c, hipEventCreate(&c->ev1));
+    // VLBA phase-cal reads the per-chunk blank-zone table the main stream
+    // rewrites, so it stays on the main stream
+    if (cfg->extract_pcal && !c->vlba) {
+        HIPCHK(c, hipStreamCreateWithFlags(&c->pstream, hipStreamNonBlocking));   // default priority
+        for (hipEvent_t* e : {&c->ev_in, &c->ev_pc, &c->ev_em})
+            HIPCHK(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
+    }
     std::string e = plan_create(c->plan, cfg->fft_points, cfg->window, c->stream);
     if (!e.empty()) return fail(c, SWSPEC_EUNSUPPORTED, e);
 
@@ -494,6 +507,10 @@ extern "C" int swspec_hip_finalize(swspec_hip_core* c)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->pstream) (void)hipStreamSynchronize(c->pstream);
+    for (hipEvent_t e : {c->ev_in, c->ev_pc, c->ev_em})
+        if (e) (void)hipEventDestroy(e);
+    if (c->pstream) (void)hipStreamDestroy(c->pstream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SWSPEC_OK;
@@ -551,6 +568,12 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
     const size_t M1 = (size_t)p.M + 1;
     const int ssb = g.fft_ssb_points;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    hipStream_t pst = c->pstream ? c->pstream : c->stream;   // phase-cal stream
+    if (c->pstream) {
+        // input copies (host mode) are queued on the main stream before this
+        HIPCHK(c, hipEventRecord(c->ev_in, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->pstream, c->ev_in, 0));
+    }
     for (size_t sp = 0; sp < sched.size(); sp++) {
         const SpecWork& w = sched[sp];
         for (int f0 = 0; f0 < w.nfft; f0 += c->G) {
@@ -589,8 +612,8 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
                 HIPCHK(c, launch_pass1(p, jobs, c->njobs, nf, c->stream));
             }
             if (any_pcal) {
-                ProfScope ps(c, SWSPEC_K_PCAL);
-                HIPCHK(c, launch_pcal(c->pg, g.fft_points, jobs, c->njobs, nf, c->stream));
+                ProfScope ps(c, SWSPEC_K_PCAL, pst);
+                HIPCHK(c, launch_pcal(c->pg, g.fft_points, jobs, c->njobs, nf, pst));
             }
             XpolDev xp;
             xp.acc = c->d_xacc;
@@ -613,13 +636,25 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
             HIPCHK(c, launch_zero(c->d_xacc, (size_t)p.smax * 4 * M1 * sizeof(float), c->stream));
         }
         if (g.extract_pcal) {
+            if (c->pstream) {   // the spectrum's folds are done before they are emitted
+                HIPCHK(c, hipEventRecord(c->ev_pc, c->pstream));
+                HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_pc, 0));
+            }
             for (int j = 0; j < c->njobs; j++) {
                 float* po = d_pcal && d_pcal[j] ? d_pcal[j] + sp * 2 * (size_t)c->pg.T : nullptr;
                 double* to = d_tones ? d_tones + ((size_t)j * sched.size() + sp) * 2 * c->ntones : nullptr;
                 HIPCHK(c, launch_pcal_emit(c->pg, c->d_pcal_acc + (size_t)j * c->pg.T * 2, po, to, c->ntones,
                                            c->tone_step, c->stream));
             }
+            if (c->pstream) {   // the next spectrum's folds add to the cleared accumulators
+                HIPCHK(c, hipEventRecord(c->ev_em, c->stream));
+                HIPCHK(c, hipStreamWaitEvent(c->pstream, c->ev_em, 0));
+            }
         }
+    }
+    if (c->pstream) {   // all folds done before the run completes (join, input reuse)
+        HIPCHK(c, hipEventRecord(c->ev_pc, c->pstream));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_pc, 0));
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->last_nspectra = (int)sched.size();
