@@ -314,8 +314,11 @@ __device__ __forceinline__ void fft_lds(float2* lds, const SubPlan& sp, const fl
 
 __device__ __forceinline__ float map2bit(unsigned s)
 {
-    const float mag = (s & 2u) ? 3.3359f : 1.0f;
-    return (s & 1u) ? -mag : mag;
+    // built as IEEE bits: sign from bit 0, magnitude 1.0f or 3.3359f chosen by
+    // bit 1 through an integer multiply-add on the two bit patterns (exact)
+    constexpr unsigned one = 0x3F800000u;
+    constexpr unsigned hi = __builtin_bit_cast(unsigned, 3.3359f);
+    return __uint_as_float(((s & 1u) << 31) | (__umul24(s >> 1, hi - one) + one));
 }
 
 __device__ __forceinline__ unsigned field2(const uint8_t* __restrict__ raw, const Unpack& up, long long j)
@@ -473,47 +476,71 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     const long long pay = J.pay0 + (long long)(active ? f : 0) * J.fft_stride;   // framed only
     const bool even = ((uintptr_t)raw & 1) == 0;
 
-    // Window on the fly: sample j = 2(N2 e + c) + r has angle phi_e + psi_{c,r}
-    // with phi_e from a per-row table and psi fixed per thread (its column c
-    // never changes: nth is a multiple of B).  w = c0 + c1 cos + c2 sin + c3 cos2.
-    // Without a window the table holds phi = 0, psi = 0 and (c0..c3) = (1,0,0,0),
-    // so the same arithmetic gives w = 1 exactly.
+    // Window on the fly: real sample j = 2(N2 e + c) + r has angle
+    // theta = phi_e + psi_r, phi_e = 2 pi (2 N2 e) / D from a per-row table and
+    // psi_r = 2 pi (2c + r + delta) / D fixed per thread (its column c never
+    // changes: nth is a multiple of B).  Expanding the cosine sums,
+    //   w_r = a0 + cos(phi) P_r + sin(phi) Q_r + cos(2 phi) R_r + sin(2 phi) S_r
+    //   P = a1 cos psi + a2 sin psi,  Q = a2 cos psi - a1 sin psi,
+    //   R = a3 cos 2psi,              S = -a3 sin 2psi
+    // (per thread, FP64, rounded once).  phi is linear in the row, so the rows
+    // e0 + u*estep of a batch take (cos, sin)(phi) as the product of the table
+    // entries of e0 and of u*estep: one product each, nothing accumulates.
     const WindowGen wg = a.win;
     const int bcol = tid & (B - 1);
-    float2 psi0 = make_float2(1.f, 0.f), psi1 = make_float2(1.f, 0.f);
+    const int cb = c0 + bcol;
+    float P[2] = {0.f, 0.f}, Q[2] = {0.f, 0.f}, R2[2] = {0.f, 0.f}, S2[2] = {0.f, 0.f};
     if (wg.kind != 0) {
-        double s, c;
-        sincospi(2.0 * ((double)(2 * (c0 + bcol)) + wg.delta) / wg.D, &s, &c);
-        psi0 = make_float2((float)c, (float)s);
-        sincospi(2.0 * ((double)(2 * (c0 + bcol) + 1) + wg.delta) / wg.D, &s, &c);
-        psi1 = make_float2((float)c, (float)s);
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const double x = ((double)(2 * cb + r) + wg.delta) / wg.D;
+            double s, c;
+            sincospi(2.0 * x, &s, &c);
+            const double c2 = c * c - s * s, s2 = 2.0 * s * c;
+            P[r] = (float)((double)wg.a1 * c + (double)wg.a2 * s);
+            Q[r] = (float)((double)wg.a2 * c - (double)wg.a1 * s);
+            R2[r] = (float)((double)wg.a3 * c2);
+            S2[r] = (float)(-(double)wg.a3 * s2);
+        }
     }
-    const float wa0 = wg.kind ? wg.a0 : 1.f, wa1 = wg.kind ? wg.a1 : 0.f, wa2 = wg.kind ? wg.a2 : 0.f,
-                wa3 = wg.kind ? wg.a3 : 0.f;
+    const float wa0 = wg.a0;
+    // 0: no window (w = 1, no arithmetic), 1: no cos(2 theta) term, 2: all terms
+    const int wmode = wg.kind == 0 ? 0 : (wg.a3 == 0.f ? 1 : 2);
 
     // lp[e*B + b] = z[N2*e + c0 + b] windowed; idx == e*B + b.  A thread keeps
     // its column b = bcol; its row e advances by estep per strided step.  Rows
     // are clamped to N1 - 1 for the loads of a partial batch (values unused).
     // Point indices are < M <= 2^30 and the factors < 2^14: 24-bit products.
     const int total = N1 * B;
-    const int cb = c0 + bcol;
     const int estep = nth >> lgB;
     // MODE 3 (VLBA): samples decoded through the FFT's row of blank zones
     const int2* __restrict__ brow =
         up.kind == kUnpackVLBA ? up.vblank + (size_t)(active ? f : 0) * up.vnseg : nullptr;
-    auto fill = [&](auto mode) {
+    auto fill = [&](auto mode, auto wmc) __attribute__((always_inline)) {
         constexpr int MODE = decltype(mode)::value;
+        constexpr int WM = decltype(wmc)::value;
         // VLBA decode holds more per sample (segment zone, modulation word), and
         // framed Mk5B 64-bit frame addresses: smaller batches keep them under
         // the register budget of the plain 2-bit modes
         constexpr int U = MODE >= 3 ? 2 : 8;
-        for (int base = tid; base < total; base += U * nth) {
+        // The thread's k-th point is on row et + k*estep.  Its window phase is
+        // the table entry of row et + (k & ~7)*estep times that of row
+        // (k & 7)*estep: a function of the row alone, so every decode mode and
+        // batch size windows a row by the same operations.
+        constexpr int G = 8;
+        static_assert(G % U == 0, "batches must not straddle a phase group");
+        const int et = tid >> lgB;
+        // batch j: points k = j*U + u at base + u*nth; FULL: all of them in
+        // range (no clamps or checks), else the thread's last, partial batch
+        auto batch = [&](int base, int j, auto fullc) __attribute__((always_inline)) {
+            constexpr bool FULL = decltype(fullc)::value;
             float x0[U], x1[U];
-            float2 ph[U];
             const int e0 = base >> lgB;
+            const int k0 = j * U;
+            const float2 anc = WM ? wg.phi[et + (k0 & ~(G - 1)) * estep] : make_float2(1.f, 0.f);
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int e = min(e0 + u * estep, N1 - 1);
+                const int e = FULL ? e0 + u * estep : min(e0 + u * estep, N1 - 1);
                 const unsigned n = __umul24((unsigned)N2, (unsigned)e) + (unsigned)cb;
                 if constexpr (MODE == 3) {
                     x0[u] = vlba_sample(raw, up, brow, 2u * n);
@@ -527,28 +554,51 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
                     x0[u] = map2bit(f0);
                     x1[u] = map2bit(f1);
                 }
-                ph[u] = wg.phi[e];
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int idx = base + u * nth;
-                if (idx < total) {
-                    const float2 p = ph[u];
-                    const float cs0 = p.x * psi0.x - p.y * psi0.y, sn0 = p.y * psi0.x + p.x * psi0.y;
-                    const float cs1 = p.x * psi1.x - p.y * psi1.y, sn1 = p.y * psi1.x + p.x * psi1.y;
-                    const float w0 = wa0 + wa1 * cs0 + wa2 * sn0 + wa3 * (2.f * cs0 * cs0 - 1.f);
-                    const float w1 = wa0 + wa1 * cs1 + wa2 * sn1 + wa3 * (2.f * cs1 * cs1 - 1.f);
+                if (FULL || idx < total) {
+                    float w0 = 1.f, w1 = 1.f;
+                    if constexpr (WM != 0) {
+                        const int g = U == G ? u : ((k0 + u) & (G - 1));   // uniform
+                        float2 p = anc;
+                        if (g != 0) {
+                            // uniform load; rows past the table are never stored
+                            const float2 r = wg.phi[min(g * estep, N1 - 1)];
+                            p = cmul(anc, r);
+                        }
+                        w0 = wa0 + p.x * P[0] + p.y * Q[0];
+                        w1 = wa0 + p.x * P[1] + p.y * Q[1];
+                        if constexpr (WM == 2) {
+                            const float c2 = p.x * p.x - p.y * p.y, s2 = 2.f * p.x * p.y;
+                            w0 += c2 * R2[0] + s2 * S2[0];
+                            w1 += c2 * R2[1] + s2 * S2[1];
+                        }
+                    }
                     lp[idx] = make_float2(x0[u] * w0, x1[u] * w1);
                 }
             }
-        }
+        };
+        const int nfull = total - (U - 1) * nth;
+        int base = tid, j = 0;
+        for (; base < nfull; base += U * nth, j++) batch(base, j, std::true_type());
+        if (base < total) batch(base, j, std::false_type());
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    auto fill_w = [&](auto mode) __attribute__((always_inline)) {
+        if (wmode == 0) fill(mode, I0());
+        else if (wmode == 1) fill(mode, I1());
+        else fill(mode, I2());
     };
     if (active) {
-        if (up.kind == kUnpackSingleChannel) fill(std::integral_constant<int, 0>());
-        else if (up.kind == kUnpackVLBA) fill(std::integral_constant<int, 3>());
-        else if (up.framed) fill(std::integral_constant<int, 4>());
-        else if (up.step == 1 && even) fill(std::integral_constant<int, 1>());
-        else fill(std::integral_constant<int, 2>());
+        if (up.kind == kUnpackSingleChannel) fill_w(I0());
+        else if (up.kind == kUnpackVLBA) fill(std::integral_constant<int, 3>(), I2());
+        else if (up.framed) fill_w(std::integral_constant<int, 4>());   // same window code as mode 1
+        else if (up.step == 1 && even) fill_w(I1());
+        else fill_w(I2());
     }
     __syncthreads();
 
@@ -564,19 +614,25 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     // multiplication, so the relative error stays within ~U ulp.
     auto tw_at = [&](unsigned e) { return cmul(a.twA[e >> a.twshift], a.twB[e & mask]); };
     const float2 tstep = tw_at(__umul24((unsigned)cb, (unsigned)estep));   // cb*estep < N2*N1 = M
-    for (int base = tid; active && base < total; base += U * nth) {
+    auto store = [&](int base, auto fullc) __attribute__((always_inline)) {
+        constexpr bool FULL = decltype(fullc)::value;   // all U rows in range
         const int k0 = base >> lgB;
         float2 w = tw_at(__umul24((unsigned)cb, (unsigned)k0));              // n2 k1 < M
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int k1 = k0 + u * estep;
-            if (k1 < N1) {
+            if (FULL || k1 < N1) {
                 const unsigned o = __umul24((unsigned)k1, (unsigned)N2) + (unsigned)cb;
                 Y[o] = cmul(lp[base + u * nth], w);
             }
             w = cmul(w, tstep);
         }
-    }
+    };
+    if (!active) return;
+    const int nfull = total - (U - 1) * nth;
+    int base = tid;
+    for (; base < nfull; base += U * nth) store(base, std::true_type());
+    if (base < total) store(base, std::false_type());
 }
 
 // ---------------------------------------------------------------------------
