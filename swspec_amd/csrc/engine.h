@@ -100,8 +100,7 @@ struct WindowGen {
 struct Plan {
     long long N = 0, M = 0;
     int N1 = 0, N2 = 0;      // M = N1 * N2; pass 1 = N1-point columns, pass 2 = N2-point rows
-    int B = 0;               // columns per pass-1 workgroup (power of two)
-    int lgB = 0;             // log2(B)
+    int B = 0;               // columns per pass-1 workgroup pipeline (divides N2)
     int p1_threads = 0, p2_threads = 0;
     int p1_maxpts = 0, p2_maxpts = 0;
     int p1_cx = 0, p2_cx = 0;    // sub-plan uses composite register radices (10..32)

@@ -121,16 +121,20 @@ template <> struct Dft<8> {
 // Register radices.  R is a plain radix (2, 3, 4, 5, 8) or a composite
 // R = A*B (10 = 2*5, 16 = 4*4, 20 = 4*5, 25 = 5*5, 32 = 4*8).  A stage with
 // ns > 1 multiplies input r by W_D^{k r} (D = ns*R, k = q % ns).  A plain
-// radix reads those R-1 twiddles from its table; a composite one reads A-1
+// radix reads those R-1 twiddles from the stage table.  A composite reads A-1
 // values W_D^{k B n1} and B-1 values W_D^{k n2} and applies them around its
-// A-point DFTs, so a butterfly loads A+B-2 twiddles instead of R-1.
+// A-point DFTs: A+B-2 loads instead of R-1, for more multiplies (radix 16:
+// 6 loads and 24 multiplies against 15 and 15).  The row pass waits on
+// memory more than it computes, and R-1 loads made it 14% slower at 3.2M;
+// radix_factored_tw() selects the form per radix.
 __host__ __device__ constexpr int radix_a(int R)
 {
     return R == 10 ? 2 : R == 16 ? 4 : R == 20 ? 4 : R == 25 ? 5 : R == 32 ? 4 : R;
 }
+__host__ __device__ constexpr bool radix_factored_tw(int R) { return radix_a(R) != R; }
 __host__ __device__ constexpr int radix_ntw(int R)
 {
-    return radix_a(R) == R ? R - 1 : radix_a(R) + R / radix_a(R) - 2;
+    return radix_factored_tw(R) ? radix_a(R) + R / radix_a(R) - 2 : R - 1;
 }
 
 // Roots of unity at compile time: Taylor series on the angle reduced to (-pi, pi].
@@ -160,12 +164,12 @@ template <int R> struct RootTable {
     }
 };
 
-// Twiddle (when tw) + butterfly; output r goes to st(r, value).
+// Twiddle (TW: a stage after the first) + butterfly; output r goes to st(r, value).
 template <int R, bool COMP = (radix_a(R) != R)> struct DftStore {
-    template <class St>
-    __device__ __forceinline__ static void run(float2* v, const float2* __restrict__ w, bool tw, St st)
+    template <bool TW, class St>
+    __device__ __forceinline__ static void run(float2* v, const float2* __restrict__ w, St st)
     {
-        if (tw) {
+        if constexpr (TW) {
 #pragma unroll
             for (int r = 1; r < R; r++) v[r] = cmul(v[r], w[r - 1]);
         }
@@ -180,27 +184,33 @@ template <int R, bool COMP = (radix_a(R) != R)> struct DftStore {
 //                         sum_n1 (v[B n1 + n2] W_D^{k B n1}) W_A^{n1 k1}.
 // Step 1 (A-point DFTs + twiddles) runs in place; step 2 (B-point DFTs) hands
 // each output straight to the store, so no second register array is needed.
-// The twiddles are applied unconditionally: a first stage (ns = 1, k = 0)
-// reads its table's ones, which keeps one code path.
+// A first stage (ns = 1, k = 0) has only the internal W_R factors.  With
+// R-1 table twiddles (radix_factored_tw false) the stage factors multiply the
+// inputs up front, as for a plain radix.
 template <int R> struct DftStore<R, true> {
     static constexpr int A = radix_a(R), B = R / A;
+    static constexpr bool FT = radix_factored_tw(R);
     static constexpr RootTable<R> kRoots{};
-    template <class St>
-    __device__ __forceinline__ static void run(float2* v, const float2* __restrict__ w, bool, St st)
+    template <bool TW, class St>
+    __device__ __forceinline__ static void run(float2* v, const float2* __restrict__ w, St st)
     {
+        if constexpr (TW && !FT) {
+#pragma unroll
+            for (int r = 1; r < R; r++) v[r] = cmul(v[r], w[r - 1]);
+        }
 #pragma unroll
         for (int n2 = 0; n2 < B; n2++) {
             float2 u[A];
 #pragma unroll
             for (int n1 = 0; n1 < A; n1++) {
                 u[n1] = v[B * n1 + n2];
-                if (n1 > 0) u[n1] = cmul(u[n1], w[n1 - 1]);
+                if (TW && FT && n1 > 0) u[n1] = cmul(u[n1], w[n1 - 1]);
             }
             Dft<A>::run(u);
 #pragma unroll
             for (int k1 = 0; k1 < A; k1++) {
                 float2 x = u[k1];
-                if (n2 > 0) x = cmul(x, w[A - 1 + n2 - 1]);
+                if (TW && FT && n2 > 0) x = cmul(x, w[A - 1 + n2 - 1]);
                 const int e = (n2 * k1) % R;
                 v[B * k1 + n2] = (e == 0) ? x : cmul(x, make_float2(kRoots.re[e], kRoots.im[e]));
             }
@@ -218,8 +228,8 @@ template <int R> struct DftStore<R, true> {
 };
 
 // ---------------------------------------------------------------------------
-// Stockham stage in LDS.  Element e of the FFT lives at lds[ebase + (e << lge)]
-// (lge = log2 of the element stride: columns interleaved in pass 1).
+// Stockham stage in LDS.  Element e of the FFT lives at lds[ebase + e*es]
+// (es = element stride: the B columns of a pass-1 workgroup interleaved).
 // Butterfly q (q < L/R) reads q + r*L/R, writes (q - q%ns)*R + q%ns + r*ns.
 // Every thread of the workgroup must call it (it contains barriers).
 // Index products use 24-bit multiplies (full rate; every factor < 2^24).
@@ -233,34 +243,36 @@ __device__ __forceinline__ int mod_magic(int q, int ns, unsigned mag)
     return q - (int)__umul24((unsigned)d, (unsigned)ns);
 }
 
-template <int R, int NB>
+// TW: apply the stage twiddles (every stage after the first; a first stage,
+// ns = 1, has k = 0 and may also run twiddled with its table's ones).
+template <int R, int NB, bool TW>
 __device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int ns, unsigned mag,
-                                          const float2* __restrict__ tw, int ebase, int lge,
+                                          const float2* __restrict__ tw, int ebase, int es,
                                           int q0, int T)
 {
     const int nb = L / R;
-    const int rstr = nb << lge;    // uniform
+    const int rstr = (int)__umul24((unsigned)nb, (unsigned)es);    // uniform
     float2 v[NB][R];
 #pragma unroll
     for (int i = 0; i < NB; i++) {
         const int q = q0 + i * T;
         if (q < nb) {
-            const float2* __restrict__ src = lds + ebase + (q << lge);
+            const float2* __restrict__ src = lds + ebase + (int)__umul24((unsigned)q, (unsigned)es);
 #pragma unroll
             for (int r = 0; r < R; r++) v[i][r] = src[r * rstr];
         }
     }
     __syncthreads();
-    const int dstr = ns << lge;    // uniform
+    const int dstr = (int)__umul24((unsigned)ns, (unsigned)es);    // uniform
 #pragma unroll
     for (int i = 0; i < NB; i++) {
         const int q = q0 + i * T;
         if (q < nb) {
-            const int k = (ns > 1) ? mod_magic(q, ns, mag) : 0;
+            const int k = (TW && ns > 1) ? mod_magic(q, ns, mag) : 0;
             const int o = (int)__umul24((unsigned)(q - k), (unsigned)R) + k;
-            float2* __restrict__ dst = lds + ebase + (o << lge);
-            DftStore<R>::run(v[i], tw + (int)__umul24((unsigned)k, (unsigned)radix_ntw(R)), ns > 1,
-                             [&](int r, float2 x) { dst[r * dstr] = x; });
+            float2* __restrict__ dst = lds + ebase + (int)__umul24((unsigned)o, (unsigned)es);
+            DftStore<R>::template run<TW>(v[i], tw + (int)__umul24((unsigned)k, (unsigned)radix_ntw(R)),
+                                          [&](int r, float2 x) { dst[r * dstr] = x; });
         }
     }
     __syncthreads();
@@ -269,41 +281,54 @@ __device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int n
 // All radices a kernel instantiation supports are compiled into one switch and
 // the register allocation covers the largest, so the composite radices (CX)
 // live in their own instantiations: a plan uses them only where they pay.
+template <int MAXPTS, bool CX, bool TW>
+__device__ __forceinline__ void stage_any(float2* lds, const SubPlan& sp, const float2* __restrict__ stw, int s,
+                                          int ebase, int es, int q0, int T)
+{
+    const float2* tw = stw + sp.twoff[s];
+    const int ns = sp.ns[s];
+    const unsigned mag = sp.mag[s];
+    // Opaque per-stage copies: otherwise every case's per-thread index math
+    // is hoisted out of the stage loop and held live across all stages.
+    int q0s = q0, ebs = ebase;
+    asm volatile("" : "+v"(q0s), "+v"(ebs));
+    switch (sp.radix[s]) {
+    case 2: stage_lds<2, MAXPTS / 2, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T); break;
+    case 3: stage_lds<3, MAXPTS / 3, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T); break;
+    case 4: stage_lds<4, MAXPTS / 4, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T); break;
+    case 5: stage_lds<5, MAXPTS / 5, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T); break;
+    case 8: stage_lds<8, MAXPTS / 8, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T); break;
+    case 10:
+        if constexpr (CX) stage_lds<10, MAXPTS / 10, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T);
+        break;
+    case 16:
+        if constexpr (CX) stage_lds<16, MAXPTS / 16, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T);
+        break;
+    case 20:
+        if constexpr (CX && MAXPTS >= 20) stage_lds<20, MAXPTS / 20, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T);
+        break;
+    case 25:
+        if constexpr (CX && MAXPTS >= 25) stage_lds<25, MAXPTS / 25, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T);
+        break;
+    case 32:
+        // radix 32 always takes the twiddled form (its table holds ones for a
+        // first stage): without the multiplies the compiler schedules its
+        // 32-point butterfly into ~20 more registers, costing occupancy
+        if constexpr (CX && MAXPTS >= 32) stage_lds<32, MAXPTS / 32, true>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T);
+        break;
+    default: break;
+    }
+}
+
 template <int MAXPTS, bool CX>
 __device__ __forceinline__ void fft_lds(float2* lds, const SubPlan& sp, const float2* __restrict__ stw,
-                                        int ebase, int lge, int q0, int T)
+                                        int ebase, int es, int q0, int T, int nst)
 {
-    for (int s = 0; s < sp.nst; s++) {
-        const float2* tw = stw + sp.twoff[s];
-        const int ns = sp.ns[s];
-        const unsigned mag = sp.mag[s];
-        // Opaque per-stage copies: otherwise every case's per-thread index math
-        // is hoisted out of the stage loop and held live across all stages.
-        int q0s = q0, ebs = ebase;
-        asm volatile("" : "+v"(q0s), "+v"(ebs));
-        switch (sp.radix[s]) {
-        case 2: stage_lds<2, MAXPTS / 2>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T); break;
-        case 3: stage_lds<3, MAXPTS / 3>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T); break;
-        case 4: stage_lds<4, MAXPTS / 4>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T); break;
-        case 5: stage_lds<5, MAXPTS / 5>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T); break;
-        case 8: stage_lds<8, MAXPTS / 8>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T); break;
-        case 10:
-            if constexpr (CX) stage_lds<10, MAXPTS / 10>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T);
-            break;
-        case 16:
-            if constexpr (CX) stage_lds<16, MAXPTS / 16>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T);
-            break;
-        case 20:
-            if constexpr (CX && MAXPTS >= 20) stage_lds<20, MAXPTS / 20>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T);
-            break;
-        case 25:
-            if constexpr (CX && MAXPTS >= 25) stage_lds<25, MAXPTS / 25>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T);
-            break;
-        case 32:
-            if constexpr (CX && MAXPTS >= 32) stage_lds<32, MAXPTS / 32>(lds, sp.L, ns, mag, tw, ebs, lge, q0s, T);
-            break;
-        default: break;
-        }
+    // stages [0, nst); one loop (not a peeled first stage): the loop keeps the
+    // compiler from interleaving the first stage with the code before it
+    for (int s = 0; s < nst; s++) {
+        if (s == 0) stage_any<MAXPTS, CX, false>(lds, sp, stw, s, ebase, es, q0, T);
+        else stage_any<MAXPTS, CX, true>(lds, sp, stw, s, ebase, es, q0, T);
     }
 }
 
@@ -414,7 +439,7 @@ template <int U> __device__ __forceinline__ int batch_idx(int base, int u, int n
 struct P1Args {
     int N1, N2;
     long long M;
-    int B, lgB;
+    int B;            // columns per workgroup pipeline (any divisor of N2)
     int fpw, pth;     // FFT pipelines per workgroup, threads per pipeline (B * T1)
     int twshift;
     SubPlan sp;
@@ -467,7 +492,7 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     while (pipe + 1 < fpw && threadIdx.x >= (unsigned)((pipe + 1) * pth)) pipe++;
     const int f = (int)by * fpw + pipe;
     const bool active = f < J.nf;
-    const int B = a.B, lgB = a.lgB, N1 = a.N1, N2 = a.N2;
+    const int B = a.B, N1 = a.N1, N2 = a.N2;
     const int c0 = (int)bx * B;
     const int tid = threadIdx.x - pipe * pth, nth = pth;
     float2* __restrict__ lp = lds + pipe * (N1 * B);
@@ -487,7 +512,8 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     // e0 + u*estep of a batch take (cos, sin)(phi) as the product of the table
     // entries of e0 and of u*estep: one product each, nothing accumulates.
     const WindowGen wg = a.win;
-    const int bcol = tid & (B - 1);
+    const int et = tid / B;            // the thread's first row (and first butterfly)
+    const int bcol = tid - et * B;     // its column, fixed
     const int cb = c0 + bcol;
     float P[2] = {0.f, 0.f}, Q[2] = {0.f, 0.f}, R2[2] = {0.f, 0.f}, S2[2] = {0.f, 0.f};
     if (wg.kind != 0) {
@@ -512,7 +538,7 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     // are clamped to N1 - 1 for the loads of a partial batch (values unused).
     // Point indices are < M <= 2^30 and the factors < 2^14: 24-bit products.
     const int total = N1 * B;
-    const int estep = nth >> lgB;
+    const int estep = nth / B;
     // MODE 3 (VLBA): samples decoded through the FFT's row of blank zones
     const int2* __restrict__ brow =
         up.kind == kUnpackVLBA ? up.vblank + (size_t)(active ? f : 0) * up.vnseg : nullptr;
@@ -529,14 +555,14 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
         // batch size windows a row by the same operations.
         constexpr int G = 8;
         static_assert(G % U == 0, "batches must not straddle a phase group");
-        const int et = tid >> lgB;
-        // batch j: points k = j*U + u at base + u*nth; FULL: all of them in
-        // range (no clamps or checks), else the thread's last, partial batch
+        // batch j: points k = j*U + u at base + u*nth (row e0 + u*estep); FULL:
+        // all of them in range (no clamps or checks), else the thread's last,
+        // partial batch
         auto batch = [&](int base, int j, auto fullc) __attribute__((always_inline)) {
             constexpr bool FULL = decltype(fullc)::value;
             float x0[U], x1[U];
-            const int e0 = base >> lgB;
             const int k0 = j * U;
+            const int e0 = et + k0 * estep;
             const float2 anc = WM ? wg.phi[et + (k0 & ~(G - 1)) * estep] : make_float2(1.f, 0.f);
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -602,21 +628,88 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     }
     __syncthreads();
 
-    fft_lds<MAXPTS, CX>(lds, a.sp, a.stw, pipe * (N1 * B) + bcol, lgB, tid >> lgB, nth >> lgB);
+    // A plain last radix (2, 3, 4, 5, 8) after at least one LDS stage runs
+    // fused with the store below: its butterflies read LDS and write their
+    // outputs straight to Y, saving an LDS write, a barrier and a read per
+    // point.
+    const SubPlan& sp = a.sp;
+    const int rlast = sp.radix[sp.nst - 1];
+    const bool fuse = sp.nst >= 2 && (rlast == 2 || rlast == 3 || rlast == 4 || rlast == 5 || rlast == 8);
+    const int ebase = pipe * (N1 * B) + bcol;
+    fft_lds<MAXPTS, CX>(lds, sp, a.stw, ebase, B, et, estep, fuse ? sp.nst - 1 : sp.nst);
 
     float2* __restrict__ Y = J.Y + (long long)(active ? f : 0) * a.M;
     const unsigned mask = (1u << a.twshift) - 1u;
+    auto tw_at = [&](unsigned e) { return cmul(a.twA[e >> a.twshift], a.twB[e & mask]); };
+    if (fuse) {
+        if (!active) return;
+        // Last stage: butterfly q < ns = N1/R (k = q) reads q + r*ns and writes
+        // output row q + r*ns, times W_M^{cb (q + r ns)}.  A thread's butterflies
+        // q = et + i*estep take W_M^{cb q} from the split tables at the start of
+        // each chunk (<= 8 butterflies) and by the step W_M^{cb estep} within it;
+        // output r steps by W_M^{cb ns}.
+        auto last = [&](auto rc) __attribute__((always_inline)) {
+            constexpr int R = decltype(rc)::value;
+            constexpr int NB = MAXPTS / R;
+            const int ns = N1 / R;
+            const float2* __restrict__ tw = a.stw + sp.twoff[sp.nst - 1];
+            const float2 s_ns = tw_at(__umul24((unsigned)cb, (unsigned)ns));       // cb*ns < M
+            const float2 s_q = tw_at(__umul24((unsigned)cb, (unsigned)estep));     // cb*estep < M
+            const int rstr = (int)__umul24((unsigned)ns, (unsigned)B);
+            // Chunks of CH butterflies, every load of a chunk (inputs, stage
+            // twiddles, four-step anchor) before its first store: Y might alias
+            // the tables as far as the compiler knows, so a load placed after a
+            // store is issued late and its latency is exposed.
+            constexpr int CH = (16 / R) < NB ? (16 / R) : NB;   // ~50-60 VGPRs of data, <= 8
+            float2 wq = make_float2(1.f, 0.f);
+#pragma unroll
+            for (int i0 = 0; i0 < NB; i0 += CH) {
+                if (et + i0 * estep >= ns) break;
+                float2 v[CH][R], w[CH][R - 1], anc = make_float2(1.f, 0.f);
+#pragma unroll
+                for (int c = 0; c < CH; c++) {
+                    const int q = min(et + (i0 + c) * estep, ns - 1);   // clamped: values unused past ns
+                    const float2* __restrict__ src = lds + ebase + (int)__umul24((unsigned)q, (unsigned)B);
+#pragma unroll
+                    for (int r = 0; r < R; r++) v[c][r] = src[r * rstr];
+#pragma unroll
+                    for (int r = 0; r < R - 1; r++) w[c][r] = tw[(int)__umul24((unsigned)q, (unsigned)(R - 1)) + r];
+                    if (c == 0) anc = tw_at(__umul24((unsigned)cb, (unsigned)q));   // chunk anchor
+                }
+#pragma unroll
+                for (int c = 0; c < CH; c++) {
+                    const int i = i0 + c;
+                    const int q = et + i * estep;
+                    wq = (c == 0) ? anc : cmul(wq, s_q);
+                    if (q >= ns) continue;
+                    float2 wr = wq;
+                    DftStore<R>::template run<true>(v[c], w[c], [&](int r, float2 x) {
+                        if (r > 0) wr = cmul(wr, s_ns);
+                        const unsigned o = __umul24((unsigned)(q + r * ns), (unsigned)N2) + (unsigned)cb;
+                        Y[o] = cmul(x, wr);
+                    });
+                }
+            }
+        };
+        switch (rlast) {
+        case 2: last(std::integral_constant<int, 2>()); break;
+        case 3: last(std::integral_constant<int, 3>()); break;
+        case 4: last(std::integral_constant<int, 4>()); break;
+        case 5: last(std::integral_constant<int, 5>()); break;
+        default: last(std::integral_constant<int, 8>()); break;
+        }
+        return;
+    }
     constexpr int U = 8;
     // Y[k1*N2 + n2] = lp[k1*B + bcol] * W_M^{n2 k1}, n2 = cb fixed per thread.
     // Along a thread's rows k1 = k0 + u*estep the twiddle advances by the fixed
     // factor W_M^{cb*estep}: each batch of U rows takes W_M^{cb*k0} from the
     // split tables (FP64-derived, one product) and the other U-1 by complex
     // multiplication, so the relative error stays within ~U ulp.
-    auto tw_at = [&](unsigned e) { return cmul(a.twA[e >> a.twshift], a.twB[e & mask]); };
     const float2 tstep = tw_at(__umul24((unsigned)cb, (unsigned)estep));   // cb*estep < N2*N1 = M
-    auto store = [&](int base, auto fullc) __attribute__((always_inline)) {
+    // batch at lp[base + u*nth] = rows k0 + u*estep
+    auto store = [&](int base, int k0, auto fullc) __attribute__((always_inline)) {
         constexpr bool FULL = decltype(fullc)::value;   // all U rows in range
-        const int k0 = base >> lgB;
         float2 w = tw_at(__umul24((unsigned)cb, (unsigned)k0));              // n2 k1 < M
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -630,9 +723,9 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     };
     if (!active) return;
     const int nfull = total - (U - 1) * nth;
-    int base = tid;
-    for (; base < nfull; base += U * nth) store(base, std::true_type());
-    if (base < total) store(base, std::false_type());
+    int base = tid, k0 = et;
+    for (; base < nfull; base += U * nth, k0 += U * estep) store(base, k0, std::true_type());
+    if (base < total) store(base, k0, std::false_type());
 }
 
 // ---------------------------------------------------------------------------
@@ -741,7 +834,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
             }
         }
         __syncthreads();
-        fft_lds<MAXPTS, CX>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * N2, 0, q0, T);
+        fft_lds<MAXPTS, CX>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * N2, 1, q0, T, a.sp.nst);
 
         // each thread keeps the same bin pairs for every FFT of the slice
 #pragma unroll
@@ -1213,7 +1306,7 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
 {
     if (njobs < 1 || njobs > kMaxJobs || nfft < 1) return hipErrorInvalidValue;
     P1Args a;
-    a.N1 = p.N1; a.N2 = p.N2; a.M = p.M; a.B = p.B; a.lgB = p.lgB; a.twshift = p.twshift;
+    a.N1 = p.N1; a.N2 = p.N2; a.M = p.M; a.B = p.B; a.twshift = p.twshift;
     a.sp = p.sp1; a.stw = p.d_stw1; a.twA = p.d_twA; a.twB = p.d_twB; a.win = p.win;
     a.fpw = std::max(1, std::min(p.p1_fpw, nfft));
     a.zfast = 0;
@@ -1617,10 +1710,17 @@ std::string plan_geometry(Plan& p, long long N)
         while (B > 1 && B > bm) B >>= 1;
     }
     if (B == 0) return "column FFT too large";
+    // SWSPEC_P1_B forces a column count: any divisor of N2 works (the kernel
+    // takes B as a stride, not a shift).  The powers of two above stay the
+    // choice: B = 5 at 3.2M fills 250 of 256 lanes (B = 4: 200) and issues 20%
+    // fewer VALU instructions, but its 40-byte row segments of Y cost 40% more
+    // HBM write requests and its LDS stride conflicts, and it ran 19% slower.
+    if (const char* e = getenv("SWSPEC_P1_B")) {
+        const int c = atoi(e);
+        if (c >= 1 && p.N2 % c == 0 && c * T1 <= 512 && (size_t)c * p.N1 * sizeof(float2) <= 160 * 1024) B = c;
+    }
     p.sp1.T = T1;
     p.B = B;
-    p.lgB = 0;
-    while ((1 << p.lgB) < B) p.lgB++;
     p.p1_threads = B * T1;
 
     // pass 2: 2 rows (x2 for cross-power pairs) of N2 points, <= 16 bins/thread,
@@ -1666,7 +1766,7 @@ static void stage_twiddles(const SubPlan& sp, std::vector<float2>& out)
             out[sp.twoff[s] + k * nt + slot] = make_float2((float)cos(a), (float)sin(a));
         };
         for (int k = 0; k < ns; k++) {
-            if (A == R) {
+            if (!radix_factored_tw(R)) {
                 for (int r = 1; r < R; r++) put(k, r - 1, r);          // W_D^{k r}
             } else {
                 for (int n1 = 1; n1 < A; n1++) put(k, n1 - 1, (long long)Bc * n1);   // W_D^{k B n1}
