@@ -811,6 +811,13 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
     const int per = (nfj + a.S - 1) / a.S;   // FFTs per slice (uniform trip count)
     const int fb = sl_i * per;
     constexpr int U = 16 / NSL;
+    // split twiddles W_N^{k} of the thread's pairs, the same for every FFT:
+    // with up to 8 pairs per thread they are loaded once (the bin phase then
+    // issues no global loads); 16 would cost a wave of occupancy
+    constexpr bool WHOIST = MP <= 8;
+    float2 wpair[WHOIST ? MP : 1];
+#pragma unroll
+    for (int m = 0; m < (WHOIST ? MP : 0); m++) wpair[m] = cmul(a.rA[rowA], a.rB[min(tid + m * nth, N2 - 1)]);
     for (int it = 0; it < per; it++) {
         const int f = fb + it;
         const bool act = sact && f < nfj;
@@ -845,7 +852,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
             asm volatile("" : "+v"(pi));
             if (!act || pi >= npair) continue;
             const int mc = mirror_col(pi), ms = mirror_row();
-            const float2 w = cmul(a.rA[rowA], a.rB[pi]);
+            const float2 w = WHOIST ? wpair[WHOIST ? m : 0] : cmul(a.rA[rowA], a.rB[pi]);
             float2 X1[NSRC], X2[NSRC];
 #pragma unroll
             for (int s = 0; s < NSRC; s++) {
