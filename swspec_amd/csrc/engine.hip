@@ -811,48 +811,90 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
     const int per = (nfj + a.S - 1) / a.S;   // FFTs per slice (uniform trip count)
     const int fb = sl_i * per;
     constexpr int U = 16 / NSL;
-    // split twiddles W_N^{k} of the thread's pairs, the same for every FFT:
-    // with up to 8 pairs per thread they are loaded once (the bin phase then
-    // issues no global loads); 16 would cost a wave of occupancy
-    constexpr bool WHOIST = MP <= 8;
-    float2 wpair[WHOIST ? MP : 1];
+    // Split twiddles W_N^{rowA + N1 pi} of the thread's pairs pi = tid + m*nth,
+    // the same for every FFT: two values loaded once per slice, W for m = 0
+    // and the step W_N^{N1 nth} (rB holds W_N^{N1 k}); pair m takes the step
+    // m times (<= 7 products, a few ulp).  The bin phase then issues no
+    // global load, which would also wait for the rows being prefetched.
+    // (nth >= N2 leaves a single pair per thread and the step unused.)
+    const float2 wpair0 = cmul(a.rA[rowA], a.rB[min(tid, N2 - 1)]);
+    const float2 wstep = a.rB[min(nth, N2 - 1)];
+
+    auto row_src = [&](int sl, int f) {
+        return Yp[sl >> 1] + (long long)f * a.M + (long long)((sl & 1) ? rowB : rowA) * N2;
+    };
+    // 16-point plans (rows fit one batch of U points per thread; the launcher
+    // checks it) prefetch: FFT f+1's row loads are issued after FFT f's last
+    // stage, into registers the FFT no longer needs, and land while FFT f's
+    // bins are formed; they go to LDS after the bins.  Every path after the
+    // FFT defines t, so t is never live across the FFT's registers.  Other
+    // plans load each FFT's rows in batches at the top of its iteration.
+    constexpr bool PREF = MAXPTS == 16;
+    float2 t[NSL][U];
+    auto load_rows = [&](int f) __attribute__((always_inline)) {
 #pragma unroll
-    for (int m = 0; m < (WHOIST ? MP : 0); m++) wpair[m] = cmul(a.rA[rowA], a.rB[min(tid + m * nth, N2 - 1)]);
+        for (int sl = 0; sl < NSL; sl++) {
+            const float2* __restrict__ Ys = row_src(sl, f);
+#pragma unroll
+            for (int u = 0; u < U; u++) t[sl][u] = Ys[batch_idx<U>(tid, u, nth, N2)];
+        }
+    };
+    auto store_rows = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int sl = 0; sl < NSL; sl++)
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (tid + u * nth < N2) lds[sl * N2 + tid + u * nth] = t[sl][u];
+    };
+    if (PREF && sact && fb < nfj) load_rows(fb);
     for (int it = 0; it < per; it++) {
         const int f = fb + it;
         const bool act = sact && f < nfj;
-        if (act) {
+        if (PREF) {
+            if (act) store_rows();
+        } else if (act) {
             const float2* __restrict__ Ys[NSL];
 #pragma unroll
-            for (int sl = 0; sl < NSL; sl++)
-                Ys[sl] = Yp[sl >> 1] + (long long)f * a.M + (long long)((sl & 1) ? rowB : rowA) * N2;
+            for (int sl = 0; sl < NSL; sl++) Ys[sl] = row_src(sl, f);
             // all rows' loads of a batch in flight together
             for (int base = tid; base < N2; base += U * nth) {
-                float2 t[NSL][U];
+                float2 tb[NSL][U];
 #pragma unroll
                 for (int sl = 0; sl < NSL; sl++)
 #pragma unroll
-                    for (int u = 0; u < U; u++) t[sl][u] = Ys[sl][batch_idx<U>(base, u, nth, N2)];
+                    for (int u = 0; u < U; u++) tb[sl][u] = Ys[sl][batch_idx<U>(base, u, nth, N2)];
 #pragma unroll
                 for (int sl = 0; sl < NSL; sl++)
 #pragma unroll
                     for (int u = 0; u < U; u++)
-                        if (base + u * nth < N2) lds[sl * N2 + base + u * nth] = t[sl][u];
+                        if (base + u * nth < N2) lds[sl * N2 + base + u * nth] = tb[sl][u];
             }
         }
         __syncthreads();
         fft_lds<MAXPTS, CX>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * N2, 1, q0, T, a.sp.nst);
+        if (PREF) {
+            if (sact && f + 1 < nfj && it + 1 < per) {
+                load_rows(f + 1);
+            } else {
+#pragma unroll
+                for (int sl = 0; sl < NSL; sl++)
+#pragma unroll
+                    for (int u = 0; u < U; u++) t[sl][u] = make_float2(0.f, 0.f);
+            }
+        }
 
         // each thread keeps the same bin pairs for every FFT of the slice
+        float2 w = wpair0;
+        asm volatile("" : "+v"(w.x), "+v"(w.y));   // not hoisted: no 8 live pair twiddles
 #pragma unroll
         for (int m = 0; m < MP; m++) {
             // opaque per pair: keeps the unrolled pairs' loads and index math
             // from being hoisted and held live together
             int pi = tid + m * nth;
             asm volatile("" : "+v"(pi));
+            if (m > 0) w = cmul(w, wstep);
             if (!act || pi >= npair) continue;
             const int mc = mirror_col(pi), ms = mirror_row();
-            const float2 w = WHOIST ? wpair[WHOIST ? m : 0] : cmul(a.rA[rowA], a.rB[pi]);
             float2 X1[NSRC], X2[NSRC];
 #pragma unroll
             for (int s = 0; s < NSRC; s++) {
