@@ -252,7 +252,13 @@ __device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int n
 {
     const int nb = L / R;
     const int rstr = (int)__umul24((unsigned)nb, (unsigned)es);    // uniform
+    constexpr int NT = radix_ntw(R);
+    // Up to 8 twiddles per thread are loaded before the barrier, so their
+    // latency overlaps the LDS reads and the barrier wait (k = q % ns < ns
+    // indexes the table for any q).
+    constexpr bool PRE = TW && NB * NT <= 8;
     float2 v[NB][R];
+    float2 wr[PRE ? NB : 1][PRE ? NT : 1];
 #pragma unroll
     for (int i = 0; i < NB; i++) {
         const int q = q0 + i * T;
@@ -260,6 +266,12 @@ __device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int n
             const float2* __restrict__ src = lds + ebase + (int)__umul24((unsigned)q, (unsigned)es);
 #pragma unroll
             for (int r = 0; r < R; r++) v[i][r] = src[r * rstr];
+        }
+        if constexpr (PRE) {
+            const int k = ns > 1 ? mod_magic(q, ns, mag) : 0;
+            const float2* __restrict__ w = tw + (int)__umul24((unsigned)k, (unsigned)NT);
+#pragma unroll
+            for (int j = 0; j < NT; j++) wr[i][j] = w[j];
         }
     }
     __syncthreads();
@@ -271,7 +283,7 @@ __device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int n
             const int k = (TW && ns > 1) ? mod_magic(q, ns, mag) : 0;
             const int o = (int)__umul24((unsigned)(q - k), (unsigned)R) + k;
             float2* __restrict__ dst = lds + ebase + (int)__umul24((unsigned)o, (unsigned)es);
-            DftStore<R>::template run<TW>(v[i], tw + (int)__umul24((unsigned)k, (unsigned)radix_ntw(R)),
+            DftStore<R>::template run<TW>(v[i], PRE ? wr[PRE ? i : 0] : tw + (int)__umul24((unsigned)k, (unsigned)NT),
                                           [&](int r, float2 x) { dst[r * dstr] = x; });
         }
     }
@@ -1118,6 +1130,33 @@ __global__ void __launch_bounds__(1024) k_pcal_fast(PcArgs a)
             const int g = tid + gi * nth;
             if (g >= ngroups) break;
             int A[4] = {0, 0, 0, 0}, Bc[4] = {0, 0, 0, 0};
+            if constexpr (STEP == 1) {
+                // One byte per time sample: the 4 positions are the 4 bytes of
+                // one word, counted in parallel as byte lanes (SWAR) -- big
+                // (magnitude bit), big & sign, ~big & sign -- with
+                //   A = (#periods - #big) - 2 #(~big & sign),  B = #big - 2 #(big & sign)
+                // flushed to integers every 255 periods (byte-lane capacity).
+                constexpr unsigned M = 0x01010101u;
+                for (int pc = p0; pc < p1; pc += 255) {
+                    const int pe = min(pc + 255, p1);
+                    unsigned cb = 0, cn1 = 0, cn0 = 0;
+#pragma unroll 8
+                    for (int p = pc; p < pe; p++) {
+                        const unsigned w = *(const uint32_t*)(raw + (long long)p * L + 4 * g + cbyte) >> shift;
+                        const unsigned x = w >> 1;
+                        cb += x & M;
+                        cn1 += w & x & M;
+                        cn0 += w & ~x & M;
+                    }
+                    const int np = pe - pc;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int nb = (int)((cb >> (8 * u)) & 0xFFu);
+                        A[u] += (np - nb) - 2 * (int)((cn0 >> (8 * u)) & 0xFFu);
+                        Bc[u] += nb - 2 * (int)((cn1 >> (8 * u)) & 0xFFu);
+                    }
+                }
+            } else
             for (int p = p0; p < p1; p++) {
                 const long long j0 = (long long)p * L + 4 * g;
                 unsigned fld[4];
