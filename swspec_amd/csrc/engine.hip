@@ -1807,6 +1807,18 @@ std::string plan_geometry(Plan& p, long long N)
         const int c = atoi(e);
         if (c >= 1 && p.N2 % c == 0 && c * T1 <= 512 && (size_t)c * p.N1 * sizeof(float2) <= 160 * 1024) B = c;
     }
+    // Whole waves: a last wave with few threads issues every instruction at
+    // full cost.  Threads beyond what the FFT stages need idle there (q >=
+    // L/R) but share the decode/window fill and the Y store (3.2M: 4 x 50 ->
+    // 4 x 64 threads).  SWSPEC_WAVE_FILL bit 0 (pass 1), bit 1 (pass 2).
+    static const int wave_fill = [] {
+        const char* e = getenv("SWSPEC_WAVE_FILL");
+        return e ? atoi(e) & 3 : 3;
+    }();
+    if ((wave_fill & 1) && B * T1 > 128 && (B * T1) % 64 != 0) {
+        const int thr = ((B * T1 + 63) / 64) * 64;
+        if (thr % B == 0 && thr <= 512) T1 = thr / B;
+    }
     p.sp1.T = T1;
     p.B = B;
     p.p1_threads = B * T1;
@@ -1827,6 +1839,10 @@ std::string plan_geometry(Plan& p, long long N)
     int T2 = subplan_threads(p.sp2, p.p2_maxpts);
     if (4 * T2 > 512) { p.p2_maxpts = 32; p.sp2 = r32; p.p2_cx = rx32; T2 = subplan_threads(p.sp2, 32); }
     if (4 * T2 > 512) return "row FFT too large";
+    // whole waves for the two rows of a pair (the extra threads share the
+    // row loads and the bin pairs), e.g. 3.2M: 2 x 86 -> 2 x 96
+    if ((wave_fill & 2) && 2 * T2 > 64 && (2 * T2) % 64 != 0 && 4 * (((2 * T2 + 63) / 64) * 32) <= 512)
+        T2 = ((2 * T2 + 63) / 64) * 32;
     p.sp2.T = T2;
     p.p2_threads = 2 * T2;
     p.npairs = p.N1 / 2 + 1;
