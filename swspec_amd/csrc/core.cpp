@@ -626,26 +626,34 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
         ProfScope psf(c, SWSPEC_K_FINALIZE);
         // TaskCoreIPP.cpp:491-498 scale 1/K for complete spectra; partial unscaled
         const double scale = w.complete ? 1.0 / (double)g.core_overlapped_ffts : 1.0;
-        for (int j = 0; j < c->njobs; j++) {
-            float* acc = c->d_acc + (size_t)j * p.smax * 2 * M1;
-            HIPCHK(c, launch_finalize(p, acc, d_auto[j] + sp * (size_t)ssb, scale, p.smax, c->stream));
-            HIPCHK(c, launch_zero(acc, (size_t)p.smax * 2 * M1 * sizeof(float), c->stream));
+        {   // every job in one launch; the accumulators read are cleared in it
+            float* accs[kMaxJobs];
+            float* outs[kMaxJobs];
+            for (int j = 0; j < c->njobs; j++) {
+                accs[j] = c->d_acc + (size_t)j * p.smax * 2 * M1;
+                outs[j] = d_auto[j] + sp * (size_t)ssb;
+            }
+            HIPCHK(c, launch_finalize_jobs(p, accs, outs, c->njobs, scale, p.smax, false, true, c->stream));
         }
         if (c->nx) {
-            HIPCHK(c, launch_finalize_xpol(p, c->d_xacc, d_xpol + sp * 2 * (size_t)ssb, scale, p.smax, c->stream));
-            HIPCHK(c, launch_zero(c->d_xacc, (size_t)p.smax * 4 * M1 * sizeof(float), c->stream));
+            float* xout = d_xpol + sp * 2 * (size_t)ssb;
+            HIPCHK(c, launch_finalize_jobs(p, &c->d_xacc, &xout, 1, scale, p.smax, true, true, c->stream));
         }
         if (g.extract_pcal) {
             if (c->pstream) {   // the spectrum's folds are done before they are emitted
                 HIPCHK(c, hipEventRecord(c->ev_pc, c->pstream));
                 HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_pc, 0));
             }
+            double* paccs[kMaxJobs];
+            float* pouts[kMaxJobs];
+            double* touts[kMaxJobs];
             for (int j = 0; j < c->njobs; j++) {
-                float* po = d_pcal && d_pcal[j] ? d_pcal[j] + sp * 2 * (size_t)c->pg.T : nullptr;
-                double* to = d_tones ? d_tones + ((size_t)j * sched.size() + sp) * 2 * c->ntones : nullptr;
-                HIPCHK(c, launch_pcal_emit(c->pg, c->d_pcal_acc + (size_t)j * c->pg.T * 2, po, to, c->ntones,
-                                           c->tone_step, c->stream));
+                paccs[j] = c->d_pcal_acc + (size_t)j * c->pg.T * 2;
+                pouts[j] = d_pcal && d_pcal[j] ? d_pcal[j] + sp * 2 * (size_t)c->pg.T : nullptr;
+                touts[j] = d_tones ? d_tones + ((size_t)j * sched.size() + sp) * 2 * c->ntones : nullptr;
             }
+            HIPCHK(c, launch_pcal_emit_jobs(c->pg, paccs, pouts, touts, c->njobs, c->ntones, c->tone_step,
+                                            c->stream));
             if (c->pstream) {   // the next spectrum's folds add to the cleared accumulators
                 HIPCHK(c, hipEventRecord(c->ev_em, c->stream));
                 HIPCHK(c, hipStreamWaitEvent(c->pstream, c->ev_em, 0));

@@ -190,6 +190,13 @@ hipError_t launch_finalize_xpol(const Plan& p, const float* acc, float* out, dou
 hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s);
 hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int njobs, int nfft,
                        hipStream_t s);
+// All jobs of a spectrum in one launch: acc[j] holds nslices slices of
+// (2 floats; cross-power: 4) x (M+1) entries, out[j] the natural-order
+// spectrum; zero clears what is read (no separate fill).
+hipError_t launch_finalize_jobs(const Plan& p, float* const* acc, float* const* out, int njobs, double scale,
+                                int nslices, bool xpol, bool zero, hipStream_t s);
+hipError_t launch_pcal_emit_jobs(const PcalGeom& g, double* const* acc, float* const* out, double* const* tones,
+                                 int njobs, int ntones, int tone_step, hipStream_t s);
 hipError_t launch_pcal_emit(const PcalGeom& g, double* acc, float* out, double* tones, int ntones,
                             int tone_step, hipStream_t s);
 hipError_t launch_unpack(const uint8_t* raw, Unpack up, long long count, float* out, hipStream_t s);

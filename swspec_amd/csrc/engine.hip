@@ -1046,6 +1046,81 @@ __global__ void k_finalize_xpol(float* __restrict__ acc, float* __restrict__ out
     }
 }
 
+// All jobs of a spectrum in one launch, through 32 x 32 tiles: a tile's
+// accumulator entries (k1, k2) are read along k2 (contiguous, 256-byte row
+// segments), summed over slices in slice order in FP64, scaled, and written
+// along k1 to natural bin k = k2 N1 + k1 through LDS, so reads and writes are
+// both coalesced; with `zero` the entries read are cleared for the next
+// spectrum in the same pass.  Grid (ceil(N2/32), ceil(N1/32), jobs); the
+// Nyquist entry (index M) is taken by tile (0, 0).
+struct FinArgs {
+    long long M;
+    int N1, N2;
+    double scale;
+    int S;
+    long long stride;   // floats between slices
+    int zero;
+    float* acc[kMaxJobs];
+    float* out[kMaxJobs];
+};
+
+template <bool XP>
+__global__ void __launch_bounds__(256) k_finalize_t(FinArgs a)
+{
+    constexpr int W = XP ? 4 : 2;   // floats per accumulator entry
+    __shared__ float2 tile[32][33];
+    float* __restrict__ acc = a.acc[blockIdx.z];
+    float* __restrict__ out = a.out[blockIdx.z];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int k2b = blockIdx.x * 32, k1b = blockIdx.y * 32;
+    auto entry = [&](long long idx) -> float2 {
+        double re = 0.0, im = 0.0;
+        for (int s = 0; s < a.S; s++) {
+            float* e = acc + s * a.stride + W * idx;
+            if (XP) {
+                const float4 v = *(const float4*)e;
+                re += (double)v.x - (double)v.z;
+                im += (double)v.y - (double)v.w;
+                if (a.zero) *(float4*)e = make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                const float2 v = *(const float2*)e;
+                re += (double)v.x - (double)v.y;
+                if (a.zero) *(float2*)e = make_float2(0.f, 0.f);
+            }
+        }
+        return make_float2((float)(re * a.scale), (float)(im * a.scale));
+    };
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int k1 = k1b + ty + 8 * r, k2 = k2b + tx;
+        if (k1 < a.N1 && k2 < a.N2) tile[ty + 8 * r][tx] = entry((long long)k1 * a.N2 + k2);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int k2 = k2b + ty + 8 * r, k1 = k1b + tx;
+        if (k1 < a.N1 && k2 < a.N2) {
+            const long long k = (long long)k2 * a.N1 + k1;
+            const float2 v = tile[tx][ty + 8 * r];
+            if (XP) {
+                out[2 * k] = v.x;
+                out[2 * k + 1] = k == 0 ? 0.0f : v.y;   // DC cross product is real (TaskCoreIPP.cpp:482-485)
+            } else {
+                out[k] = v.x;
+            }
+        }
+    }
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        const float2 v = entry(a.M);   // Nyquist
+        if (XP) {
+            out[2 * a.M] = v.x;
+            out[2 * a.M + 1] = 0.0f;
+        } else {
+            out[a.M] = v.x;
+        }
+    }
+}
+
 __global__ void k_zero(uint32_t* __restrict__ p, size_t nwords)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x)
@@ -1252,8 +1327,8 @@ __global__ void __launch_bounds__(1024) k_pcal_reduce(PcRedArgs a)
 
 // Folded vector out (FP32, as the reference's pcal sink carries) and tones in
 // FP64: tone n = (1/T) sum_b acc[b] e^{-2 pi i (n*step) b / T} (PCal.cpp:489-515).
-__global__ void k_pcal_emit(double* __restrict__ acc, float* __restrict__ out, double* __restrict__ tones,
-                            int T, int ntones, int step)
+__device__ __forceinline__ void pcal_emit_one(double* __restrict__ acc, float* __restrict__ out,
+                                              double* __restrict__ tones, int T, int ntones, int step)
 {
     __shared__ double2 s[1024];
     const int t = threadIdx.x;
@@ -1283,6 +1358,26 @@ __global__ void k_pcal_emit(double* __restrict__ acc, float* __restrict__ out, d
         acc[2 * t] = 0.0;
         acc[2 * t + 1] = 0.0;
     }
+}
+
+__global__ void k_pcal_emit(double* __restrict__ acc, float* __restrict__ out, double* __restrict__ tones,
+                            int T, int ntones, int step)
+{
+    pcal_emit_one(acc, out, tones, T, ntones, step);
+}
+
+// every job of a spectrum in one launch: block j emits job j
+struct EmitArgs {
+    int T, ntones, step;
+    double* acc[kMaxJobs];
+    float* out[kMaxJobs];
+    double* tones[kMaxJobs];
+};
+
+__global__ void k_pcal_emit_jobs(EmitArgs a)
+{
+    const int j = blockIdx.x;
+    pcal_emit_one(a.acc[j], a.out[j], a.tones[j], a.T, a.ntones, a.step);
 }
 
 // Decoder only: samples [0, count) of one extract_samples call (VLBA: the
@@ -1583,6 +1678,33 @@ hipError_t launch_pcal_emit(const PcalGeom& g, double* acc, float* out, double* 
     const int th = ((std::max(g.T, ntones) + 63) / 64) * 64;
     if (th > 1024) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_pcal_emit, dim3(1), dim3(th), 0, s, acc, out, tones, g.T, ntones, tone_step);
+    return hipGetLastError();
+}
+
+hipError_t launch_pcal_emit_jobs(const PcalGeom& g, double* const* acc, float* const* out, double* const* tones,
+                                 int njobs, int ntones, int tone_step, hipStream_t s)
+{
+    const int th = ((std::max(g.T, ntones) + 63) / 64) * 64;
+    if (th > 1024 || njobs < 1 || njobs > kMaxJobs) return hipErrorInvalidValue;
+    EmitArgs a;
+    a.T = g.T; a.ntones = ntones; a.step = tone_step;
+    for (int j = 0; j < njobs; j++) { a.acc[j] = acc[j]; a.out[j] = out[j]; a.tones[j] = tones[j]; }
+    hipLaunchKernelGGL(k_pcal_emit_jobs, dim3(njobs), dim3(th), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize_jobs(const Plan& p, float* const* acc, float* const* out, int njobs, double scale,
+                                int nslices, bool xpol, bool zero, hipStream_t s)
+{
+    if (njobs < 1 || njobs > kMaxJobs || nslices < 1) return hipErrorInvalidValue;
+    FinArgs a;
+    a.M = p.M; a.N1 = p.N1; a.N2 = p.N2; a.scale = scale; a.S = nslices;
+    a.stride = (xpol ? 4 : 2) * (p.M + 1);
+    a.zero = zero ? 1 : 0;
+    for (int j = 0; j < njobs; j++) { a.acc[j] = acc[j]; a.out[j] = out[j]; }
+    const dim3 grid((p.N2 + 31) / 32, (p.N1 + 31) / 32, njobs);
+    if (xpol) hipLaunchKernelGGL(k_finalize_t<true>, grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_finalize_t<false>, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -343,8 +343,8 @@ static int step_enqueue(swspec_hip_server* s, bool capture, std::vector<SJob*>* 
                 SJob* j = js[b + k];
                 j->done += jd[k].nf;
                 if (j->done >= j->d.nffts) {
-                    SCHK(s, launch_finalize(*p, j->acc, j->d.d_out_auto, 1.0 / (double)j->d.nffts, p->smax, st));
-                    SCHK(s, launch_zero(j->acc, (size_t)p->smax * 2 * (size_t)(p->M + 1) * sizeof(float), st));
+                    SCHK(s, launch_finalize_jobs(*p, &j->acc, &j->d.d_out_auto, 1, 1.0 / (double)j->d.nffts,
+                                                 p->smax, false, true, st));
                     if (j->pcal)
                         SCHK(s, launch_pcal_emit(j->pg, j->pcal_acc, j->d.d_out_pcal, j->d.d_tones, j->ntones,
                                                  j->tone_step, st));
