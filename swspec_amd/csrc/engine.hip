@@ -1763,21 +1763,37 @@ static int radices_threads(int L, const int* rad, int nst, int maxpts)
 // search takes the fewest stages from the register radices that fit maxpts
 // points per thread (with `composite`, 10, 16, 20, 25, 32 as well as 2, 3, 4,
 // 5, 8), then the fewest threads per FFT.
-static bool subplan_radices(int L, SubPlan& sp, int maxpts, bool composite)
+static void subplan_from(int L, const int* best, int bn, SubPlan& sp);
+
+// fused_last: the column pass, whose last stage (a plain radix) runs fused
+// with the Y store.  Among the shortest sequences it prefers the one with the
+// largest plain radix (fewest fused rounds per thread), then the fewest
+// threads, and orders composites first, smallest first, then plain radices
+// smallest first, so the largest plain radix is last (3.2M columns: 10·25·5
+// ran 4% faster than 25·25·2).
+static bool subplan_radices(int L, SubPlan& sp, int maxpts, bool composite, bool fused_last = false)
 {
     static const int kAll[] = {32, 25, 20, 16, 10, 8, 5, 4, 3, 2};
     static const int kBasic[] = {8, 5, 4, 3, 2};
     const int* set = composite ? kAll : kBasic;
     const int nset = composite ? 10 : 5;
-    int best[kMaxStages], bn = 0, bT = 0;
+    int best[kMaxStages], bn = 0, bT = 0, bP = 0;
     int cur[kMaxStages];
+    auto plain_max = [&](int depth) {
+        int m = 0;
+        for (int i = 0; i < depth; i++)
+            if (radix_a(cur[i]) == cur[i] && cur[i] > m) m = cur[i];
+        return m;
+    };
     // depth-first over non-increasing radix sequences
     auto dfs = [&](auto&& self, int n, int depth, int from) -> void {
         if (n == 1) {
             const int T = radices_threads(L, cur, depth, maxpts);
-            if (bn == 0 || depth < bn || (depth == bn && T < bT)) {
+            const int P = fused_last ? plain_max(depth) : 0;
+            if (bn == 0 || depth < bn || (depth == bn && (P > bP || (P == bP && T < bT)))) {
                 bn = depth;
                 bT = T;
+                bP = P;
                 for (int i = 0; i < depth; i++) best[i] = cur[i];
             }
             return;
@@ -1792,15 +1808,53 @@ static bool subplan_radices(int L, SubPlan& sp, int maxpts, bool composite)
     };
     dfs(dfs, L, 0, 0);
     if (bn == 0) return false;
-    // Stage order: odd radices first (largest first), then even ones smallest
-    // first.  A first stage (ns = 1) writes with a stride of R elements, which
-    // for an even R maps many lanes of a b64 access onto few LDS banks; an odd
-    // stride spreads them.  (Order does not change the arithmetic.)
-    std::stable_sort(best, best + bn, [](int x, int y) {
-        const bool ox = (x & 1) != 0, oy = (y & 1) != 0;
-        if (ox != oy) return ox;
-        return ox ? x > y : x < y;
-    });
+    if (fused_last && bP > 0) {
+        std::stable_sort(best, best + bn, [](int x, int y) {
+            const bool cx = radix_a(x) != x, cy = radix_a(y) != y;
+            if (cx != cy) return cx;
+            return x < y;
+        });
+    } else {
+        // Stage order: odd radices first (largest first), then even ones
+        // smallest first.  A first stage (ns = 1) writes with a stride of R
+        // elements, which for an even R maps many lanes of a b64 access onto
+        // few LDS banks; an odd stride spreads them.  (Order does not change
+        // the arithmetic.)
+        std::stable_sort(best, best + bn, [](int x, int y) {
+            const bool ox = (x & 1) != 0, oy = (y & 1) != 0;
+            if (ox != oy) return ox;
+            return ox ? x > y : x < y;
+        });
+    }
+    subplan_from(L, best, bn, sp);
+    return true;
+}
+
+// A sub-plan from an explicit stage sequence (tuning: SWSPEC_P2_RADICES);
+// false unless the radices are supported, fit maxpts and multiply to L.
+static bool subplan_forced(int L, const char* list, int maxpts, SubPlan& sp, int& cx)
+{
+    int rad[kMaxStages], n = 0;
+    long long prod = 1;
+    cx = 0;
+    for (const char* q = list; *q && n < kMaxStages;) {
+        const int R = atoi(q);
+        const bool ok = R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 10 || R == 16 || R == 20 ||
+                        R == 25 || R == 32;
+        if (!ok || R > maxpts) return false;
+        if (radix_a(R) != R) cx = 1;
+        rad[n++] = R;
+        prod *= R;
+        while (*q && *q != ',') q++;
+        if (*q == ',') q++;
+    }
+    if (n == 0 || prod != L) return false;
+    subplan_from(L, rad, n, sp);
+    return true;
+}
+
+static void subplan_from(int L, const int* best, int bn, SubPlan& sp)
+{
     int ns = 1, twl = 0;
     sp.L = L;
     sp.nst = bn;
@@ -1814,7 +1868,6 @@ static bool subplan_radices(int L, SubPlan& sp, int maxpts, bool composite)
         ns *= R;
     }
     sp.twlen = twl;
-    return true;
 }
 
 static int subplan_threads(const SubPlan& sp, int maxpts)
@@ -1850,20 +1903,24 @@ std::string plan_geometry(Plan& p, long long N)
     int cxmask = 3;
     if (const char* e = getenv("SWSPEC_COMPOSITE")) cxmask = atoi(e) & 3;
     // sub-plan for (maxpts, composite); composite only where it saves a stage
-    auto subplan = [&](int L, int maxpts, bool cx, SubPlan& out, int& use_cx) -> bool {
+    static const bool p1_order = [] {
+        const char* e = getenv("SWSPEC_P1_ORDER");
+        return !(e && atoi(e) == 0);
+    }();
+    auto subplan = [&](int L, int maxpts, bool cx, SubPlan& out, int& use_cx, bool fl = false) -> bool {
         SubPlan b;
-        if (!subplan_radices(L, b, maxpts, false)) return false;
+        if (!subplan_radices(L, b, maxpts, false, fl)) return false;
         out = b;
         use_cx = 0;
         if (cx) {
             SubPlan c;
-            if (subplan_radices(L, c, maxpts, true) && c.nst < b.nst) { out = c; use_cx = 1; }
+            if (subplan_radices(L, c, maxpts, true, fl) && c.nst < b.nst) { out = c; use_cx = 1; }
         }
         return true;
     };
     SubPlan c16, c32;
     int cx16 = 0, cx32 = 0;
-    if (!subplan(p.N1, 16, cxmask & 1, c16, cx16) || !subplan(p.N1, 32, cxmask & 1, c32, cx32))
+    if (!subplan(p.N1, 16, cxmask & 1, c16, cx16, p1_order) || !subplan(p.N1, 32, cxmask & 1, c32, cx32, p1_order))
         return "radix decomposition failed";
 
     // pass 1: B columns of N1 points (B a power of two dividing N2), <= 512
@@ -1915,6 +1972,16 @@ std::string plan_geometry(Plan& p, long long N)
             B = pick_B(T1);
         }
     }
+    if (const char* e = getenv("SWSPEC_P1_RADICES")) {   // tuning: explicit column-stage sequence
+        SubPlan f;
+        int fcx = 0;
+        if (subplan_forced(p.N1, e, p.p1_maxpts, f, fcx)) {
+            p.sp1 = f;
+            p.p1_cx = fcx;
+            T1 = subplan_threads(p.sp1, p.p1_maxpts);
+            B = pick_B(T1);
+        }
+    }
     if (const char* e = getenv("SWSPEC_P1_BMAX")) {
         const int bm = atoi(e);
         while (B > 1 && B > bm) B >>= 1;
@@ -1958,6 +2025,11 @@ std::string plan_geometry(Plan& p, long long N)
     }
     p.sp2 = p.p2_maxpts == 16 ? r16 : r32;
     p.p2_cx = p.p2_maxpts == 16 ? rx16 : rx32;
+    if (const char* e = getenv("SWSPEC_P2_RADICES")) {   // tuning: explicit row-stage sequence
+        SubPlan f;
+        int fcx = 0;
+        if (subplan_forced(p.N2, e, p.p2_maxpts, f, fcx)) { p.sp2 = f; p.p2_cx = fcx; }
+    }
     int T2 = subplan_threads(p.sp2, p.p2_maxpts);
     if (4 * T2 > 512) { p.p2_maxpts = 32; p.sp2 = r32; p.p2_cx = rx32; T2 = subplan_threads(p.sp2, 32); }
     if (4 * T2 > 512) return "row FFT too large";
