@@ -45,6 +45,13 @@ struct swspec_hip_core {
     // runs beside the FFT passes; events order it against input and emission
     hipStream_t pstream = nullptr;
     hipEvent_t ev_in = nullptr, ev_pc = nullptr, ev_em = nullptr;
+    // row pass on its own stream over two Y buffers: the column pass of chunk
+    // i+1 runs beside the row pass of chunk i (one is VALU-bound, the other
+    // waits on memory as much as it computes); ev_p1/ev_p2 order each buffer's
+    // writer and reader
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_p1[2] = {nullptr, nullptr}, ev_p2[2] = {nullptr, nullptr};
+    int ybufs = 1;
     Plan plan;
     PcalGeom pg;
     float2* d_rot = nullptr;
@@ -333,7 +340,7 @@ static int core_alloc(swspec_hip_core* c)
     const Plan& p = c->plan;
     const size_t M1 = (size_t)p.M + 1;
     const size_t S = (size_t)p.smax;   // accumulator slices per job
-    HIPCHK(c, hipMalloc(&c->d_Y, (size_t)c->njobs * c->G * (size_t)p.M * sizeof(float2)));
+    HIPCHK(c, hipMalloc(&c->d_Y, (size_t)c->ybufs * c->njobs * c->G * (size_t)p.M * sizeof(float2)));
     HIPCHK(c, hipMalloc(&c->d_acc, (size_t)c->njobs * S * 2 * M1 * sizeof(float)));
     HIPCHK(c, (hipError_t)launch_zero(c->d_acc, (size_t)c->njobs * S * 2 * M1 * sizeof(float), c->stream));
     if (c->nx) {
@@ -424,6 +431,22 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
         c->G = (int)std::max(1.0, std::min((double)gcap, std::floor(1.05e9 / per)));
     }
     c->G = std::max(1, std::min(c->G, cfg->core_overlapped_ffts));
+    {
+        // Row pass on its own stream for one-job cores whose spectra span
+        // several chunks: the column pass of the next chunk then fills the row
+        // pass's launch tail (single-channel 3.2M cores with phase-cal: +8%).
+        // Multi-job cores (4-channel 3.2M: -2%) and one-chunk spectra (no
+        // next chunk, only the stream handoffs) keep both passes on one
+        // stream.  SWSPEC_ROW_STREAM=1/0 forces it on/off.
+        bool on = c->njobs == 1 && cfg->core_overlapped_ffts > c->G;
+        if (const char* e = getenv("SWSPEC_ROW_STREAM")) on = atoi(e) != 0;
+        if (on) {
+            HIPCHK(c, hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking));   // default priority
+            for (hipEvent_t* ev : {&c->ev_p1[0], &c->ev_p1[1], &c->ev_p2[0], &c->ev_p2[1]})
+                HIPCHK(c, hipEventCreateWithFlags(ev, hipEventDisableTiming));
+            c->ybufs = 2;
+        }
+    }
 
     if (cfg->extract_pcal) {
         const int T = cfg->pcal_tonebins, L = cfg->pcal_rotatorlen;
@@ -500,6 +523,8 @@ extern "C" int swspec_hip_finalize(swspec_hip_core* c)
 {
     if (!c) return fail(nullptr, SWSPEC_EINVAL, "null core");
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->s2) (void)hipStreamSynchronize(c->s2);
+    if (c->pstream) (void)hipStreamSynchronize(c->pstream);
     plan_destroy(c->plan);
     for (void* p : {(void*)c->d_rot, (void*)c->d_Y, (void*)c->d_acc, (void*)c->d_xacc, (void*)c->d_pcal_part,
                     (void*)c->d_pcal_acc, (void*)c->d_in[0], (void*)c->d_in[1], (void*)c->d_out, (void*)c->d_tones,
@@ -508,9 +533,11 @@ extern "C" int swspec_hip_finalize(swspec_hip_core* c)
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->pstream) (void)hipStreamSynchronize(c->pstream);
-    for (hipEvent_t e : {c->ev_in, c->ev_pc, c->ev_em})
+    if (c->s2) (void)hipStreamSynchronize(c->s2);
+    for (hipEvent_t e : {c->ev_in, c->ev_pc, c->ev_em, c->ev_p1[0], c->ev_p1[1], c->ev_p2[0], c->ev_p2[1]})
         if (e) (void)hipEventDestroy(e);
     if (c->pstream) (void)hipStreamDestroy(c->pstream);
+    if (c->s2) (void)hipStreamDestroy(c->s2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SWSPEC_OK;
@@ -574,10 +601,17 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
         HIPCHK(c, hipEventRecord(c->ev_in, c->stream));
         HIPCHK(c, hipStreamWaitEvent(c->pstream, c->ev_in, 0));
     }
+    hipStream_t rst = c->s2 ? c->s2 : c->stream;   // row-pass stream
+    long long ci = 0;                             // chunk counter: Y buffer ci % ybufs
+    int ylast = 0;
     for (size_t sp = 0; sp < sched.size(); sp++) {
         const SpecWork& w = sched[sp];
-        for (int f0 = 0; f0 < w.nfft; f0 += c->G) {
+        for (int f0 = 0; f0 < w.nfft; f0 += c->G, ci++) {
             const int nf = std::min(c->G, w.nfft - f0);
+            const int yb = (int)(ci % c->ybufs);
+            ylast = yb;
+            // the row pass of chunk ci-2 has finished reading this Y buffer
+            if (c->s2 && ci >= 2) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_p2[yb], 0));
             JobDev jobs[kMaxJobs];
             bool any_pcal = false;
             for (int j = 0; j < c->njobs; j++) {
@@ -594,7 +628,7 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
                 J.fft_stride = g.raw_overlap_bytes;
                 J.nf = nf;
                 J.up = c->up[j];
-                J.Y = c->d_Y + (size_t)j * c->G * (size_t)p.M;
+                J.Y = c->d_Y + ((size_t)yb * c->njobs + j) * c->G * (size_t)p.M;
                 J.acc = c->d_acc + (size_t)j * p.smax * 2 * M1;
                 if (g.extract_pcal) {
                     J.pcal_part = c->d_pcal_part + (size_t)j * c->G * c->pg.nblk_max * c->pg.T * 2;
@@ -611,6 +645,10 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
                 if (c->vlba) HIPCHK(c, launch_vlba_blank(jobs, c->njobs, nf, g.fft_points, c->stream));
                 HIPCHK(c, launch_pass1(p, jobs, c->njobs, nf, c->stream));
             }
+            if (c->s2) {   // rows after their columns
+                HIPCHK(c, hipEventRecord(c->ev_p1[yb], c->stream));
+                HIPCHK(c, hipStreamWaitEvent(c->s2, c->ev_p1[yb], 0));
+            }
             if (any_pcal) {
                 ProfScope ps(c, SWSPEC_K_PCAL, pst);
                 HIPCHK(c, launch_pcal(c->pg, g.fft_points, jobs, c->njobs, nf, pst));
@@ -618,11 +656,15 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
             XpolDev xp;
             xp.acc = c->d_xacc;
             {
-                ProfScope ps(c, SWSPEC_K_PASS2);
+                ProfScope ps(c, SWSPEC_K_PASS2, rst);
                 HIPCHK(c, launch_pass2(p, jobs, c->njobs, c->nx ? &xp : nullptr, nf,
-                                       (g.flags & SWSPEC_FLAG_NO_KAHAN) ? 0 : 1, p.smax, c->stream));
+                                       (g.flags & SWSPEC_FLAG_NO_KAHAN) ? 0 : 1, p.smax, rst));
             }
+            if (c->s2) HIPCHK(c, hipEventRecord(c->ev_p2[yb], c->s2));
         }
+        // the spectrum's rows are summed before finalize reads and clears the
+        // accumulators (later rows wait for later columns, queued after it)
+        if (c->s2) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_p2[ylast], 0));
         ProfScope psf(c, SWSPEC_K_FINALIZE);
         // TaskCoreIPP.cpp:491-498 scale 1/K for complete spectra; partial unscaled
         const double scale = w.complete ? 1.0 / (double)g.core_overlapped_ffts : 1.0;
