@@ -334,11 +334,11 @@ __device__ __forceinline__ void stage_any(float2* lds, const SubPlan& sp, const 
 
 template <int MAXPTS, bool CX>
 __device__ __forceinline__ void fft_lds(float2* lds, const SubPlan& sp, const float2* __restrict__ stw,
-                                        int ebase, int es, int q0, int T, int nst)
+                                        int ebase, int es, int q0, int T, int nst, int s0 = 0)
 {
-    // stages [0, nst); one loop (not a peeled first stage): the loop keeps the
-    // compiler from interleaving the first stage with the code before it
-    for (int s = 0; s < nst; s++) {
+    // stages [s0, nst); one loop (not a peeled first stage): the loop keeps
+    // the compiler from interleaving the first stage with the code before it
+    for (int s = s0; s < nst; s++) {
         if (s == 0) stage_any<MAXPTS, CX, false>(lds, sp, stw, s, ebase, es, q0, T);
         else stage_any<MAXPTS, CX, true>(lds, sp, stw, s, ebase, es, q0, T);
     }
@@ -763,7 +763,8 @@ struct P2Args {
     XpolDev xp[kMaxJobs / 2];
 };
 
-template <int MAXPTS, int NSRC, bool CX>
+// F1: plain first row radix run from registers (0: none; see fuse1 below)
+template <int MAXPTS, int NSRC, bool CX, int F1 = 0>
 __global__ void __launch_bounds__(512) k_pass2(P2Args a)
 {
     extern __shared__ float2 lds_all[];
@@ -842,13 +843,13 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
     // FFT defines t, so t is never live across the FFT's registers.  Other
     // plans load each FFT's rows in batches at the top of its iteration.
     constexpr bool PREF = MAXPTS == 16;
-    float2 t[NSL][U];
+    float2 t[NSL * U];   // prefetched row points t[sl*U + u], or first-stage inputs
     auto load_rows = [&](int f) __attribute__((always_inline)) {
 #pragma unroll
         for (int sl = 0; sl < NSL; sl++) {
             const float2* __restrict__ Ys = row_src(sl, f);
 #pragma unroll
-            for (int u = 0; u < U; u++) t[sl][u] = Ys[batch_idx<U>(tid, u, nth, N2)];
+            for (int u = 0; u < U; u++) t[sl * U + u] = Ys[batch_idx<U>(tid, u, nth, N2)];
         }
     };
     auto store_rows = [&]() __attribute__((always_inline)) {
@@ -856,14 +857,68 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
         for (int sl = 0; sl < NSL; sl++)
 #pragma unroll
             for (int u = 0; u < U; u++)
-                if (tid + u * nth < N2) lds[sl * N2 + tid + u * nth] = t[sl][u];
+                if (tid + u * nth < N2) lds[sl * N2 + tid + u * nth] = t[sl * U + u];
     };
-    if (PREF && sact && fb < nfj) load_rows(fb);
+    // A plain first radix F1 (16-point plans: 3.2M rows 5*16*16) runs from
+    // registers: the prefetch loads each thread's first-stage butterfly inputs
+    // q + r*L/F1 of its own row (q = q0 + i*T, i < 16/F1, in the 16 registers
+    // of t) and the butterflies write the stage's output to LDS, saving a
+    // store, a barrier and a read of every point.  A compile-time choice: a
+    // kernel holding both paths is register-allocated for their union.
+    constexpr bool fuse1 = PREF && F1 != 0;
+    static_assert(NSL * U == 16, "first-stage registers");
+    auto with_r1 = [&](auto fn) __attribute__((always_inline)) {
+        if constexpr (fuse1) fn(std::integral_constant<int, F1>());
+    };
+    auto load_bfly = [&](int f) __attribute__((always_inline)) {
+        with_r1([&](auto rc) __attribute__((always_inline)) {
+            constexpr int R = decltype(rc)::value;
+            const int nb = N2 / R;
+            const float2* __restrict__ Ys = row_src(slot, f);
+            // opaque: the index math of every radix case would otherwise be
+            // hoisted out of the FFT loop and held live across it
+            int qb = q0;
+            asm volatile("" : "+v"(qb));
+#pragma unroll
+            for (int i = 0; i < 16 / R; i++) {
+                const int q = min(qb + i * T, nb - 1);   // clamped: values unused past nb
+#pragma unroll
+                for (int r = 0; r < R; r++) t[i * R + r] = Ys[q + r * nb];
+            }
+        });
+    };
+    auto first_stage = [&]() __attribute__((always_inline)) {
+        with_r1([&](auto rc) __attribute__((always_inline)) {
+            constexpr int R = decltype(rc)::value;
+            const int nb = N2 / R;
+            float2* __restrict__ row = lds + slot * N2;
+            int qb = q0;
+            asm volatile("" : "+v"(qb));   // as in load_bfly
+#pragma unroll
+            for (int i = 0; i < 16 / R; i++) {
+                const int q = qb + i * T;
+                if (q < nb) {
+                    float2 v[R];
+#pragma unroll
+                    for (int r = 0; r < R; r++) v[r] = t[i * R + r];
+                    float2* __restrict__ dst = row + q * R;   // first stage: k = 0, o = q R
+                    DftStore<R>::template run<false>(v, nullptr, [&](int r, float2 x) { dst[r] = x; });
+                }
+            }
+        });
+    };
+    if (PREF && sact && fb < nfj) {
+        if (fuse1) load_bfly(fb);
+        else load_rows(fb);
+    }
     for (int it = 0; it < per; it++) {
         const int f = fb + it;
         const bool act = sact && f < nfj;
         if (PREF) {
-            if (act) store_rows();
+            if (act) {
+                if (fuse1) first_stage();
+                else store_rows();
+            }
         } else if (act) {
             const float2* __restrict__ Ys[NSL];
 #pragma unroll
@@ -883,15 +938,14 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
             }
         }
         __syncthreads();
-        fft_lds<MAXPTS, CX>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * N2, 1, q0, T, a.sp.nst);
+        fft_lds<MAXPTS, CX>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * N2, 1, q0, T, a.sp.nst, fuse1 ? 1 : 0);
         if (PREF) {
             if (sact && f + 1 < nfj && it + 1 < per) {
-                load_rows(f + 1);
+                if (fuse1) load_bfly(f + 1);
+                else load_rows(f + 1);
             } else {
 #pragma unroll
-                for (int sl = 0; sl < NSL; sl++)
-#pragma unroll
-                    for (int u = 0; u < U; u++) t[sl][u] = make_float2(0.f, 0.f);
+                for (int u = 0; u < NSL * U; u++) t[u] = make_float2(0.f, 0.f);
             }
         }
 
@@ -1554,8 +1608,20 @@ hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const Xpol
     static const K table[2][2][2] = {
         {{k_pass2<16, 1, false>, k_pass2<16, 1, true>}, {k_pass2<16, 2, false>, k_pass2<16, 2, true>}},
         {{k_pass2<32, 1, false>, k_pass2<32, 1, true>}, {k_pass2<32, 2, false>, k_pass2<32, 2, true>}}};
+    // 16-point plans with a plain first radix 4, 5 or 8: [radix][nsrc][composite]
+    static const K table_f1[3][2][2] = {
+        {{k_pass2<16, 1, false, 4>, k_pass2<16, 1, true, 4>}, {k_pass2<16, 2, false, 4>, k_pass2<16, 2, true, 4>}},
+        {{k_pass2<16, 1, false, 5>, k_pass2<16, 1, true, 5>}, {k_pass2<16, 2, false, 5>, k_pass2<16, 2, true, 5>}},
+        {{k_pass2<16, 1, false, 8>, k_pass2<16, 1, true, 8>}, {k_pass2<16, 2, false, 8>, k_pass2<16, 2, true, 8>}}};
     if (p.p2_maxpts != 16 && p.p2_maxpts != 32) return hipErrorInvalidValue;
-    const K k = table[p.p2_maxpts == 32][nsrc - 1][p.p2_cx ? 1 : 0];
+    K k = table[p.p2_maxpts == 32][nsrc - 1][p.p2_cx ? 1 : 0];
+    static const bool f1_env = [] {
+        const char* e = getenv("SWSPEC_P2_FUSE1");
+        return !(e && atoi(e) == 0);
+    }();
+    const int r1 = p.sp2.radix[0];
+    if (f1_env && p.p2_maxpts == 16 && (r1 == 4 || r1 == 5 || r1 == 8))
+        k = table_f1[r1 == 4 ? 0 : (r1 == 5 ? 1 : 2)][nsrc - 1][p.p2_cx ? 1 : 0];
     hipLaunchKernelGGL(k, grid, block, lds, s, a);
     return hipGetLastError();
 }
