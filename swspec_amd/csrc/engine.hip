@@ -1175,6 +1175,53 @@ __global__ void __launch_bounds__(256) k_finalize_t(FinArgs a)
     }
 }
 
+// Many slices (short FFTs: up to 1024): 64 accumulator entries per block of
+// 1024 threads, 16 slice groups; group h sums slices h, h+16, ... in order
+// (FP64), clearing them with `zero`, and the 16 partials are added in group
+// order: a fixed order, so results are reproducible.  Grid (ceil((M+1)/64),
+// 1, jobs).
+template <bool XP>
+__global__ void __launch_bounds__(1024) k_finalize_s(FinArgs a)
+{
+    constexpr int W = XP ? 4 : 2;
+    constexpr int NG = 16;
+    __shared__ double2 part[NG][64];
+    float* __restrict__ acc = a.acc[blockIdx.z];
+    float* __restrict__ out = a.out[blockIdx.z];
+    const int lane = threadIdx.x & 63, h = threadIdx.x >> 6;
+    const long long idx = (long long)blockIdx.x * 64 + lane;
+    double re = 0.0, im = 0.0;
+    if (idx <= a.M) {
+#pragma unroll 4
+        for (int s = h; s < a.S; s += NG) {
+            float* e = acc + s * a.stride + W * idx;
+            if (XP) {
+                const float4 v = *(const float4*)e;
+                re += (double)v.x - (double)v.z;
+                im += (double)v.y - (double)v.w;
+                if (a.zero) *(float4*)e = make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                const float2 v = *(const float2*)e;
+                re += (double)v.x - (double)v.y;
+                if (a.zero) *(float2*)e = make_float2(0.f, 0.f);
+            }
+        }
+    }
+    part[h][lane] = make_double2(re, im);
+    __syncthreads();
+    if (h == 0 && idx <= a.M) {
+        for (int g = 1; g < NG; g++) { re += part[g][lane].x; im += part[g][lane].y; }
+        // natural bin of accumulator entry idx = k1*N2 + k2 is k = k2*N1 + k1
+        const long long k = idx == a.M ? a.M : (idx % a.N2) * a.N1 + idx / a.N2;
+        if (XP) {
+            out[2 * k] = (float)(re * a.scale);
+            out[2 * k + 1] = (k == 0 || k == a.M) ? 0.0f : (float)(im * a.scale);
+        } else {
+            out[k] = (float)(re * a.scale);
+        }
+    }
+}
+
 __global__ void k_zero(uint32_t* __restrict__ p, size_t nwords)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x)
@@ -1768,6 +1815,12 @@ hipError_t launch_finalize_jobs(const Plan& p, float* const* acc, float* const* 
     a.stride = (xpol ? 4 : 2) * (p.M + 1);
     a.zero = zero ? 1 : 0;
     for (int j = 0; j < njobs; j++) { a.acc[j] = acc[j]; a.out[j] = out[j]; }
+    if (nslices >= 8) {   // short FFTs: parallel over slices (a tile grid would be a few blocks)
+        const dim3 grid((unsigned)((p.M + 1 + 63) / 64), 1, njobs);
+        if (xpol) hipLaunchKernelGGL(k_finalize_s<true>, grid, dim3(1024), 0, s, a);
+        else hipLaunchKernelGGL(k_finalize_s<false>, grid, dim3(1024), 0, s, a);
+        return hipGetLastError();
+    }
     const dim3 grid((p.N2 + 31) / 32, (p.N1 + 31) / 32, njobs);
     if (xpol) hipLaunchKernelGGL(k_finalize_t<true>, grid, dim3(256), 0, s, a);
     else hipLaunchKernelGGL(k_finalize_t<false>, grid, dim3(256), 0, s, a);
