@@ -204,6 +204,21 @@ int swspec_hip_combine(swspec_hip_core* core, float* const* combined_out, float*
 /* TaskCore::resetBuffer (src/TaskCore.h:87; TaskCoreIPP.cpp:297-300): zero floats. */
 void swspec_hip_reset(void* buf, size_t bytes);
 
+/*
+ * TaskCore::combineResults (TaskCoreIPP.cpp:311-349) for device-resident
+ * results: after a swspec_hip_run_device run has been joined, adds its first
+ * spectrum of each output into the caller's DEVICE combined arrays (same
+ * layout as swspec_hip_combine) and clears the core's device outputs, on the
+ * core's stream; returns when done.  Lengths as swspec_hip_combine.  Keeps
+ * combined spectra resident across buffers (no host round trip per buffer).
+ */
+int swspec_hip_combine_device(swspec_hip_core* core, float* const* d_combined_out, float* const* d_combined_pcal,
+                              size_t* combined_out_len, size_t* combined_pcal_len);
+
+/* TaskCore::resetBuffer (TaskCoreIPP.cpp:297-300) on device memory: zero the
+ * floats of [d_buf, d_buf + bytes) with a fill kernel; returns when done. */
+int swspec_hip_reset_device(void* d_buf, size_t bytes);
+
 /* TaskCore::finalize (src/TaskCore.h:71; TaskCoreIPP.cpp:189-227): release all. */
 int swspec_hip_finalize(swspec_hip_core* core);
 

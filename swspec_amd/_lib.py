@@ -21,6 +21,7 @@ FORMATS = {"RawSigned": 0, "RawUnsigned": 1, "Mk5B": 2, "iBOB": 3, "VDIF": 4, "V
 EXPORTS = [
     "swspec_hip_derive", "swspec_hip_cfg_from_ini", "swspec_hip_create", "swspec_hip_create_multichannel",
     "swspec_hip_run", "swspec_hip_run_device", "swspec_hip_join", "swspec_hip_combine", "swspec_hip_reset",
+    "swspec_hip_combine_device", "swspec_hip_reset_device",
     "swspec_hip_finalize", "swspec_hip_last_error", "swspec_hip_pcal_tones", "swspec_hip_pcal_ntones",
     "swspec_hip_last_run_seconds", "swspec_hip_host_alloc", "swspec_hip_host_free",
     "swspec_hip_unpack_device", "swspec_hip_plan_info", "swspec_hip_set_profiling", "swspec_hip_kernel_stats",
@@ -112,6 +113,8 @@ def load():
     L.swspec_hip_combine.argtypes = [P, PP, PP, SZ, SZ]
     L.swspec_hip_reset.argtypes = [P, C.c_size_t]
     L.swspec_hip_reset.restype = None
+    L.swspec_hip_combine_device.argtypes = [P, PP, PP, SZ, SZ]
+    L.swspec_hip_reset_device.argtypes = [P, C.c_size_t]
     L.swspec_hip_finalize.argtypes = [P]
     L.swspec_hip_last_error.argtypes = [P]
     L.swspec_hip_last_error.restype = C.c_char_p

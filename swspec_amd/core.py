@@ -238,6 +238,27 @@ class TaskCoreHIP:
     def resetBuffer(self, buf):
         load().swspec_hip_reset(buf.ptr(), buf.getAllocated())
 
+    # combineResults for a run_device run: device combined arrays (tensors or
+    # pointers), num_sources auto arrays then num_xpols cross arrays; returns
+    # (out lengths, pcal lengths) in bytes as combineResults sets them
+    def combine_device(self, d_out, d_pcal=None):
+        def ptr(x):
+            return x if isinstance(x, int) else x.data_ptr()
+        outs = self._pp([ptr(x) for x in d_out])
+        pc = self._pp([ptr(x) for x in d_pcal] if d_pcal else [None] * self.njobs)
+        ol = (C.c_size_t * max(len(d_out), 1))()
+        pl = (C.c_size_t * max(len(d_pcal) if d_pcal else 1, 1))()
+        check(load().swspec_hip_combine_device(self._core, outs, pc, ol, pl), self._core,
+              "swspec_hip_combine_device")
+        return list(ol)[: len(d_out)], (list(pl)[: len(d_pcal)] if d_pcal else [])
+
+    # resetBuffer for a device array (tensor or pointer) of `nbytes`
+    @staticmethod
+    def reset_device(d_buf, nbytes=None):
+        p = d_buf if isinstance(d_buf, int) else d_buf.data_ptr()
+        n = nbytes if nbytes is not None else d_buf.numel() * d_buf.element_size()
+        check(load().swspec_hip_reset_device(p, n), None, "swspec_hip_reset_device")
+
     # TaskCore::finalize
     def finalize(self):
         if self._core is not None:

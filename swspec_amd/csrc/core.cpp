@@ -753,6 +753,11 @@ extern "C" int swspec_hip_run_device(swspec_hip_core* c, const void* const* d_in
     c->host_mode = false;
     rc = run_common(c, ins, len, autos, c->nx ? d_out_xpol[0] : nullptr, pcals, c->d_tones);
     if (rc) return rc;
+    for (int j = 0; j < c->njobs; j++) {   // what swspec_hip_combine_device adds up
+        c->o_auto[j] = autos[j];
+        c->o_pcal[j] = pcals[j];
+    }
+    c->o_xpol = c->nx ? d_out_xpol[0] : nullptr;
     c->pending = true;
     return SWSPEC_OK;
 }
@@ -878,6 +883,49 @@ extern "C" int swspec_hip_combine(swspec_hip_core* c, float* const* out, float* 
 extern "C" void swspec_hip_reset(void* buf, size_t bytes)
 {
     if (buf) std::memset(buf, 0, (bytes / sizeof(float)) * sizeof(float));
+}
+
+extern "C" int swspec_hip_combine_device(swspec_hip_core* c, float* const* d_out, float* const* d_pcal,
+                                         size_t* out_len, size_t* pcal_len)
+{
+    // TaskCoreIPP::combineResults (:311-349) on device arrays: the first
+    // spectrum of the last device-mode run is added into the caller's
+    // combined arrays and cleared, on the core's stream; returns once done
+    if (!c || !d_out) return fail(c, SWSPEC_EINVAL, "null argument");
+    if (c->pending) return fail(c, SWSPEC_ESTATE, "combine while a run is pending");
+    if (c->host_mode) return fail(c, SWSPEC_ESTATE, "combine_device needs a device-mode run");
+    const size_t ssb = (size_t)c->cfg.fft_ssb_points;
+    for (int s = 0; s < c->njobs; s++) {
+        if (!d_out[s] || !c->o_auto[s]) continue;
+        HIPCHK(c, launch_add_clear(d_out[s], c->o_auto[s], ssb, c->stream));
+        if (out_len) out_len[s] = sizeof(float) * ssb;
+    }
+    for (int x = 0; x < c->nx; x++) {
+        float* dst = d_out[c->njobs + x];
+        if (!dst || !c->o_xpol) continue;
+        HIPCHK(c, launch_add_clear(dst, c->o_xpol, 2 * ssb, c->stream));
+        if (out_len) out_len[c->njobs + x] = sizeof(float) * ssb;   // Q5: float-sized length (:333)
+    }
+    if (c->cfg.extract_pcal && d_pcal) {
+        const size_t T = (size_t)c->cfg.pcal_tonebins;
+        for (int s = 0; s < c->njobs; s++) {
+            if (!d_pcal[s] || !c->o_pcal[s]) continue;
+            HIPCHK(c, launch_add_clear(d_pcal[s], c->o_pcal[s], 2 * T, c->stream));
+            if (pcal_len) pcal_len[s] = sizeof(float) * 2 * T;
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SWSPEC_OK;
+}
+
+extern "C" int swspec_hip_reset_device(void* d_buf, size_t bytes)
+{
+    // TaskCoreIPP::resetBuffer (:297-300) on device memory, by a fill kernel
+    if (!d_buf) return fail(nullptr, SWSPEC_EINVAL, "null buffer");
+    hipError_t e = launch_zero(d_buf, (bytes / sizeof(float)) * sizeof(float), nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return hipfail(nullptr, e, "swspec_hip_reset_device");
+    return SWSPEC_OK;
 }
 
 extern "C" const char* swspec_hip_last_error(const swspec_hip_core* c)

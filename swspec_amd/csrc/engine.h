@@ -188,6 +188,8 @@ hipError_t launch_finalize(const Plan& p, const float* acc, float* out, double s
 hipError_t launch_finalize_xpol(const Plan& p, const float* acc, float* out, double scale, int nslices,
                                 hipStream_t s);
 hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s);
+// dst[i] += src[i]; src[i] = 0 for i < n
+hipError_t launch_add_clear(float* dst, float* src, size_t n, hipStream_t s);
 hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int njobs, int nfft,
                        hipStream_t s);
 // All jobs of a spectrum in one launch: acc[j] holds nslices slices of

@@ -1222,6 +1222,15 @@ __global__ void __launch_bounds__(1024) k_finalize_s(FinArgs a)
     }
 }
 
+// dst += src, src = 0 (combineResults on device arrays)
+__global__ void k_add_clear(float* __restrict__ dst, float* __restrict__ src, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        dst[i] += src[i];
+        src[i] = 0.0f;
+    }
+}
+
 __global__ void k_zero(uint32_t* __restrict__ p, size_t nwords)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x)
@@ -1689,6 +1698,14 @@ hipError_t launch_finalize_xpol(const Plan& p, const float* acc, float* out, dou
     const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_finalize_xpol, dim3(blocks), dim3(256), 0, s, const_cast<float*>(acc), out, p.M,
                        p.N1, p.N2, scale, nslices, 4 * (p.M + 1));
+    return hipGetLastError();
+}
+
+hipError_t launch_add_clear(float* dst, float* src, size_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_add_clear, dim3(blocks), dim3(256), 0, s, dst, src, n);
     return hipGetLastError();
 }
 

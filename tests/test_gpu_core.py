@@ -217,6 +217,78 @@ def test_host_mode_matches_device_mode(hip, oracle):
     c2.finalize()
 
 
+def test_combine_device_matches_host_combine(hip):
+    """combineResults on device arrays (auto, cross and phase-cal outputs of
+    two sources, two runs added up) equals the host combineResults of the
+    same runs bit for bit (TaskCoreIPP.cpp:311-349)."""
+    import torch
+    n, fs = 32000, 32e6
+    say("combine_device")
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * 2 / fs, FFToverlapFactor=1, WindowType="Hann",
+                            BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=1, DoCrossPolarization=True,
+                            ExtractPCal=True, PCalOffsetHz=10e3, MaxSourceBufferMB=4096, num_sources=2)
+    assert s.num_xpols == 1 and s.extract_pcal == 1
+    ssb, T = s.fft_ssb_points, s.pcal_tonebins
+    runs = [[synth.raw_1ch(40 + 2 * r + k, n * 2, fs=fs) for k in range(2)] for r in range(2)]
+    # host mode: run, join, combineResults twice
+    hc = hip.TaskCoreHIP(1)
+    hc.prepare(s)
+    h_comb = [hip.Buffer(4 * ssb) for _ in range(2)] + [hip.Buffer(8 * ssb)]
+    h_pc = [hip.Buffer(8 * T) for _ in range(2)]
+    for b in h_comb + h_pc:
+        hc.resetBuffer(b)
+    for raws in runs:
+        ins = []
+        for r in raws:
+            b = hip.Buffer(len(r), pinned=True)
+            b.getData()[:] = r
+            b.setLength(len(r))
+            ins.append(b)
+        outs = [hip.Buffer(4 * ssb) for _ in range(2)]
+        xb = [hip.Buffer(8 * ssb)]
+        pb = [hip.Buffer(8 * T) for _ in range(2)]
+        hc.run(ins, outs, xb, pb)
+        assert hc.join() == 1
+        hc.combineResults(h_comb, h_pc)
+    hc.finalize()
+    # device mode: run_device, join, combine_device twice
+    dev = torch.device("cuda:0")
+    dc = hip.TaskCoreHIP(0)
+    dc.prepare(s)
+    d_comb = [torch.full((ssb,), 7.0, device=dev) for _ in range(2)] + [torch.full((2 * ssb,), 7.0, device=dev)]
+    d_pc = [torch.full((2 * T,), 7.0, device=dev) for _ in range(2)]
+    for t in d_comb + d_pc:
+        hip.TaskCoreHIP.reset_device(t)
+    assert all(float(t.abs().sum()) == 0 for t in d_comb + d_pc)
+    for raws in runs:
+        d_in = [torch.from_numpy(r).to(dev) for r in raws]
+        d_auto = [torch.full((ssb,), float("nan"), device=dev) for _ in range(2)]
+        d_x = [torch.full((2 * ssb,), float("nan"), device=dev)]
+        d_p = [torch.full((2 * T,), float("nan"), device=dev) for _ in range(2)]
+        dc.run_device(d_in, [len(r) for r in raws], d_auto, d_x, d_p)
+        assert dc.join() == 1
+        ol, pl = dc.combine_device(d_comb, d_pc)
+        assert ol == [4 * ssb, 4 * ssb, 4 * ssb]   # cross length float-sized, as the reference sets it
+        assert pl == [8 * T, 8 * T]
+        # the core's outputs are cleared once added
+        assert all(float(t.abs().sum()) == 0 for t in d_auto + d_x + d_p)
+    for k in range(3):
+        h = h_comb[k].getData().view(np.float32)[: d_comb[k].numel()]
+        assert np.array_equal(d_comb[k].cpu().numpy(), h), k
+    for k in range(2):
+        assert np.array_equal(d_pc[k].cpu().numpy(), h_pc[k].getData().view(np.float32)[: 2 * T]), k
+    assert float(d_comb[0].sum()) > 0
+    # combine before any run-in-progress is joined is refused
+    d_in = [torch.from_numpy(r).to(dev) for r in runs[0]]
+    d_auto = [torch.empty(ssb, device=dev) for _ in range(2)]
+    dc.run_device(d_in, [len(r) for r in runs[0]], d_auto, [torch.empty(2 * ssb, device=dev)],
+                  [torch.empty(2 * T, device=dev) for _ in range(2)])
+    with pytest.raises(RuntimeError):
+        dc.combine_device(d_comb, d_pc)
+    assert dc.join() == 1
+    dc.finalize()
+
+
 def test_parseval_full_cfg2_length(hip):
     """Size-independent property at the full 3.2M-point length with many FFTs:
     sum_k c_k P[k] = N * mean_f sum_n (w x)^2 (c = 1 at DC/Nyquist, 2 otherwise)."""
