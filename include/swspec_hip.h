@@ -325,6 +325,57 @@ int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d_raw, int c
  * pass-1 / pass-2 LDS stages. */
 int swspec_hip_plan_info(int64_t fft_points, int32_t* info, int ninfo);
 
+/*
+ * ---- phase-cal extractor classes --------------------------------------------
+ * The reference's standalone extractor family (src/IA-32/PhaseCal/PCal.h:51-190,
+ * PCal.cpp): PCal::getNew picks PCalExtractorTrivial (offset 0),
+ * PCalExtractorImplicitShift (the offset's repeat length is a multiple of the
+ * comb's) or PCalExtractorShifting (FP32 rotator).  Samples are float32 on the
+ * device; accumulation is FP64 (the reference's is FP32); the Shifting class
+ * keeps the reference's sample map, including the samples its whole-period
+ * loop leaves out when rotatorlen is not a multiple of nbins (PCal.cpp:432-441).
+ * The reference leaves the fold indices unset until adjustSampleOffset; here
+ * create applies its sampleoffset as adjustSampleOffset does.
+ */
+typedef struct swspec_hip_pcal swspec_hip_pcal;
+#define SWSPEC_PCAL_TRIVIAL 0
+#define SWSPEC_PCAL_SHIFTING 1
+#define SWSPEC_PCAL_IMPLICIT 2
+/* PCal::getNew (PCal.cpp:91-110) when kind == -1, else the constructor of
+ * one class (:182, :309, :520).  device: HIP device ordinal. */
+int swspec_hip_pcal_create(double bandwidth_hz, double spacing_hz, int offset_hz, size_t sampleoffset, int kind,
+                           int device, swspec_hip_pcal** out);
+/* ~PCal */
+void swspec_hip_pcal_destroy(swspec_hip_pcal* p);
+/* getLength (tones), getNBins and the class picked; rotatorlen 0 unless Shifting. */
+int swspec_hip_pcal_info(const swspec_hip_pcal* p, int* kind, int* ntones, int* nbins, int64_t* rotatorlen);
+/* clear (PCal.h:86): zero the accumulation and the sample count. */
+int swspec_hip_pcal_clear(swspec_hip_pcal* p);
+/* adjustSampleOffset (PCal.h:95). */
+int swspec_hip_pcal_adjust_offset(swspec_hip_pcal* p, size_t sampleoffset);
+/* extractAndIntegrate (PCal.h:113): d_samples is device memory of len floats,
+ * read asynchronously on `stream` (a hipStream_t; NULL = the null stream);
+ * keep it unchanged until work queued after it on that stream has run.
+ * SWSPEC_ESTATE once finalized (the reference returns false). */
+int swspec_hip_pcal_extract(swspec_hip_pcal* p, const float* d_samples, size_t len, void* stream);
+/* getFinalPCal (PCal.h:141): finalizes once, then writes the tone bins the
+ * reference copies into out (complex float, host) and/or out64 (complex
+ * double, host), either may be NULL; entries past the reference's bin-range
+ * break are left as they were.  *samplecount = samples integrated. */
+int swspec_hip_pcal_final(swspec_hip_pcal* p, float* out, double* out64, uint64_t* samplecount);
+/* Tones getFinalPCal writes (<= getLength). */
+int swspec_hip_pcal_copied_tones(const swspec_hip_pcal* p);
+/* getSeconds (PCal.h:120): samples / fs. */
+double swspec_hip_pcal_seconds(const swspec_hip_pcal* p);
+/* Time-domain delay of the comb (the peak-position analysis PCal.h:15-20
+ * describes, for which the reference has no code): peak of
+ * |sum_n T_n e^{2 pi i n spacing t}| over one comb period on upsample*tones
+ * points, refined by a parabola through its neighbours; seconds in
+ * [0, 1/spacing).  Needs getFinalPCal first. */
+int swspec_hip_pcal_delay(swspec_hip_pcal* p, int upsample, double* seconds);
+/* Last error of an extractor (or of create when p == NULL). */
+const char* swspec_hip_pcal_last_error(const swspec_hip_pcal* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -88,6 +88,15 @@ def lib():
         L.so_pcal_tones64.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_double, P]
         L.so_pcal_ntones.argtypes = [C.c_double, C.c_double, C.c_double]
         L.so_pcal_reference64.argtypes = [P, C.c_long, C.c_double, C.c_double, C.c_double, C.c_uint64, P]
+        L.so_pcalx_init.argtypes = [P, C.c_int, C.c_double, C.c_double, C.c_int, C.c_size_t]
+        L.so_pcalx_free.argtypes = [P]
+        L.so_pcalx_clear.argtypes = [P]
+        L.so_pcalx_adjust.argtypes = [P, C.c_size_t]
+        L.so_pcalx_extract.argtypes = [P, P, C.c_size_t]
+        L.so_pcalx_final.restype = C.c_uint64
+        L.so_pcalx_final.argtypes = [P, P]
+        L.so_pcal_delay64.restype = C.c_double
+        L.so_pcal_delay64.argtypes = [P, C.c_int, C.c_double, C.c_int]
         L.so_mk5b_strip.restype = C.c_long
         L.so_mk5b_strip.argtypes = [P, C.c_long, P]
         L.so_vlba_modulate.argtypes = [P]
@@ -240,6 +249,61 @@ def pcal_reference64(x, bw, spacing, offset, sampleoffset=0):
     out = np.zeros(2 * nt, dtype=np.float64)
     lib().so_pcal_reference64(_p(x), len(x), bw, spacing, offset, sampleoffset, _p(out))
     return out[0::2] + 1j * out[1::2]
+
+
+class PCalX(C.Structure):
+    """so_pcalx: the PCal extractor classes of PCal.cpp (FP64 accumulation)."""
+    _fields_ = [("kind", C.c_int), ("fs_hz", C.c_double), ("spacing_hz", C.c_double), ("offset_hz", C.c_int),
+                ("nbins", C.c_int), ("ntones", C.c_int), ("rotatorlen", C.c_long), ("pcal_index", C.c_size_t),
+                ("rotator_index", C.c_size_t), ("samplecount", C.c_uint64), ("finalized", C.c_int),
+                ("buf", C.c_void_p), ("rot", C.c_void_p), ("dft", C.c_void_p)]
+
+
+PCAL_KINDS = {"Trivial": 0, "Shifting": 1, "ImplicitShift": 2}
+
+
+class PCalExtractor:
+    """PCal::getNew (kind=None) or one class by name; methods as PCal.h."""
+
+    def __init__(self, bandwidth_hz, spacing_hz, offset_hz, sampleoffset=0, kind=None):
+        self.s = PCalX()
+        k = -1 if kind is None else PCAL_KINDS[kind]
+        if lib().so_pcalx_init(C.byref(self.s), k, bandwidth_hz, spacing_hz, int(offset_hz), sampleoffset) != 0:
+            raise ValueError("bad PCal parameters")
+        self.kind = {v: k for k, v in PCAL_KINDS.items()}[self.s.kind]
+
+    def __del__(self):
+        if getattr(self, "s", None) is not None and _lib is not None:
+            _lib.so_pcalx_free(C.byref(self.s))
+            self.s = None
+
+    def clear(self):
+        lib().so_pcalx_clear(C.byref(self.s))
+
+    def adjustSampleOffset(self, so):
+        lib().so_pcalx_adjust(C.byref(self.s), so)
+
+    def extractAndIntegrate(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        return bool(lib().so_pcalx_extract(C.byref(self.s), _p(x), len(x)))
+
+    def getLength(self):
+        return self.s.ntones
+
+    def getNBins(self):
+        return self.s.nbins
+
+    def getFinalPCal(self, out=None):
+        """Returns (samplecount, tones complex128); `out` seeds the entries
+        past the selection's break (left as they were)."""
+        o = np.zeros(2 * self.s.ntones) if out is None else np.stack([out.real, out.imag], -1).reshape(-1).copy()
+        n = lib().so_pcalx_final(C.byref(self.s), _p(o))
+        return int(n), o[0::2] + 1j * o[1::2]
+
+
+def pcal_delay64(tones, spacing_hz, upsample=64):
+    t = np.ascontiguousarray(np.stack([np.real(tones), np.imag(tones)], -1).reshape(-1), dtype=np.float64)
+    return lib().so_pcal_delay64(_p(t), len(tones), spacing_hz, upsample)
 
 
 def mk5b_strip(frames):

@@ -14,6 +14,9 @@
  *                      identity, FP32-vs-FP64 agreement, XY* of equal inputs
  *   so_extract_pcal*   brute-force per-sample rotator/fold (PCal.cpp:139-175)
  *   so_pcal_tones64    PCal.cpp unit-test KAT (phase -90 + 5*t deg within 0.1 deg)
+ *   so_pcalx_*         the same unit test per extractor class, the brute-force
+ *                      rotating fold, the DFT closed form, impulse responses
+ *   so_pcal_delay64    delayed synthetic combs (no reference code exists)
  *   so_mk5b_strip      synthetic frames with known payload
  */
 #include "swspec_oracle.h"
@@ -614,6 +617,198 @@ void so_pcal_reference64(const double* x, long len, double bandwidth_hz, double 
         tones_out[2 * t + 1] = ai / nbins;
     }
     free(acc);
+}
+
+/* ------------------------------------------------------------------------- */
+/* PCal extractor classes (src/IA-32/PhaseCal/PCal.cpp)                      */
+/* ------------------------------------------------------------------------- */
+
+/* PCal::gcd (:115-127) takes longs: double arguments truncate */
+static long long pc_gcd(double a, double b) { return so_gcd((long long)(long)a, (long long)(long)b); }
+
+int so_pcalx_init(so_pcalx* p, int kind, double bw, double spacing, int offset, size_t sampleoffset)
+{
+    memset(p, 0, sizeof(*p));
+    double fs = 2 * bw;
+    if (kind < 0) {   /* PCal::getNew (:91-110) */
+        if (offset == 0) {
+            kind = SO_PCAL_TRIVIAL;
+        } else {
+            int No = (int)(fs / (double)pc_gcd(offset, fs));
+            int Np = (int)(fs / (double)pc_gcd(spacing, fs));
+            kind = (No % Np == 0) ? SO_PCAL_IMPLICIT : SO_PCAL_SHIFTING;
+        }
+    }
+    p->kind = kind;
+    p->fs_hz = fs;
+    if (kind == SO_PCAL_TRIVIAL) {   /* :182-190, spacing arrives as int */
+        int sp = (int)spacing;
+        p->spacing_hz = sp;
+        p->offset_hz = 0;
+        p->nbins = (int)(fs / (double)pc_gcd(fabs((double)sp), fs));
+        p->ntones = (int)floor(bw / sp);
+    } else if (kind == SO_PCAL_SHIFTING) {   /* :311-330 */
+        p->spacing_hz = spacing;
+        p->offset_hz = offset;
+        p->nbins = (int)(fs / (double)pc_gcd(fabs(spacing), fs));
+        p->ntones = (int)floor((bw - offset) / spacing) + 1;
+        p->rotatorlen = (long)(fs / (double)pc_gcd(fabs((double)offset), fs));
+    } else {   /* :522-528 */
+        p->spacing_hz = spacing;
+        p->offset_hz = offset;
+        p->nbins = (int)(fs / (double)pc_gcd(fabs((double)offset), fs));
+        p->ntones = (int)floor((bw - offset) / spacing) + 1;
+    }
+    if (p->nbins <= 0 || p->ntones <= 0) return -1;
+    p->buf = (double*)calloc((size_t)p->nbins * 4, sizeof(double));
+    p->dft = (double*)calloc((size_t)p->nbins * 2, sizeof(double));
+    if (kind == SO_PCAL_SHIFTING) {   /* :353-359 */
+        double dphi = 2 * M_PI * (-(double)offset / fs);
+        p->rot = (float*)malloc(sizeof(float) * 4 * (size_t)p->rotatorlen);
+        for (long n = 0; n < 2 * p->rotatorlen; n++) {
+            double arg = dphi * (double)n;
+            p->rot[2 * n] = (float)cos(arg);
+            p->rot[2 * n + 1] = (float)sin(arg);
+        }
+    }
+    so_pcalx_clear(p);
+    so_pcalx_adjust(p, sampleoffset);
+    return 0;
+}
+
+void so_pcalx_free(so_pcalx* p)
+{
+    free(p->buf);
+    free(p->dft);
+    free(p->rot);
+    memset(p, 0, sizeof(*p));
+}
+
+void so_pcalx_clear(so_pcalx* p)
+{
+    p->samplecount = 0;
+    p->finalized = 0;
+    memset(p->buf, 0, sizeof(double) * 4 * (size_t)p->nbins);
+}
+
+void so_pcalx_adjust(so_pcalx* p, size_t sampleoffset)
+{
+    p->pcal_index = sampleoffset % (size_t)p->nbins;
+    p->rotator_index = (p->kind == SO_PCAL_SHIFTING) ? sampleoffset % (size_t)p->rotatorlen : 0;
+}
+
+/* dst[i] += x[i] (real) for i < n, into the complex buffer at index d */
+static void add_real(double* buf, size_t d, const float* x, size_t n)
+{
+    for (size_t i = 0; i < n; i++) buf[2 * (d + i)] += (double)x[i];
+}
+
+/* dst[i] += x[i] * rot[i] for i < n */
+static void add_rotated(double* buf, size_t d, const float* x, const float* rot, size_t n)
+{
+    for (size_t i = 0; i < n; i++) {
+        buf[2 * (d + i)] += (double)x[i] * (double)rot[2 * i];
+        buf[2 * (d + i) + 1] += (double)x[i] * (double)rot[2 * i + 1];
+    }
+}
+
+int so_pcalx_extract(so_pcalx* p, const float* x, size_t len)
+{
+    if (p->finalized) return 0;
+    const size_t N = (size_t)p->nbins;
+    if (p->kind != SO_PCAL_SHIFTING) {   /* :270-291 and :609-637 */
+        size_t tail = len % N, end = len - tail;
+        const float* src = x;
+        for (size_t n = 0; n < end; n += N, src += N) add_real(p->buf, p->pcal_index, src, N);
+        if (tail) {
+            add_real(p->buf, p->pcal_index, src, tail);
+            p->pcal_index = (p->pcal_index + tail) % N;
+        }
+    } else {   /* :419-477: whole rotator periods add only whole pulses */
+        const size_t L = (size_t)p->rotatorlen;
+        size_t tail = len % L, end = len - tail;
+        const float* src = x;
+        const float* rot = p->rot + 2 * p->rotator_index;
+        for (size_t n = 0; n < end; n += L, src += L)
+            for (size_t q = 0; q < L / N; q++) add_rotated(p->buf, p->pcal_index, src + q * N, rot + 2 * q * N, N);
+        if (tail) {
+            size_t tail2 = tail % N, end2 = tail - tail2;
+            for (size_t q = 0; q < end2 / N; q++) add_rotated(p->buf, p->pcal_index, src + q * N, rot + 2 * q * N, N);
+            add_rotated(p->buf, p->pcal_index, src + end2, rot + 2 * end2, tail2);
+            p->rotator_index = (p->rotator_index + tail) % L;
+            p->pcal_index = (p->pcal_index + tail2) % N;
+        }
+    }
+    p->samplecount += len;
+    return 1;
+}
+
+uint64_t so_pcalx_final(so_pcalx* p, double* tones_out)
+{
+    const int N = p->nbins;
+    if (!p->finalized) {
+        p->finalized = 1;
+        for (int b = 0; b < N; b++) {   /* fold the upper half, DFT with 1/N */
+            p->buf[2 * b] += p->buf[2 * (N + b)];
+            p->buf[2 * b + 1] += p->buf[2 * (N + b) + 1];
+        }
+    }
+    size_t base = 0, step = 1;
+    if (p->kind == SO_PCAL_SHIFTING) {   /* :503-512 */
+        step = (size_t)floor(N * (p->spacing_hz / p->fs_hz));
+    } else if (p->kind == SO_PCAL_IMPLICIT) {   /* :660-669: N*offset is an int product */
+        step = (size_t)floor((double)N * (p->spacing_hz / p->fs_hz));
+        int32_t prod = (int32_t)((uint32_t)N * (uint32_t)p->offset_hz);
+        base = (size_t)(long long)floor((double)prod / p->fs_hz);
+    }
+    if (p->finalized == 1) {   /* the DFT (1/N) at the bins copied below */
+        p->finalized = 2;
+        for (int n = 0; n < p->ntones; n++) {
+            size_t k = base + (size_t)n * step;
+            if (k >= (size_t)N) break;
+            double ar = 0, ai = 0;
+            for (int b = 0; b < N; b++) {
+                long long e = ((long long)k * b) % N;
+                double a = -2.0 * M_PI * (double)e / (double)N;
+                ar += p->buf[2 * b] * cos(a) - p->buf[2 * b + 1] * sin(a);
+                ai += p->buf[2 * b] * sin(a) + p->buf[2 * b + 1] * cos(a);
+            }
+            p->dft[2 * k] = ar / N;
+            p->dft[2 * k + 1] = ai / N;
+        }
+    }
+    for (int n = 0; n < p->ntones; n++) {
+        size_t idx = base + (size_t)n * step;
+        if (idx >= (size_t)N) break;
+        tones_out[2 * n] = p->dft[2 * idx];
+        tones_out[2 * n + 1] = p->dft[2 * idx + 1];
+    }
+    return p->samplecount;
+}
+
+double so_pcal_delay64(const double* tones, int ntones, double spacing_hz, int upsample)
+{
+    const int G = upsample * ntones;
+    double* mag = (double*)malloc(sizeof(double) * (size_t)G);
+    int best = 0;
+    for (int g = 0; g < G; g++) {
+        double hr = 0, hi = 0;
+        for (int n = 0; n < ntones; n++) {
+            long long e = ((long long)n * g) % G;
+            double a = 2.0 * M_PI * (double)e / (double)G;
+            hr += tones[2 * n] * cos(a) - tones[2 * n + 1] * sin(a);
+            hi += tones[2 * n] * sin(a) + tones[2 * n + 1] * cos(a);
+        }
+        mag[g] = sqrt(hr * hr + hi * hi);
+        if (mag[g] > mag[best]) best = g;
+    }
+    double ym = mag[(best + G - 1) % G], y0 = mag[best], yp = mag[(best + 1) % G];
+    double den = ym - 2 * y0 + yp;
+    double d = (den != 0) ? 0.5 * (ym - yp) / den : 0.0;
+    free(mag);
+    double t = ((double)best + d) / (double)G;
+    t -= floor(t);
+    return t / spacing_hz;
 }
 
 /* ------------------------------------------------------------------------- */

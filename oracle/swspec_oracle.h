@@ -190,6 +190,52 @@ int so_pcal_ntones(double bandwidth_hz, double spacing_hz, double offset_hz);
 void so_pcal_reference64(const double* x, long len, double bandwidth_hz, double spacing_hz,
                          double offset_hz, uint64_t sampleoffset, double* tones_out);
 
+/* ---- The PCal extractor classes (src/IA-32/PhaseCal/PCal.cpp) ----
+ * State machine of PCalExtractorTrivial (:182-302), PCalExtractorShifting
+ * (:309-515) and PCalExtractorImplicitShift (:520-678), chosen by
+ * PCal::getNew (:91-110) when kind < 0.  The loops follow the reference's
+ * block structure (whole blocks at a fixed destination index into a buffer
+ * of 2*nbins, the tail advancing the index, the upper half folded in at
+ * getFinalPCal); the accumulation is FP64 (the reference's is FP32) and
+ * the rotator is the reference's FP32-rounded table.  The reference leaves
+ * pcal_index/rotator_index uninitialised until adjustSampleOffset; here the
+ * constructor applies its sampleoffset argument as adjustSampleOffset would.
+ * Pinned: the PCal.cpp unit test (:861-951) for each kind it can select, the
+ * brute-force rotating fold (so_pcal_reference64) up to the documented
+ * nbins scale, and impulse inputs for the Shifting drop rule. */
+enum { SO_PCAL_TRIVIAL = 0, SO_PCAL_SHIFTING = 1, SO_PCAL_IMPLICIT = 2 };
+typedef struct {
+    int kind;
+    double fs_hz, spacing_hz;
+    int offset_hz;
+    int nbins, ntones;
+    long rotatorlen;
+    size_t pcal_index, rotator_index;
+    uint64_t samplecount;
+    int finalized;
+    double* buf;    /* 2*nbins complex accumulation buffer */
+    float* rot;     /* 2*rotatorlen complex, FP32 (Shifting) */
+    double* dft;    /* nbins complex, valid once finalized */
+} so_pcalx;
+/* kind: SO_PCAL_* or -1 for the factory.  Returns 0 or -1. */
+int so_pcalx_init(so_pcalx* p, int kind, double bandwidth_hz, double spacing_hz, int offset_hz, size_t sampleoffset);
+void so_pcalx_free(so_pcalx* p);
+void so_pcalx_clear(so_pcalx* p);
+void so_pcalx_adjust(so_pcalx* p, size_t sampleoffset);
+/* 1 on success, 0 once finalized (extractAndIntegrate's bool). */
+int so_pcalx_extract(so_pcalx* p, const float* x, size_t len);
+/* getFinalPCal: writes the tone bins it selects into tones_out (ntones
+ * complex; entries past the selection's break stay as they were) and
+ * returns the sample count. */
+uint64_t so_pcalx_final(so_pcalx* p, double* tones_out);
+/* Time-domain delay from tone vectors (the peak-position analysis PCal.h:15-20
+ * describes; the reference has no code for it): |h(t)|, h(t) = sum_n T_n
+ * e^{+2 pi i n spacing t}, on G = upsample*ntones points over one comb period,
+ * argmax (lowest index on ties), parabolic refinement of |h| through the
+ * neighbours.  Returns t in [0, 1/spacing).  Parity unpinned against the
+ * reference (no code there); pinned by delayed synthetic combs. */
+double so_pcal_delay64(const double* tones, int ntones, double spacing_hz, int upsample);
+
 /* ---- VLBA through mark5access (VLBAUnpacker, DataUnpackers.cpp:459-535) ----
  * Restated from the vendored mark5access 1.5.0 sources for the 8-track,
  * fanout-1, 2-bit case ("VLBA1_1-<Mbps>-4-2"; 4 channels, one byte per time

@@ -28,6 +28,10 @@ EXPORTS = [
     "swspec_hip_server_create", "swspec_hip_server_submit", "swspec_hip_server_step", "swspec_hip_server_poll",
     "swspec_hip_server_sync", "swspec_hip_server_run_batch", "swspec_hip_server_destroy",
     "swspec_hip_server_last_error",
+    "swspec_hip_pcal_create", "swspec_hip_pcal_destroy", "swspec_hip_pcal_info", "swspec_hip_pcal_clear",
+    "swspec_hip_pcal_adjust_offset", "swspec_hip_pcal_extract", "swspec_hip_pcal_final",
+    "swspec_hip_pcal_copied_tones", "swspec_hip_pcal_seconds", "swspec_hip_pcal_delay",
+    "swspec_hip_pcal_last_error",
 ]
 
 
@@ -139,6 +143,22 @@ def load():
     L.swspec_hip_server_destroy.argtypes = [P]
     L.swspec_hip_server_last_error.argtypes = [P]
     L.swspec_hip_server_last_error.restype = C.c_char_p
+    L.swspec_hip_pcal_create.argtypes = [C.c_double, C.c_double, C.c_int, C.c_size_t, C.c_int, C.c_int,
+                                         C.POINTER(P)]
+    L.swspec_hip_pcal_destroy.argtypes = [P]
+    L.swspec_hip_pcal_destroy.restype = None
+    L.swspec_hip_pcal_info.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int64)]
+    L.swspec_hip_pcal_clear.argtypes = [P]
+    L.swspec_hip_pcal_adjust_offset.argtypes = [P, C.c_size_t]
+    L.swspec_hip_pcal_extract.argtypes = [P, P, C.c_size_t, P]
+    L.swspec_hip_pcal_final.argtypes = [P, P, P, C.POINTER(C.c_uint64)]
+    L.swspec_hip_pcal_copied_tones.argtypes = [P]
+    L.swspec_hip_pcal_seconds.argtypes = [P]
+    L.swspec_hip_pcal_seconds.restype = C.c_double
+    L.swspec_hip_pcal_delay.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
+    L.swspec_hip_pcal_last_error.argtypes = [P]
+    L.swspec_hip_pcal_last_error.restype = C.c_char_p
     _lib = L
     return L
 
