@@ -36,6 +36,7 @@ NCH = 4
 NFFT = 100
 SAMPLES = N * NFFT            # per channel (10 s at 32 Ms/s)
 PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBPS = 8000.0
 
 
 def log(*a):
@@ -246,6 +247,43 @@ def serve_cfg5(torch, seconds, rate=200.0, seed=5, plain_seconds=None):
             "plain_arrival_window_s": plain_seconds if plain_seconds else seconds}
 
 
+def pcal_extractor_leg(torch, nsamples=1 << 28, reps=5, cpu=True):
+    """The standalone phase-cal extractor (PCal::getNew -> ImplicitShift for a
+    10 kHz-offset 1 MHz comb at 32 Ms/s) over float32 samples resident in HBM:
+    one HBM read of 4 bytes per sample, so the roofline is HBM bandwidth."""
+    from swspec_amd.pcal import PCal
+    x = torch.randn(nsamples, device="cuda")
+    g = PCal.getNew(16e6, 1e6, 10e3)
+    g.extractAndIntegrate(x)   # warm-up over the whole range (first touch of its pages)
+    g.clear()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        g.extractAndIntegrate(x)
+    e1.record()
+    torch.cuda.synchronize()
+    dt = e0.elapsed_time(e1) / 1e3 / reps
+    g.getFinalPCal64()
+    gbps = nsamples * 4 / dt / 1e9
+    out = {"class": g.kind, "nbins": g.getNBins(), "samples_per_call": nsamples, "calls": reps,
+           "gsamples_per_s": round(nsamples / dt / 1e9, 1), "us_per_call": round(dt * 1e6, 1),
+           "roofline": {"bound": "hbm", "achieved": round(gbps, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                        "frac": round(gbps / PEAK_HBM_GBPS, 4), "bytes_per_sample": 4,
+                        "timed": "whole extractAndIntegrate call (fold + partial reduce + bin map)"}}
+    if cpu:
+        from oracle import oracle as O
+        n = 1 << 26
+        xc = x[:n].cpu().numpy()
+        o = O.PCalExtractor(16e6, 1e6, 10e3)
+        t0 = time.perf_counter()
+        o.extractAndIntegrate(xc)
+        dtc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n / dtc / 1e9, 3), "unit": "Gsamples/s", "cores": 1, "kind": "port",
+                               "sample": "%d samples, one extractAndIntegrate call" % n}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +299,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (3x): the reference point that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
+    ap.add_argument("--no-pcalx", action="store_true", help="skip the standalone phase-cal extractor leg")
+    ap.add_argument("--only-pcalx", action="store_true", help="run only the phase-cal extractor leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the workload is one GPU's share; "
@@ -276,6 +316,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": n * 4, "launches": 3}), flush=True)
+        return
+
+    if args.only_pcalx:
+        print(json.dumps({"pcal_extractor": pcal_extractor_leg(torch, cpu=not args.no_cpu)}), flush=True)
         return
 
     import swspec_amd
@@ -357,6 +401,11 @@ def main():
         log("serving replay ...")
         serve = serve_cfg5(torch, args.serve_seconds, plain_seconds=min(args.serve_seconds, 3.0))
 
+    pcalx = None
+    if not args.no_pcalx:
+        log("phase-cal extractor ...")
+        pcalx = pcal_extractor_leg(torch, cpu=not args.no_cpu)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -381,6 +430,7 @@ def main():
         "batched_vs_plain": None if plain_value is None else round(value / plain_value, 2),
         "batch_64_streams": batch,
         "serving": serve,
+        "pcal_extractor": pcalx,
     }
     if cpu:
         out["gpu_vs_cpu"] = round(value / cpu["value"], 1)

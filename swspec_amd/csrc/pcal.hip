@@ -15,10 +15,10 @@
 //     trailing partial period is added whole (:444-460).  Since the rotator
 //     value depends on n mod L only, the samples are first summed per
 //     position n mod L and each position sum is rotated once.
-// The fold (k_pfold) is a streaming read of aligned float4s: a one-wave
-// workgroup owns 256 consecutive slots of the period and a group of
-// periods, each thread sums its 4 slots over the group in FP64 and writes
-// one partial per slot; k_preduce sums the partials per slot in a fixed
+// The fold (k_pfold) is a streaming read of aligned float4s: a workgroup
+// owns 256 consecutive slots of the period and a group of periods, each
+// lane sums its 4 slots over its share of the group in FP64, and the
+// workgroup writes one partial per slot; k_preduce sums the partials per slot in a fixed
 // order, k_pfinish maps slots to bins (rotating for Shifting) into the FP64
 // accumulator.  getFinalPCal's DFT is
 // evaluated only at the tone bins it copies (k_ptones, FP64, 1/nbins).
@@ -37,67 +37,89 @@ namespace {
 
 thread_local std::string g_pcal_error;
 
-constexpr int PF_T = 64;                // threads per fold workgroup (one wave)
-constexpr int PF_TARGET_WG = 8192;      // fold workgroups aimed for (32 waves per CU)
+constexpr int PF_T = 256;               // threads per fold workgroup: 4 waves on the same 256 slots
+constexpr int PF_TARGET_WG = 2048;      // fold workgroups aimed for (8 per CU)
+constexpr int PF_MIN_MP = 4096;         // slots per period at least (short periods are repeated)
 constexpr int PF_PMIN = 32;             // periods per group at least (bounds the partial traffic)
 constexpr size_t PF_PART_MAX = size_t(1) << 25;   // partial doubles kept at most (256 MB)
 
 // The fold runs over the aligned index m = n + a, where a = (x mod 16 bytes)
-// / 4, so that a thread's 4 consecutive slots u0..u0+3 (u = m mod MP, MP a
-// multiple of 4) come from one aligned float4 in every period.
-// part[y * MP + u] = sum of the samples 0 <= n < len with (n + a) mod MP == u
-// in the periods of group y, leaving out n < end whose slot is dropped
-// (Shifting: (u - a) mod L >= jmax; jmax == L keeps everything).
+// / 4, so that a lane's 4 consecutive slots u0..u0+3 (u = m mod MP, MP a
+// multiple of 4) come from one aligned float4 in every period.  The 4 waves
+// of a workgroup take every 4th period of the group and are summed through
+// LDS.  part[y * MP + u] = sum of the samples 0 <= n < len with
+// (n + a) mod MP == u in the periods of group y, leaving out n < end whose
+// slot is dropped (Shifting: (u - a) mod L >= jmax; jmax == L keeps all).
 __global__ __launch_bounds__(PF_T) void k_pfold(const float* __restrict__ x, int a, long long len, int MP, int L,
                                                 int jmax, long long end, int PG, long long P,
                                                 double* __restrict__ part)
 {
-    const int u0 = (blockIdx.x * PF_T + threadIdx.x) * 4;
-    if (u0 >= MP) return;
-    bool drop[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) drop[k] = jmax < L && ((u0 + k - a) % L + L) % L >= jmax;
-    const bool anydrop = drop[0] || drop[1] || drop[2] || drop[3];
+    __shared__ double sh[4][4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int u0 = (blockIdx.x * 64 + lane) * 4;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const long long p0 = (long long)blockIdx.y * PG;
-    const long long p1 = min(P, p0 + PG);
+    if (u0 < MP) {
+        bool drop[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) drop[k] = jmax < L && ((u0 + k - a) % L + L) % L >= jmax;
+        const bool anydrop = drop[0] || drop[1] || drop[2] || drop[3];
+        const long long p0 = (long long)blockIdx.y * PG;
+        const long long p1 = min(P, p0 + PG);
 #pragma unroll 4
-    for (long long p = p0; p < p1; p++) {
-        const long long n0 = p * MP + u0 - a;   // x + n0 is 16-byte aligned
-        float v[4];
-        if (n0 >= 0 && n0 + 3 < len) {
-            const float4 q = *reinterpret_cast<const float4*>(x + n0);
-            v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
-        } else {
+        for (long long p = p0 + w; p < p1; p += 4) {
+            const long long n0 = p * MP + u0 - a;   // x + n0 is 16-byte aligned
+            float v[4];
+            if (n0 >= 0 && n0 + 3 < len) {
+                const float4 q = *reinterpret_cast<const float4*>(x + n0);
+                v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+            } else {
 #pragma unroll
-            for (int k = 0; k < 4; k++) v[k] = (n0 + k >= 0 && n0 + k < len) ? x[n0 + k] : 0.f;
+                for (int k = 0; k < 4; k++) v[k] = (n0 + k >= 0 && n0 + k < len) ? x[n0 + k] : 0.f;
+            }
+            if (anydrop) {
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (drop[k] && n0 + k < end) v[k] = 0.f;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) acc[k] += (double)v[k];
         }
-        if (anydrop) {
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (drop[k] && n0 + k < end) v[k] = 0.f;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) acc[k] += (double)v[k];
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++) part[(size_t)blockIdx.y * MP + u0 + k] = acc[k];
+    for (int k = 0; k < 4; k++) sh[w][k][lane] = acc[k];
+    __syncthreads();
+    if (w == 0 && u0 < MP) {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            part[(size_t)blockIdx.y * MP + u0 + k] = ((sh[0][k][lane] + sh[1][k][lane]) + sh[2][k][lane]) + sh[3][k][lane];
+    }
 }
 
-// pos[c] = sum_y part[y * MP + c]: four y-strided partial sums per column,
-// combined in a fixed order.
-__global__ __launch_bounds__(256) void k_preduce(const double* __restrict__ part, int Y, int MP,
-                                                 double* __restrict__ pos)
+// pos[c] = sum_y part[y * MP + c]: 16 y-strided partial sums per column
+// (16 waves of a workgroup share 64 columns), combined in a fixed order.
+__global__ __launch_bounds__(1024) void k_preduce(const double* __restrict__ part, int Y, int MP,
+                                                  double* __restrict__ pos)
 {
-    __shared__ double sh[4][64];
+    __shared__ double sh[16][64];
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int col = blockIdx.x * 64 + c;
     double s = 0.0;
-    if (col < MP)
-        for (int y = g; y < Y; y += 4) s += part[(size_t)y * MP + col];
+    if (col < MP) {
+        int y = g;
+        for (; y + 48 < Y; y += 64) {   // four independent loads in flight
+            const double v0 = part[(size_t)y * MP + col], v1 = part[(size_t)(y + 16) * MP + col];
+            const double v2 = part[(size_t)(y + 32) * MP + col], v3 = part[(size_t)(y + 48) * MP + col];
+            s += ((v0 + v1) + v2) + v3;
+        }
+        for (; y < Y; y += 16) s += part[(size_t)y * MP + col];
+    }
     sh[g][c] = s;
     __syncthreads();
-    if (g == 0 && col < MP) pos[col] = ((sh[0][c] + sh[1][c]) + sh[2][c]) + sh[3][c];
+    if (g == 0 && col < MP) {
+        double t = 0.0;
+        for (int k = 0; k < 16; k++) t += sh[k][c];
+        pos[col] = t;
+    }
 }
 
 // Slots -> bins into acc (nbins complex).  An index i (a bin for the real
@@ -131,6 +153,47 @@ __global__ __launch_bounds__(256) void k_pfinish(const double* __restrict__ pos,
     }
     acc[2 * b] += re;
     acc[2 * b + 1] += im;
+}
+
+// The same map with one wave per bin, for bins that own many slots (short
+// periods repeated R times, or Shifting's L/nbins positions per bin): lanes
+// stride over the (position, repeat) pairs, then a fixed-order LDS tree.
+__global__ __launch_bounds__(64) void k_pfinish_w(const double* __restrict__ pos, int MP, int M, int N, int shifting,
+                                                  int sh, int pidx, int ridx, const float2* __restrict__ rot,
+                                                  double* __restrict__ acc)
+{
+    __shared__ double s0[64], s1[64];
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const int R = MP / M;
+    const int i0 = shifting ? ((b - pidx) % N + N) % N : b;
+    const int J = shifting ? (M - i0 + N - 1) / N : 1;   // positions j = i0 + k*N, k < J
+    double re = 0.0, im = 0.0;
+    for (long long t = lane; t < (long long)J * R; t += 64) {
+        const int k = (int)(t / R), r = (int)(t % R);
+        const int j = i0 + k * N;
+        const double v = pos[(size_t)r * M + (j + sh) % M];
+        if (shifting) {
+            const float2 w = rot[ridx + j];
+            re += v * (double)w.x;
+            im += v * (double)w.y;
+        } else {
+            re += v;
+        }
+    }
+    s0[lane] = re;
+    s1[lane] = im;
+    __syncthreads();
+    for (int h = 32; h > 0; h >>= 1) {
+        if (lane < h) {
+            s0[lane] += s0[lane + h];
+            s1[lane] += s1[lane + h];
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        acc[2 * b] += s0[0];
+        acc[2 * b + 1] += s1[0];
+    }
 }
 
 __device__ inline void block_sum2(double& a, double& b, double (*sh)[256])
@@ -448,13 +511,25 @@ extern "C" int swspec_hip_pcal_extract(swspec_hip_pcal* p, const float* d_sample
     const int M = shifting ? (int)p->L : N;
     const long long end = shifting ? (long long)(len - len % (size_t)M) : 0;
     const int jmax = shifting ? (M / N) * N : M;
-    int R = 1;   // slots per period MP = R * M: a multiple of 4, at least 256
-    while ((((long long)M * R) % 4) != 0 || (long long)M * R < 256) R++;
+    // slots per period MP = R * M: a multiple of 4, at least PF_MIN_MP, and
+    // filling the last 256-slot tile (idle lanes cost bandwidth) as far as a
+    // few more repeats allow
+    const int R0 = std::max(1, (PF_MIN_MP + M - 1) / M);
+    int R = 0;
+    double best = 2.0;
+    for (int r = R0; r <= 8 * R0 && (long long)M * r <= (1LL << 30); r++) {
+        const long long mp = (long long)M * r;
+        if (mp % 4) continue;
+        const double waste = (double)(((mp / 4 + 63) / 64) * 256 - mp) / (double)mp;
+        if (waste < best - 1e-12) { best = waste; R = r; }
+        if (waste <= 1.0 / 64) break;
+    }
+    if (R == 0) return pfail(p, SWSPEC_EUNSUPPORTED, "no slot layout for this period");
     const int MP = M * R;
     const int a = (int)(((uintptr_t)d_samples / sizeof(float)) % 4);
-    const int tiles = (MP / 4 + PF_T - 1) / PF_T;
+    const int tiles = (MP / 4 + 63) / 64;
     const long long P = ((long long)len + a + MP - 1) / MP;
-    long long Y = std::max<long long>(1, (PF_TARGET_WG + tiles - 1) / tiles);
+    long long Y = std::max<long long>(1, PF_TARGET_WG / tiles);   // one round of resident workgroups
     Y = std::min<long long>(Y, std::max<long long>(1, (P + PF_PMIN - 1) / PF_PMIN));
     Y = std::min<long long>(Y, std::max<long long>(1, (long long)(PF_PART_MAX / (size_t)MP)));
     Y = std::min<long long>(Y, 65535);
@@ -467,12 +542,17 @@ extern "C" int swspec_hip_pcal_extract(swspec_hip_pcal* p, const float* d_sample
     hipLaunchKernelGGL(k_pfold, dim3(tiles, (unsigned)Y), dim3(PF_T), 0, st, d_samples, a, (long long)len, MP, M,
                        jmax, end, (int)PG, P, p->d_part);
     PCHK(p, hipGetLastError());
-    hipLaunchKernelGGL(k_preduce, dim3((MP + 63) / 64), dim3(256), 0, st, p->d_part, (int)Y, MP, p->d_pos);
+    hipLaunchKernelGGL(k_preduce, dim3((MP + 63) / 64), dim3(1024), 0, st, p->d_part, (int)Y, MP, p->d_pos);
     PCHK(p, hipGetLastError());
     // slot of bin b (real kinds): (b + a - pidx) mod M; of rotator position j (Shifting): (j + a) mod L
     const int sh = shifting ? a % M : (int)(((long long)a - (long long)p->pidx) % M + M) % M;
-    hipLaunchKernelGGL(k_pfinish, dim3((N + 255) / 256), dim3(256), 0, st, p->d_pos, MP, M, N, shifting ? 1 : 0,
-                       sh, (int)p->pidx, (int)p->ridx, p->d_rot, p->d_acc);
+    const long long per_bin = (long long)R * (shifting ? (M + N - 1) / N : 1);
+    if (per_bin >= 16)
+        hipLaunchKernelGGL(k_pfinish_w, dim3(N), dim3(64), 0, st, p->d_pos, MP, M, N, shifting ? 1 : 0, sh,
+                           (int)p->pidx, (int)p->ridx, p->d_rot, p->d_acc);
+    else
+        hipLaunchKernelGGL(k_pfinish, dim3((N + 255) / 256), dim3(256), 0, st, p->d_pos, MP, M, N,
+                           shifting ? 1 : 0, sh, (int)p->pidx, (int)p->ridx, p->d_rot, p->d_acc);
     PCHK(p, hipGetLastError());
     p->last = st;
     // index updates of extractAndIntegrate (:286-289, :457-458, :632-635)

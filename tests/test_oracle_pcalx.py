@@ -149,7 +149,7 @@ def test_shifting_drop_rule_impulses(pos, kept):
 
 
 def test_implicit_tone_base_int_wrap():
-    """nbins * offset is an int product (PCal.cpp:661): 6400 * 510000 wraps,
+    """nbins * offset is an int product (PCal.cpp:669): 6400 * 510000 wraps,
     the base bin is out of range and no tone is copied."""
     ex = O.PCalExtractor(32e6, 1e6, 510e3)
     assert ex.kind == "ImplicitShift" and ex.getNBins() == 6400
