@@ -45,6 +45,14 @@ enum swspec_format { SWSPEC_FMT_UNKNOWN = -1, SWSPEC_FMT_RAWSIGNED = 0, SWSPEC_F
  * every length passed to run()/run_device() is in frame bytes, and a
  * trailing partial frame is ignored.  Sync words are not checked. */
 #define SWSPEC_FLAG_MK5B_FRAMED 0x2u
+/* Open the column FFT with a radix-64 stage on the matrix cores: 64-point
+ * DFTs as a split-FP16 product (x = hi + lo, W = hi + lo, the lo*lo term
+ * dropped) accumulated in FP32 by v_mfma_f32_16x16x32_f16.  No counterpart
+ * in the reference (IPP's DFT runs in FP32).  Needs an FFT length whose half
+ * has a divisor that is a multiple of 64 with a cofactor <= 16384 (e.g.
+ * 2^20 and 3.2M, not 32000); prepare fails otherwise.  Spectra stay within
+ * the FP32 path's 1e-5 relative error of the FP64 result. */
+#define SWSPEC_FLAG_MFMA64 0x4u
 
 /*
  * Flat copy of the derived settings a core reads (src/Settings.h:59-159,
@@ -319,11 +327,12 @@ void swspec_hip_host_free(void* p);
 int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d_raw, int channel, int64_t count,
                              float* d_out);
 
-/* Plan geometry for an FFT length (no device work).  info[0..11] = N1, N2,
- * pass-1 columns per workgroup, pass-1 threads, pass-2 threads, row pairs,
- * pass-1 / pass-2 points per thread, pass-1 / pass-2 composite-radix flags,
- * pass-1 / pass-2 LDS stages. */
-int swspec_hip_plan_info(int64_t fft_points, int32_t* info, int ninfo);
+/* Plan geometry for an FFT length (no device work) under the plan-shaping
+ * flags (SWSPEC_FLAG_MFMA64).  info[0..13] = N1, N2, pass-1 columns per
+ * workgroup, pass-1 threads, pass-2 threads, row pairs, pass-1 / pass-2
+ * points per thread, pass-1 / pass-2 composite-radix flags, pass-1 / pass-2
+ * stages, MFMA radix-64 stage on, first pass-1 radix. */
+int swspec_hip_plan_info(int64_t fft_points, uint32_t flags, int32_t* info, int ninfo);
 
 /*
  * ---- phase-cal extractor classes --------------------------------------------

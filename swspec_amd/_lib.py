@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libswspec_hip.so")
 SWSPEC_HIP_ABI_VERSION = 2
 FLAG_NO_KAHAN = 0x1
 FLAG_MK5B_FRAMED = 0x2   # Mk5B input as whole 10016-byte frames; headers skipped on the device
+FLAG_MFMA64 = 0x4        # first column stage: radix 64 as a split-FP16 MFMA product
 
 WINDOWS = {"None": 0, "Cosine": 1, "Cosine2": 2, "Hamming": 3, "Hann": 4, "Blackman": 5}
 FORMATS = {"RawSigned": 0, "RawUnsigned": 1, "Mk5B": 2, "iBOB": 3, "VDIF": 4, "VLBA": 5,
@@ -131,7 +132,7 @@ def load():
     L.swspec_hip_host_free.argtypes = [P]
     L.swspec_hip_host_free.restype = None
     L.swspec_hip_unpack_device.argtypes = [C.POINTER(Cfg), P, C.c_int, C.c_int64, P]
-    L.swspec_hip_plan_info.argtypes = [C.c_int64, C.POINTER(C.c_int32), C.c_int]
+    L.swspec_hip_plan_info.argtypes = [C.c_int64, C.c_uint32, C.POINTER(C.c_int32), C.c_int]
     L.swspec_hip_set_profiling.argtypes = [P, C.c_int]
     L.swspec_hip_kernel_stats.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.swspec_hip_server_create.argtypes = [C.c_int, C.c_int64, C.c_int, C.POINTER(P)]

@@ -416,7 +416,8 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
         for (hipEvent_t* e : {&c->ev_in, &c->ev_pc, &c->ev_em})
             HIPCHK(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
-    std::string e = plan_create(c->plan, cfg->fft_points, cfg->window, c->stream);
+    std::string e = plan_create(c->plan, cfg->fft_points, cfg->window, c->stream,
+                                (cfg->flags & SWSPEC_FLAG_MFMA64) != 0);
     if (!e.empty()) return fail(c, SWSPEC_EUNSUPPORTED, e);
 
     const long long M = c->plan.M;
@@ -1019,13 +1020,13 @@ extern "C" int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d
     return SWSPEC_OK;
 }
 
-extern "C" int swspec_hip_plan_info(int64_t fft_points, int32_t* info, int ninfo)
+extern "C" int swspec_hip_plan_info(int64_t fft_points, uint32_t flags, int32_t* info, int ninfo)
 {
     Plan p;
-    std::string e = plan_geometry(p, fft_points);
+    std::string e = plan_geometry(p, fft_points, (flags & SWSPEC_FLAG_MFMA64) != 0);
     if (!e.empty()) return fail(nullptr, SWSPEC_EUNSUPPORTED, e);
-    int32_t v[12] = {p.N1,        p.N2,        p.B,      p.p1_threads, p.sp2.T * 2, p.npairs,
-                     p.p1_maxpts, p.p2_maxpts, p.p1_cx, p.p2_cx,      p.sp1.nst,   p.sp2.nst};
-    for (int i = 0; i < ninfo && i < 12; i++) info[i] = v[i];
+    int32_t v[14] = {p.N1,        p.N2,        p.B,      p.p1_threads, p.sp2.T * 2, p.npairs,    p.p1_maxpts,
+                     p.p2_maxpts, p.p1_cx,     p.p2_cx,  p.sp1.nst,    p.sp2.nst,   p.mfma64,    p.sp1.radix[0]};
+    for (int i = 0; i < ninfo && i < 14; i++) info[i] = v[i];
     return SWSPEC_OK;
 }

@@ -105,6 +105,7 @@ struct Plan {
     int p1_maxpts = 0, p2_maxpts = 0;
     int p1_cx = 0, p2_cx = 0;    // sub-plan uses composite register radices (10..32)
     int p1_fpw = 1;          // FFT pipelines per pass-1 workgroup (short columns)
+    int mfma64 = 0;          // pass 1 opens with the radix-64 split-FP16 MFMA stage
     int smax = 1;            // accumulator slices per job (pass-2 FFT parallelism)
     int npairs = 0;          // pass-2 workgroups per FFT (row pairs)
     int twshift = 0;         // four-step twiddle split W_M^e = A[e>>s] * B[e & (2^s-1)]
@@ -118,15 +119,18 @@ struct Plan {
     float2* d_rA = nullptr;     // W_N^{k1}, k1 < N1
     float2* d_rB = nullptr;     // W_N^{N1 k2}, k2 < N2
     float2* d_winphi = nullptr; // N1 window row phases (cos, sin)
+    uint4* d_mft = nullptr;     // radix-64 MFMA stage: DFT-64 A operands [8][4][hi, lo][64 lanes]
     WindowGen win;
 };
 
 // Build a plan (host tables in FP64, rounded to FP32 once).  Returns an
 // error string (empty on success).
-std::string plan_create(Plan& p, long long N, int window, hipStream_t stream);
+// mfma64: open the column FFT with a radix-64 stage on the matrix cores
+// (needs a column length that is a multiple of 64).
+std::string plan_create(Plan& p, long long N, int window, hipStream_t stream, bool mfma64 = false);
 void plan_destroy(Plan& p);
 // Factorisation only (no device work): used by tests and the tuner.
-std::string plan_geometry(Plan& p, long long N);
+std::string plan_geometry(Plan& p, long long N, bool mfma64 = false);
 
 // One job as the kernels see it.
 struct JobDev {

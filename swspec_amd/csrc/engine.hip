@@ -448,6 +448,111 @@ template <int U> __device__ __forceinline__ int batch_idx(int base, int u, int n
     return min(base + u * nth, total - 1);
 }
 
+// ---------------------------------------------------------------------------
+// Radix-64 first column stage on the matrix cores (split FP16, FP32 sums)
+// ---------------------------------------------------------------------------
+
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+constexpr float kMfLo = 2048.f;   // lo parts are stored times 2^11
+constexpr int kMfPairs = 8;       // (row tile, column tile) pairs per wave
+
+// x = hi + lo / 2^11 with hi, lo in FP16 (round to nearest): the relative
+// error of the pair is about 2^-22.  Returned as the bit patterns of one
+// LDS element: .x = (hi re, hi im), .y = (lo re, lo im).
+// lo must be taken against the hi that is stored.  Left alone, the compiler
+// contracts the multiply that produced a and b into one of the two uses of
+// (_Float16)a (v_fma_mixlo_f16 rounds the exact product) and not the other
+// (v_cvt_pk_f16_f32 of the FP32 product); a value on an FP16 rounding tie
+// then came back one FP16 ulp off.  The empty asm statements pin a, b and hi
+// to single values.
+__device__ __forceinline__ float2 split16(float a, float b)
+{
+    asm volatile("" : "+v"(a), "+v"(b));
+    const h2_t h0 = {(_Float16)a, (_Float16)b};
+    unsigned hb = __builtin_bit_cast(unsigned, h0);
+    asm volatile("" : "+v"(hb));
+    const h2_t hi = __builtin_bit_cast(h2_t, hb);
+    const h2_t lo = {(_Float16)((a - (float)hi.x) * kMfLo), (_Float16)((b - (float)hi.y) * kMfLo)};
+    return make_float2(__uint_as_float(hb), __builtin_bit_cast(float, lo));
+}
+
+// The stage's 64-point DFTs X[r] = sum_s x[s] W_64^{rs} of all V = (N1/64)*B
+// column vectors of the workgroup form one real product
+//     [X re/im interleaved] (128 x V) = F (128 x 128) [x re/im interleaved] (128 x V).
+// The fill stored x split (split16); F = Fhi + Flo/2^11 comes from a host
+// table in FP64.  D = Fhi xhi + (Fhi xlo + Flo xhi)/2^11: the dropped lo*lo
+// term is ~2^-22 relative.  v_mfma_f32_16x16x32_f16 makes one 16 x 16 tile
+// from K = 32 (16 complex inputs) per instruction.  Wave w takes row tiles
+// rt = w, w + W, ... (16 rows = 8 frequencies each) and every column tile ct
+// of 16 vectors; the A operands of a row tile (4 K steps x hi/lo) stay in
+// registers.  Lane l covers row / column l % 16 of a tile and K slots
+// 8(l/16) + t, t < 8, which both operands map to complex input
+// s = 16 kc + 4(l/16) + t/2, part t % 2; the result rows of lane l are
+// 4(l/16) + t, t < 4.  Vector v = q*B + b is butterfly q of column b: its
+// input s sits at lp[v + s*V] (element q + s*N1/64) and output r goes to
+// element 64 q + r, the first Stockham stage's order.  A last column tile
+// that V does not fill computes zeros in its lanes v >= V and stores nothing
+// there.  All outputs are held in registers until every wave has read its
+// inputs (in-place stage).
+__device__ __forceinline__ void stage_mfma64(float2* __restrict__ lp, const uint4* __restrict__ mft, int N1, int B)
+{
+    const int lane = (int)(threadIdx.x & 63u);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int W = (int)(blockDim.x >> 6);
+    const int g = lane >> 4, j = lane & 15;
+    const int V = (N1 >> 6) * B;
+    const int nct = (V + 15) >> 4;
+    f4_t res[kMfPairs];
+    h8_t ah[4], al[4];
+#pragma unroll
+    for (int i = 0; i < kMfPairs; i++) {
+        const int rt = wv + W * (i / nct);
+        const int ct = i % nct;
+        if (rt >= 8) break;
+        if (ct == 0) {
+#pragma unroll
+            for (int kc = 0; kc < 4; kc++) {
+                ah[kc] = __builtin_bit_cast(h8_t, mft[((rt * 4 + kc) * 2 + 0) * 64 + lane]);
+                al[kc] = __builtin_bit_cast(h8_t, mft[((rt * 4 + kc) * 2 + 1) * 64 + lane]);
+            }
+        }
+        const int v = ct * 16 + j;
+        const bool in = v < V;
+        f4_t d = {0.f, 0.f, 0.f, 0.f}, e = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) {
+            float2 x[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) x[t] = in ? lp[v + (16 * kc + 4 * g + t) * V] : make_float2(0.f, 0.f);
+            const uint4 bh4 = make_uint4(__float_as_uint(x[0].x), __float_as_uint(x[1].x), __float_as_uint(x[2].x),
+                                         __float_as_uint(x[3].x));
+            const uint4 bl4 = make_uint4(__float_as_uint(x[0].y), __float_as_uint(x[1].y), __float_as_uint(x[2].y),
+                                         __float_as_uint(x[3].y));
+            const h8_t bh = __builtin_bit_cast(h8_t, bh4), bl = __builtin_bit_cast(h8_t, bl4);
+            d = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[kc], bh, d, 0, 0, 0);
+            e = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[kc], bl, e, 0, 0, 0);
+            e = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[kc], bh, e, 0, 0, 0);
+        }
+        res[i] = d + e * (1.f / kMfLo);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kMfPairs; i++) {
+        const int rt = wv + W * (i / nct);
+        const int ct = i % nct;
+        if (rt >= 8) break;
+        const int v = ct * 16 + j;
+        if (v >= V) continue;
+        const int q = v / B, b = v - q * B;
+        const int o = (q * 64 + 8 * rt + 2 * g) * B + b;
+        lp[o] = make_float2(res[i].x, res[i].y);
+        lp[o + B] = make_float2(res[i].z, res[i].w);
+    }
+    __syncthreads();
+}
+
 struct P1Args {
     int N1, N2;
     long long M;
@@ -458,6 +563,7 @@ struct P1Args {
     const float2* stw;
     const float2* twA;
     const float2* twB;
+    const uint4* mft; // MF: DFT-64 operand table
     WindowGen win;
     int zfast;        // jobs share raw bytes: job index fastest in the workgroup order
     JobDev jobs[kMaxJobs];
@@ -490,7 +596,9 @@ __device__ __forceinline__ void xcd_remap(unsigned& x, unsigned& y, unsigned& z,
     z = yz / gy;
 }
 
-template <int MAXPTS, bool CX>
+// MF: the first column stage is radix 64 on the matrix cores (stage_mfma64;
+// one pipeline per workgroup, whole waves)
+template <int MAXPTS, bool CX, bool MF = false>
 __global__ void __launch_bounds__(512) k_pass1(P1Args a)
 {
     extern __shared__ float2 lds[];
@@ -614,7 +722,8 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
                             w1 += c2 * R2[1] + s2 * S2[1];
                         }
                     }
-                    lp[idx] = make_float2(x0[u] * w0, x1[u] * w1);
+                    if constexpr (MF) lp[idx] = split16(x0[u] * w0, x1[u] * w1);
+                    else lp[idx] = make_float2(x0[u] * w0, x1[u] * w1);
                 }
             }
         };
@@ -648,7 +757,8 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     const int rlast = sp.radix[sp.nst - 1];
     const bool fuse = sp.nst >= 2 && (rlast == 2 || rlast == 3 || rlast == 4 || rlast == 5 || rlast == 8);
     const int ebase = pipe * (N1 * B) + bcol;
-    fft_lds<MAXPTS, CX>(lds, sp, a.stw, ebase, B, et, estep, fuse ? sp.nst - 1 : sp.nst);
+    if constexpr (MF) stage_mfma64(lds, a.mft, N1, B);
+    fft_lds<MAXPTS, CX>(lds, sp, a.stw, ebase, B, et, estep, fuse ? sp.nst - 1 : sp.nst, MF ? 1 : 0);
 
     float2* __restrict__ Y = J.Y + (long long)(active ? f : 0) * a.M;
     const unsigned mask = (1u << a.twshift) - 1u;
@@ -1600,8 +1710,17 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
     if (njobs < 1 || njobs > kMaxJobs || nfft < 1) return hipErrorInvalidValue;
     P1Args a;
     a.N1 = p.N1; a.N2 = p.N2; a.M = p.M; a.B = p.B; a.twshift = p.twshift;
-    a.sp = p.sp1; a.stw = p.d_stw1; a.twA = p.d_twA; a.twB = p.d_twB; a.win = p.win;
+    a.sp = p.sp1; a.stw = p.d_stw1; a.twA = p.d_twA; a.twB = p.d_twB; a.win = p.win; a.mft = p.d_mft;
     a.fpw = std::max(1, std::min(p.p1_fpw, nfft));
+    if (p.mfma64) {
+        // stage_mfma64 assumes one pipeline of whole waves and at most
+        // kMfPairs (row tile, column tile) pairs per wave
+        const int W = p.p1_threads / 64, nct = ((p.N1 / 64) * p.B + 15) / 16;
+        if (!p.d_mft || p.p1_maxpts != 16 || p.p1_fpw != 1 || p.N1 % 64 || p.p1_threads % 64 || W < 1 ||
+            ((8 + W - 1) / W) * nct > kMfPairs)
+            return hipErrorInvalidValue;
+        a.fpw = 1;
+    }
     a.zfast = 0;
     for (int j = 1; j < njobs; j++)
         if (jobs[j].raw == jobs[0].raw && jobs[j].fft_stride == jobs[0].fft_stride) a.zfast = 1;
@@ -1611,6 +1730,11 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
     dim3 grid(p.N2 / p.B, (nfft + a.fpw - 1) / a.fpw, njobs);
     const size_t lds = p1_lds(p) * a.fpw;
     const dim3 block(a.fpw * a.pth);
+    if (p.mfma64) {
+        if (p.p1_cx) hipLaunchKernelGGL((k_pass1<16, true, true>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((k_pass1<16, false, true>), grid, block, lds, s, a);
+        return hipGetLastError();
+    }
     switch (p.p1_maxpts * 2 + (p.p1_cx ? 1 : 0)) {
     case 32: hipLaunchKernelGGL((k_pass1<16, false>), grid, block, lds, s, a); break;
     case 33: hipLaunchKernelGGL((k_pass1<16, true>), grid, block, lds, s, a); break;
@@ -2011,7 +2135,7 @@ static int subplan_threads(const SubPlan& sp, int maxpts)
     return radices_threads(sp.L, sp.radix, sp.nst, maxpts);
 }
 
-std::string plan_geometry(Plan& p, long long N)
+std::string plan_geometry(Plan& p, long long N, bool mfma64)
 {
     if (N < 16 || (N % 2) != 0) return "FFT length must be even and >= 16";
     const long long M = N / 2;
@@ -2025,8 +2149,21 @@ std::string plan_geometry(Plan& p, long long N)
         best1 = d;
     }
     if (best1 < 2) return "no usable factorisation";
+    // radix-64 MFMA stage: the column length must be a multiple of 64; the
+    // divisor closest to sqrt(M) among those (either side of it)
+    if (mfma64) {
+        long long m1 = 0;
+        double bestd = 0.0;
+        for (long long d = 64; d <= M && d <= 8192; d += 64) {
+            if (M % d || !smooth235(d) || M / d > 16384) continue;
+            const double dist = std::fabs(std::log((double)d) - 0.5 * std::log((double)M));
+            if (m1 == 0 || dist < bestd) { m1 = d; bestd = dist; }
+        }
+        if (m1 == 0) return "radix-64 MFMA stage needs a column length that is a multiple of 64";
+        best1 = m1;
+    }
     // tuning override: SWSPEC_N1 picks the column length when it divides M
-    if (const char* e = getenv("SWSPEC_N1")) {
+    if (const char* e = mfma64 ? nullptr : getenv("SWSPEC_N1")) {
         const long long n1 = atoll(e);
         if (n1 >= 2 && (M % n1) == 0 && smooth235(n1) && smooth235(M / n1) && M / n1 <= 16384) best1 = n1;
     }
@@ -2147,6 +2284,59 @@ std::string plan_geometry(Plan& p, long long N)
     p.sp1.T = T1;
     p.B = B;
     p.p1_threads = B * T1;
+    if (mfma64) {
+        // stages [64, rest of N1/64]; B columns (the most that fit the LDS
+        // budget), whole waves, and at most kMfPairs (row tile, column tile)
+        // pairs per wave for the V = (N1/64)*B vectors in 16-column tiles
+        const int L1 = p.N1 / 64;
+        int list[kMaxStages], nl = 1, rcx = 0;
+        list[0] = 64;
+        if (L1 > 1) {
+            SubPlan r;
+            if (!subplan(L1, 16, cxmask & 1, r, rcx, p1_order)) return "radix decomposition failed";
+            for (int s = 0; s < r.nst && nl < kMaxStages; s++) list[nl++] = r.radix[s];
+        }
+        subplan_from(p.N1, list, nl, p.sp1);
+        const int T1m = nl > 1 ? radices_threads(p.N1, list + 1, nl - 1, 16) : 1;
+        int Bm = 0, thr = 0;
+        for (size_t budget : {(size_t)40 * 1024, (size_t)80 * 1024}) {
+            for (int c : {16, 8, 4, 2, 1}) {
+                if (p.N2 % c || (size_t)c * p.N1 * sizeof(float2) > budget) continue;
+                const int nct = (L1 * c + 15) / 16;
+                int t = ((c * T1m + 63) / 64) * 64;
+                while (t <= 512 && ((8 + t / 64 - 1) / (t / 64)) * nct > kMfPairs) t += 64;
+                if (t > 512) continue;
+                Bm = c;
+                thr = t;
+                break;
+            }
+            if (Bm) break;
+        }
+        if (!Bm) return "radix-64 MFMA stage: no column-pass workgroup shape fits";
+        // tuning overrides: SWSPEC_MF_B (columns, a divisor of N2), SWSPEC_MF_THREADS
+        if (const char* e = getenv("SWSPEC_MF_B")) {
+            const int c = atoi(e);
+            if (c >= 1 && c <= 16 && p.N2 % c == 0 && (size_t)c * p.N1 * sizeof(float2) <= 80 * 1024) {
+                const int nct = (L1 * c + 15) / 16;
+                int t = ((c * T1m + 63) / 64) * 64;
+                while (t % c) t += 64;
+                while (t <= 512 && ((8 + t / 64 - 1) / (t / 64)) * nct > kMfPairs) t += 64;
+                if (t <= 512) { Bm = c; thr = t; }
+            }
+        }
+        if (const char* e = getenv("SWSPEC_MF_THREADS")) {
+            const int t = atoi(e), nct = (L1 * Bm + 15) / 16;
+            if (t >= 64 && t <= 512 && t % 64 == 0 && t % Bm == 0 && t >= Bm * T1m &&
+                ((8 + t / 64 - 1) / (t / 64)) * nct <= kMfPairs)
+                thr = t;
+        }
+        p.p1_maxpts = 16;
+        p.p1_cx = rcx;
+        p.B = Bm;
+        p.sp1.T = thr / Bm;
+        p.p1_threads = thr;
+        p.mfma64 = 1;
+    }
 
     // pass 2: 2 rows (x2 for cross-power pairs) of N2 points, <= 16 bins/thread,
     // <= 512 threads with cross-power
@@ -2180,6 +2370,7 @@ std::string plan_geometry(Plan& p, long long N)
     // short columns: several FFTs per pass-1 workgroup (<= 256 threads, <= 64 KB LDS)
     p.p1_fpw = std::max(1, std::min(256 / p.p1_threads,
                                     (int)((64 * 1024) / ((size_t)p.N1 * p.B * sizeof(float2)))));
+    if (p.mfma64) p.p1_fpw = 1;
     // accumulator slices: up to 1024, at most 16 MiB of slices per job
     long long slice_mb = 16;
     if (const char* e = getenv("SWSPEC_SLICE_MB")) slice_mb = std::max(1LL, atoll(e));
@@ -2247,10 +2438,43 @@ static hipError_t upload(T** d, const std::vector<T>& h, hipStream_t s)
     return hipStreamSynchronize(s);
 }
 
-std::string plan_create(Plan& p, long long N, int window, hipStream_t stream)
+// A operands of stage_mfma64: entry ((rt*4 + kc)*2 + h)*64 + lane holds the
+// 8 FP16 values of lane `lane` for row tile rt, K step kc, h = 0 (hi) or 1
+// (lo * 2^11).  Lane row i = lane % 16 is output row R = 16 rt + i (frequency
+// r = R/2, part R % 2); slot t is input s = 16 kc + 4 (lane/16) + t/2, part
+// t % 2.  With theta = 2 pi (r s mod 64)/64 the real 2 x 2 block of W_64^{rs}
+// acting on (x re, x im) is [cos, sin; -sin, cos].
+static std::vector<uint4> mfma64_table()
 {
-    std::string err = plan_geometry(p, N);
+    std::vector<uint4> t(8 * 4 * 2 * 64);
+    for (int rt = 0; rt < 8; rt++)
+        for (int kc = 0; kc < 4; kc++)
+            for (int lane = 0; lane < 64; lane++) {
+                const int R = 16 * rt + (lane & 15), r = R >> 1, pr = R & 1;
+                _Float16 hi[8], lo[8];
+                for (int k = 0; k < 8; k++) {
+                    const int sidx = 16 * kc + 4 * (lane >> 4) + (k >> 1), pc = k & 1;
+                    const double th = 2.0 * M_PI * (double)((r * sidx) % 64) / 64.0;
+                    const double c = cos(th), sn = sin(th);
+                    const double val = pr == 0 ? (pc == 0 ? c : sn) : (pc == 0 ? -sn : c);
+                    hi[k] = (_Float16)val;
+                    lo[k] = (_Float16)((val - (double)hi[k]) * (double)kMfLo);
+                }
+                uint4 h, l;
+                memcpy(&h, hi, 16);
+                memcpy(&l, lo, 16);
+                t[((rt * 4 + kc) * 2 + 0) * 64 + lane] = h;
+                t[((rt * 4 + kc) * 2 + 1) * 64 + lane] = l;
+            }
+    return t;
+}
+
+std::string plan_create(Plan& p, long long N, int window, hipStream_t stream, bool mfma64)
+{
+    std::string err = plan_geometry(p, N, mfma64);
     if (!err.empty()) return err;
+    if (p.mfma64 && upload(&p.d_mft, mfma64_table(), stream) != hipSuccess)
+        return "device allocation failed (MFMA stage table)";
     p.window = window;
     std::vector<float2> t;
     stage_twiddles(p.sp1, t);
@@ -2291,10 +2515,11 @@ std::string plan_create(Plan& p, long long N, int window, hipStream_t stream)
 void plan_destroy(Plan& p)
 {
     for (void* q : {(void*)p.d_stw1, (void*)p.d_stw2, (void*)p.d_twA, (void*)p.d_twB, (void*)p.d_rA,
-                    (void*)p.d_rB, (void*)p.d_winphi})
+                    (void*)p.d_rB, (void*)p.d_winphi, (void*)p.d_mft})
         if (q) (void)hipFree(q);
     p.d_stw1 = p.d_stw2 = p.d_twA = p.d_twB = p.d_rA = p.d_rB = nullptr;
     p.d_winphi = nullptr;
+    p.d_mft = nullptr;
 }
 
 }  // namespace swspec

@@ -4,6 +4,8 @@ import ctypes as C
 import os
 import re
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -49,3 +51,30 @@ def test_derive_errors_and_plan_info():
     assert info["N1"] * info["N2"] == 1600000
     with pytest.raises(RuntimeError):
         swspec_amd.plan_info(2 * 7 * 1000)   # prime factor 7: no plan
+    assert info["mfma64"] == 0
+
+
+@pytest.mark.parametrize("n", [1 << 20, 3200000, 65536, 32000, 8192, 4096, 320000, 1 << 22])
+def test_plan_info_mfma64(n):
+    """Radix-64 MFMA stage geometry: a column length N1 that is a multiple
+    of 64 and divides N/2, stages [64, ...] multiplying to N1, one pipeline
+    of whole waves with at most 8 (row tile, column tile) pairs per wave."""
+    import swspec_amd
+    from swspec_amd._lib import FLAG_MFMA64
+    i = swspec_amd.plan_info(n, FLAG_MFMA64)
+    assert i["mfma64"] == 1 and i["p1_radix0"] == 64
+    assert i["N1"] % 64 == 0 and i["N1"] * i["N2"] == n // 2 and i["N2"] % i["B"] == 0
+    assert i["p1_threads"] % 64 == 0 and i["p1_threads"] <= 512 and i["p1_maxpts"] == 16
+    waves = i["p1_threads"] // 64
+    tiles = ((i["N1"] // 64) * i["B"] + 15) // 16
+    assert -(-8 // waves) * tiles <= 8
+    assert i["N1"] * i["B"] * 8 <= 80 * 1024
+
+
+@pytest.mark.parametrize("n", [20000, 1000000])
+def test_plan_info_mfma64_rejects(n):
+    """N/2 = 10^4 and 5 x 10^5 have no divisor that is a multiple of 64."""
+    import swspec_amd
+    from swspec_amd._lib import FLAG_MFMA64
+    with pytest.raises(RuntimeError):
+        swspec_amd.plan_info(n, FLAG_MFMA64)
