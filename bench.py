@@ -174,6 +174,51 @@ def plain_cfg2(torch, swspec_amd, raw, s, reps=2):
     return NCH * SAMPLES / dt / 1e9, dt * 1e3
 
 
+def xpol_cfg3_leg(torch, swspec_amd, nfft=64, reps=5):
+    """Cross-pol pair at 2^20 points: X and Y 2-bit single-channel streams at
+    64 Ms/s, XX, YY and X.Y* with Kahan FP32 accumulators, nfft FFTs per
+    source per step, device-resident input; column FFT in FP32 and with the
+    radix-64 split-FP16 MFMA first stage."""
+    from swspec_amd import TaskCoreHIP
+    from swspec_amd._lib import FLAG_MFMA64
+    n, fs = 1 << 20, 64e6
+    s = swspec_amd.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nfft / fs, FFToverlapFactor=1,
+                                   WindowType="Hann", BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=1,
+                                   DoCrossPolarization=True, MaxSourceBufferMB=4096, num_sources=2)
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    nbytes = n // 4 * nfft
+    raws = [torch.randint(0, 256, (nbytes,), dtype=torch.uint8, device=dev, generator=g) for _ in range(2)]
+    ssb = s.fft_ssb_points
+    res = {"fft_points": n, "ffts_per_source": nfft, "sources": 2, "window": "Hann"}
+    for name, fl in (("fp32", 0), ("mfma64", FLAG_MFMA64)):
+        core = TaskCoreHIP(0)
+        core.prepare(s, flags=fl)
+        d_auto = [torch.empty(ssb, device=dev) for _ in range(2)]
+        d_x = [torch.empty(2 * ssb, device=dev)]
+        for _ in range(2):
+            core.run_device(raws, [nbytes, nbytes], d_auto, d_x, None)
+            assert core.join() == 1
+        core.set_profiling(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            core.run_device(raws, [nbytes, nbytes], d_auto, d_x, None)
+            assert core.join() == 1
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        kst = core.kernel_stats()
+        core.set_profiling(False)
+        core.finalize()
+        res[name] = {"gsamples_per_s": round(2 * n * nfft / dt / 1e9, 2), "ms_per_step": round(dt * 1e3, 3),
+                     "pass1_ms_per_step": round(kst["pass1"][0] / reps, 3),
+                     "pass2_ms_per_step": round(kst["pass2"][0] / reps, 3),
+                     "plan": swspec_amd.plan_info(n, fl)}
+    res["mfma_vs_fp32"] = round(res["mfma64"]["gsamples_per_s"] / res["fp32"]["gsamples_per_s"], 3)
+    return res
+
+
 def batch_cfg4(torch, replays=3, seed=4):
     """64 streams of mixed FFT sizes: one hipGraph replay of the whole batch
     vs plain per-stream launches (each stream alone, step by step)."""
@@ -301,6 +346,8 @@ def main():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     ap.add_argument("--no-pcalx", action="store_true", help="skip the standalone phase-cal extractor leg")
     ap.add_argument("--only-pcalx", action="store_true", help="run only the phase-cal extractor leg")
+    ap.add_argument("--no-xpol", action="store_true", help="skip the 2^20-point cross-pol leg (FP32 vs MFMA stage)")
+    ap.add_argument("--only-xpol", action="store_true", help="run only the 2^20-point cross-pol leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the workload is one GPU's share; "
@@ -324,6 +371,10 @@ def main():
 
     import swspec_amd
     from swspec_amd import TaskCoreHIP, plan_info
+
+    if args.only_xpol:
+        print(json.dumps({"xpol_2p20": xpol_cfg3_leg(torch, swspec_amd)}), flush=True)
+        return
 
     log("generating input on the device ...")
     raw, _ = make_input(torch)
@@ -401,6 +452,11 @@ def main():
         log("serving replay ...")
         serve = serve_cfg5(torch, args.serve_seconds, plain_seconds=min(args.serve_seconds, 3.0))
 
+    xpol = None
+    if not args.no_xpol:
+        log("cross-pol 2^20 (fp32 / mfma64 column stage) ...")
+        xpol = xpol_cfg3_leg(torch, swspec_amd)
+
     pcalx = None
     if not args.no_pcalx:
         log("phase-cal extractor ...")
@@ -418,7 +474,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded on-device noise + comb)",
         "config": {"workload": "4-ch Mk5B 2-bit, 32 Ms/s/ch, 10 s/ch, 3.2M-pt Cosine2 FFT, 100 FFTs/ch, "
-                               "16-tone phase-cal, plain per-job launch",
+                               "16-tone phase-cal, all 4 channels in one core (batched)",
                    "fft_points": N, "channels": NCH, "samples_per_channel": SAMPLES, "ffts_per_channel": NFFT,
                    "pcal_tones": 16, "plan": plan, "parallelism": "1 GPU"},
         "roofline": roofline,
@@ -431,6 +487,7 @@ def main():
         "batch_64_streams": batch,
         "serving": serve,
         "pcal_extractor": pcalx,
+        "xpol_2p20": xpol,
     }
     if cpu:
         out["gpu_vs_cpu"] = round(value / cpu["value"], 1)
