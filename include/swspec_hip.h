@@ -286,7 +286,7 @@ typedef struct swspec_hip_job {
     int32_t  channel;              /* 0-based */
     int32_t  nffts;                /* FFTs integrated (>= 1) */
     int32_t  extract_pcal;
-    int32_t  pad_;
+    int32_t  flags;                /* SWSPEC_FLAG_MFMA64 (other bits ignored) */
     double   samplingfreq;
     double   pcaloffsethz;
     double   pcalharmonicshz;

@@ -41,7 +41,7 @@ class Job(C.Structure):
         ("struct_size", C.c_uint32), ("window", C.c_int32), ("fft_points", C.c_int64),
         ("source_format", C.c_int32), ("source_channels", C.c_int32), ("bits_per_sample", C.c_int32),
         ("channelorder_increasing", C.c_int32), ("channel", C.c_int32), ("nffts", C.c_int32),
-        ("extract_pcal", C.c_int32), ("pad_", C.c_int32), ("samplingfreq", C.c_double),
+        ("extract_pcal", C.c_int32), ("flags", C.c_int32), ("samplingfreq", C.c_double),
         ("pcaloffsethz", C.c_double), ("pcalharmonicshz", C.c_double), ("d_raw", C.c_void_p),
         ("raw_len", C.c_uint64), ("d_out_auto", C.c_void_p), ("d_out_pcal", C.c_void_p),
         ("d_tones", C.c_void_p),

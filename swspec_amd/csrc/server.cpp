@@ -187,12 +187,13 @@ extern "C" int swspec_hip_server_submit(swspec_hip_server* s, const swspec_hip_j
     j->nwrap = (long long)(jd->raw_len / (uint64_t)j->rawfft);
     if (j->nwrap < 1) { delete j; return sfail(s, SWSPEC_EINVAL, "input shorter than one FFT"); }
     if (j->nwrap >= jd->nffts) j->nwrap = 0;   // no wrap needed
-    // plan cache
-    const auto key = std::make_pair((long long)jd->fft_points, (int)jd->window);
+    // plan cache, per (length, window, column stage kind)
+    const bool mfma = (jd->flags & SWSPEC_FLAG_MFMA64) != 0;
+    const auto key = std::make_pair((long long)jd->fft_points, (int)jd->window | (mfma ? 0x100 : 0));
     auto it = s->plans.find(key);
     if (it == s->plans.end()) {
         Plan* p = new Plan();
-        std::string e = plan_create(*p, jd->fft_points, jd->window, s->st[0]);
+        std::string e = plan_create(*p, jd->fft_points, jd->window, s->st[0], mfma);
         if (!e.empty()) { delete p; delete j; return sfail(s, SWSPEC_EUNSUPPORTED, e); }
         it = s->plans.emplace(key, p).first;
     }

@@ -29,7 +29,8 @@ class Server:
 
     def submit(self, fft_points, nffts, d_raw, raw_len, d_out_auto, window="Cosine2", source_format="RawSigned",
                source_channels=1, channel=0, samplingfreq=32e6, extract_pcal=False, pcaloffsethz=10e3,
-               pcalharmonicshz=1e6, d_out_pcal=None, d_tones=None, channelorder_increasing=True, keep=None):
+               pcalharmonicshz=1e6, d_out_pcal=None, d_tones=None, channelorder_increasing=True, keep=None,
+               flags=0):
         """Admit a job; returns its id, or None when the server is full (retry later)."""
         def ptr(x):
             return None if x is None else (x if isinstance(x, int) else x.data_ptr())
@@ -44,6 +45,7 @@ class Server:
         j.channel = int(channel)
         j.nffts = int(nffts)
         j.extract_pcal = 1 if extract_pcal else 0
+        j.flags = int(flags)
         j.samplingfreq = float(samplingfreq)
         j.pcaloffsethz = float(pcaloffsethz)
         j.pcalharmonicshz = float(pcalharmonicshz)

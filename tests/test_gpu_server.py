@@ -149,6 +149,39 @@ def test_result_same_alone_and_in_mix(hip, oracle):
     srv.close()
 
 
+def test_server_mfma64_jobs_in_mix(hip, oracle):
+    """A job flagged SWSPEC_FLAG_MFMA64 gets its own plan (same length,
+    radix-64 MFMA column stage) and runs beside the FP32-plan copy of itself
+    and jobs of other sizes: both within 1e-5 of the oracle and within 1e-6
+    of each other."""
+    import torch
+    from swspec_amd._lib import FLAG_MFMA64
+    from swspec_amd.server import Server
+    fs, n, nffts = 32e6, 320000, 6
+    raw, fmt, nch = _mkjob("mk5b4", n, 23, nffts, fs)
+    d_raw = torch.from_numpy(raw).cuda()
+    srv = Server()
+    outs = {}
+    for name, fl in (("fp32", 0), ("mfma", FLAG_MFMA64)):
+        outs[name] = torch.empty(n // 2 + 1, device="cuda")
+        assert srv.submit(n, nffts, d_raw, len(raw), outs[name], source_format=fmt, source_channels=4, channel=1,
+                          samplingfreq=fs, flags=fl) is not None
+    for i, (m, k) in enumerate([(65536, 9), (1 << 20, 3)]):
+        r, f, c = _mkjob("raw1", m, 400 + i, k, fs)
+        d = torch.from_numpy(r).cuda()
+        o = torch.empty(m // 2 + 1, device="cuda")
+        srv.submit(m, k, d, len(r), o, source_format=f, source_channels=c, samplingfreq=fs, keep=d,
+                   flags=FLAG_MFMA64 if i else 0)
+    while srv.step() > 0:
+        pass
+    srv.sync()
+    ref, _, _ = _oracle_mean(oracle, raw, n, nffts, fmt, nch, 1, "Cosine2", fs)
+    a, b = outs["fp32"].cpu().numpy(), outs["mfma"].cpu().numpy()
+    say("fp32 vs oracle %.3e, mfma vs oracle %.3e, mfma vs fp32 %.3e" % (relerr(a, ref), relerr(b, ref), relerr(b, a)))
+    assert relerr(a, ref) <= TOL and relerr(b, ref) <= TOL and relerr(b, a) <= 1e-6
+    srv.close()
+
+
 def test_server_admission_cap(hip):
     import torch
     from swspec_amd.server import Server
