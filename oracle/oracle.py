@@ -57,6 +57,38 @@ def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
 
 
+REF_SRC = "/root/reference/src"
+REF_FILESINK = os.path.join(HERE, "_ref", "filesink_ref")
+
+
+def build_ref():
+    """Compile oracle/_ref/ from the reference's own sources when they are
+    present (the reference FileSink and Buffer with our driver); returns
+    whether the reference FileSink binary is available."""
+    if os.path.isdir(REF_SRC):
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref", "REFSRC=" + REF_SRC])
+    return os.path.exists(REF_FILESINK)
+
+
+def ref_filesink(fmt, arrays, out_path, lengths=None):
+    """Write float32 arrays through the reference's FileSink::write (one
+    Buffer per array, length = its bytes unless `lengths` gives them);
+    returns the per-write return values."""
+    import tempfile
+    bufs = [np.ascontiguousarray(a, dtype=np.float32).view(np.uint8).ravel() for a in arrays]
+    lens = lengths if lengths is not None else [b.size for b in bufs]
+    # the driver takes each write's bytes in turn from one input file
+    data = np.concatenate([b[:int(n)] for b, n in zip(bufs, lens)]) if bufs else np.zeros(0, np.uint8)
+    with tempfile.NamedTemporaryFile(suffix=".f32", delete=False) as t:
+        data.tofile(t.name)
+    try:
+        out = subprocess.run([REF_FILESINK, fmt, t.name, str(out_path)] + [str(int(x)) for x in lens],
+                             capture_output=True, text=True, check=True).stdout
+    finally:
+        os.unlink(t.name)
+    return [int(v) for v in out.split()]
+
+
 _lib = None
 
 

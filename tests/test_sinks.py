@@ -1,6 +1,7 @@
 """Output side: file names (src/swspectrometer.cpp:234-254, 487-514) and the
 FileSink formats (src/FileSink.cpp:30-107), on the CPU."""
 import numpy as np
+import pytest
 
 from swspec_amd import sinks
 
@@ -79,3 +80,55 @@ def test_ascii_format_matches_iostream_precision_12():
     assert sinks._fmt12(np.float32(0.1)) == "0.10000000149"
     assert sinks._fmt12(np.float32(123456.7)) == "123456.703125"
     assert sinks._fmt12(np.float32(3.3359)) == "3.33590006828"
+
+
+# ---------------------------------------------------------------- vs the reference's own FileSink
+
+def _ref_available():
+    from oracle import oracle as O
+    try:
+        return O.build_ref()
+    except Exception:
+        return False
+
+
+def _write_both(tmp_path, fmt, arrays, lengths):
+    from oracle import oracle as O
+    ref, ours = tmp_path / ("ref_" + fmt), tmp_path / ("ours_" + fmt)
+    got_ref = O.ref_filesink(fmt, arrays, ref, lengths)
+    k = sinks.FileSink(fmt)
+    k.open(str(ours))
+    got = [k.write(_Buf(np.asarray(a, np.float32), n)) for a, n in zip(arrays, lengths)]
+    k.close()
+    return ref.read_bytes(), ours.read_bytes(), got_ref, got
+
+
+@pytest.mark.parametrize("fmt", ["Binary", "Ascii"])
+def test_sink_bytes_match_reference_filesink(tmp_path, fmt):
+    """sinks.FileSink against src/FileSink.cpp itself (built into oracle/_ref
+    from the reference sources with g++): random spectra of odd and even
+    lengths written in turn, an empty buffer and a length shorter than the
+    array (getLength, not the allocation) -- identical files and returns."""
+    if not _ref_available():
+        pytest.skip("reference sources not present: oracle/_ref not buildable")
+    rng = np.random.default_rng(11)
+    arrays = [rng.standard_normal(17).astype(np.float32) * 1e3,
+              (rng.random(16001) * 1e-3).astype(np.float32),
+              np.zeros(8, np.float32),
+              rng.standard_normal(10).astype(np.float32)]
+    lengths = [arrays[0].nbytes, arrays[1].nbytes, 0, 24]
+    ref, ours, rr, rg = _write_both(tmp_path, fmt, arrays, lengths)
+    assert rr == rg == [lengths[0], lengths[1], 0, 24]
+    assert ours == ref
+
+
+def test_ascii_special_values_match_reference_filesink(tmp_path):
+    """Values iostream prints in its own ways: -0, subnormals, powers of ten
+    at the 12-digit boundary, the largest float, +-inf and NaN."""
+    if not _ref_available():
+        pytest.skip("reference sources not present: oracle/_ref not buildable")
+    a = np.array([-0.0, 1e-45, 1e-38, 1e12, 123456789012.0, 3.4028235e38, np.inf, -np.inf, np.nan, 0.5,
+                  1e-5, 100.0], dtype=np.float32)
+    ref, ours, rr, rg = _write_both(tmp_path, "Ascii", [a], [a.nbytes])
+    assert rr == rg
+    assert ours.decode() == ref.decode()
