@@ -219,6 +219,56 @@ def xpol_cfg3_leg(torch, swspec_amd, nfft=64, reps=5):
     return res
 
 
+def host_cfg2_leg(torch, swspec_amd, raw, s, reps=4):
+    """The drop-in path as TaskDispatcher drives it: the headline 4-channel
+    job from a pinned host Buffer through run()/join(), outputs to host
+    Buffers, PCIe copies included.  Input copied in 16 MB segments on a copy
+    stream while earlier chunks compute, and as one copy before the passes
+    (SWSPEC_H2D_SEG_MB=0)."""
+    from swspec_amd import Buffer, TaskCoreHIP
+    inb = Buffer(SAMPLES, pinned=True)
+    inb.getData()[:] = raw[:SAMPLES].cpu().numpy()
+    inb.setLength(SAMPLES)
+    outs = [Buffer(s.fft_bytes_ssb, pinned=True) for _ in range(NCH)]
+    pcs = [Buffer(8 * s.pcal_tonebins, pinned=True) for _ in range(NCH)]
+    res = {"input": "pinned host Buffer, %d bytes (Mk5B payload)" % SAMPLES}
+    old = os.environ.pop("SWSPEC_H2D_SEG_MB", None)
+    try:
+        for name, seg in (("segmented_16MB", None), ("one_copy", "0")):
+            if seg is not None:
+                os.environ["SWSPEC_H2D_SEG_MB"] = seg
+            core = TaskCoreHIP(0)
+            core.prepare(s, channels=list(range(NCH)))
+            for _ in range(2):
+                core.run([inb], outs, [], pcs)
+                assert core.join() == 1
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                core.run([inb], outs, [], pcs)
+                assert core.join() == 1
+            dt = (time.perf_counter() - t0) / reps
+            core.finalize()
+            os.environ.pop("SWSPEC_H2D_SEG_MB", None)
+            res[name] = {"gsamples_per_s": round(NCH * SAMPLES / dt / 1e9, 2), "ms_per_step": round(dt * 1e3, 3)}
+    finally:
+        os.environ.pop("SWSPEC_H2D_SEG_MB", None)
+        if old is not None:
+            os.environ["SWSPEC_H2D_SEG_MB"] = old
+    # PCIe alone: one pinned 320 MB host-to-device copy
+    d = torch.empty(SAMPLES, dtype=torch.uint8, device="cuda:0")
+    h = torch.empty(SAMPLES, dtype=torch.uint8, pin_memory=True)
+    d.copy_(h, non_blocking=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        d.copy_(h, non_blocking=True)
+    torch.cuda.synchronize()
+    dtc = (time.perf_counter() - t0) / reps
+    res["h2d_copy_gb_per_s"] = round(SAMPLES / dtc / 1e9, 1)
+    res["pcie_bound_gsamples_per_s"] = round(NCH * SAMPLES / dtc / 1e9, 1)
+    return res
+
+
 def batch_cfg4(torch, replays=3, seed=4):
     """64 streams of mixed FFT sizes: one hipGraph replay of the whole batch
     vs plain per-stream launches (each stream alone, step by step)."""
@@ -348,6 +398,7 @@ def main():
     ap.add_argument("--only-pcalx", action="store_true", help="run only the phase-cal extractor leg")
     ap.add_argument("--no-xpol", action="store_true", help="skip the 2^20-point cross-pol leg (FP32 vs MFMA stage)")
     ap.add_argument("--only-xpol", action="store_true", help="run only the 2^20-point cross-pol leg")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-Buffer (PCIe-inclusive) leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the workload is one GPU's share; "
@@ -440,6 +491,11 @@ def main():
     assert np.isfinite(a0).all()
     core.finalize()
 
+    host = None
+    if not args.no_host:
+        log("host Buffers through run()/join() (PCIe included) ...")
+        host = host_cfg2_leg(torch, swspec_amd, raw, s)
+
     plain_value = plain_ms = None
     if not args.no_plain:
         log("plain per-channel cores ...")
@@ -488,6 +544,7 @@ def main():
         "serving": serve,
         "pcal_extractor": pcalx,
         "xpol_2p20": xpol,
+        "host_buffers_pcie_inclusive": host,
     }
     if cpu:
         out["gpu_vs_cpu"] = round(value / cpu["value"], 1)

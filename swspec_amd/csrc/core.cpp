@@ -68,6 +68,15 @@ struct swspec_hip_core {
     bool vlba = false;             // some job decodes VLBA
     uint32_t* d_vmod = nullptr;    // modulation bits
     int2* d_vblank = nullptr;      // [job][FFT of the chunk][segment] blank zones
+    // host-mode input copied in segments on its own stream (pinned host
+    // memory makes them asynchronous): a chunk's launches wait only for the
+    // segments holding its bytes, so PCIe overlaps the FFT passes
+    hipStream_t cstream = nullptr;
+    hipEvent_t ev_cs = nullptr;
+    std::vector<hipEvent_t> ev_seg;
+    size_t seg_bytes = 0;          // 0: input device-resident or copied whole
+    int nseg = 0;
+    int seg_waited = -1, seg_waited_p = -1;   // last segment waited for (main, phase-cal stream)
     // host-mode staging
     uint8_t* d_in[2] = {nullptr, nullptr};
     size_t d_in_cap = 0;
@@ -526,6 +535,7 @@ extern "C" int swspec_hip_finalize(swspec_hip_core* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->s2) (void)hipStreamSynchronize(c->s2);
     if (c->pstream) (void)hipStreamSynchronize(c->pstream);
+    if (c->cstream) (void)hipStreamSynchronize(c->cstream);
     plan_destroy(c->plan);
     for (void* p : {(void*)c->d_rot, (void*)c->d_Y, (void*)c->d_acc, (void*)c->d_xacc, (void*)c->d_pcal_part,
                     (void*)c->d_pcal_acc, (void*)c->d_in[0], (void*)c->d_in[1], (void*)c->d_out, (void*)c->d_tones,
@@ -537,6 +547,9 @@ extern "C" int swspec_hip_finalize(swspec_hip_core* c)
     if (c->s2) (void)hipStreamSynchronize(c->s2);
     for (hipEvent_t e : {c->ev_in, c->ev_pc, c->ev_em, c->ev_p1[0], c->ev_p1[1], c->ev_p2[0], c->ev_p2[1]})
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->ev_seg) (void)hipEventDestroy(e);
+    if (c->ev_cs) (void)hipEventDestroy(c->ev_cs);
+    if (c->cstream) (void)hipStreamDestroy(c->cstream);
     if (c->pstream) (void)hipStreamDestroy(c->pstream);
     if (c->s2) (void)hipStreamDestroy(c->s2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -603,6 +616,14 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
         HIPCHK(c, hipStreamWaitEvent(c->pstream, c->ev_in, 0));
     }
     hipStream_t rst = c->s2 ? c->s2 : c->stream;   // row-pass stream
+    // segmented host input: stream st waits for the segment holding byte need-1
+    auto wait_input = [&](hipStream_t st, int& waited, size_t need) -> hipError_t {
+        if (!c->seg_bytes || c->nseg < 1) return hipSuccess;
+        const int k = need == 0 ? 0 : (int)std::min<size_t>((need - 1) / c->seg_bytes, (size_t)c->nseg - 1);
+        if (k <= waited) return hipSuccess;
+        waited = k;
+        return hipStreamWaitEvent(st, c->ev_seg[k], 0);
+    };
     long long ci = 0;                             // chunk counter: Y buffer ci % ybufs
     int ylast = 0;
     for (size_t sp = 0; sp < sched.size(); sp++) {
@@ -641,6 +662,11 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
                     any_pcal |= (m != 0);
                 }
             }
+            // bytes the chunk reads (plus a word-alignment margin): the
+            // segmented copy is used only for plain payload layouts
+            const size_t need = std::min(len, (size_t)(w.start + (long long)(f0 + nf - 1) * g.raw_overlap_bytes +
+                                                       g.raw_fullfft_bytes + 64));
+            HIPCHK(c, wait_input(c->stream, c->seg_waited, need));
             {
                 ProfScope ps(c, SWSPEC_K_PASS1);
                 if (c->vlba) HIPCHK(c, launch_vlba_blank(jobs, c->njobs, nf, g.fft_points, c->stream));
@@ -651,6 +677,7 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
                 HIPCHK(c, hipStreamWaitEvent(c->s2, c->ev_p1[yb], 0));
             }
             if (any_pcal) {
+                HIPCHK(c, wait_input(pst, pst == c->stream ? c->seg_waited : c->seg_waited_p, need));
                 ProfScope ps(c, SWSPEC_K_PCAL, pst);
                 HIPCHK(c, launch_pcal(c->pg, g.fft_points, jobs, c->njobs, nf, pst));
             }
@@ -792,8 +819,43 @@ extern "C" int swspec_hip_run(swspec_hip_core* c, const void* const* in, const s
     }
     int rc = ensure_tones(c, nsp);
     if (rc) return rc;
-    for (int s = 0; s < c->nsrc; s++)
-        HIPCHK(c, hipMemcpyAsync(c->d_in[s], in[s], in_len[s], hipMemcpyHostToDevice, c->stream));
+    // Input copies: segments of SWSPEC_H2D_SEG_MB (default 16 MB; 0 = one
+    // copy on the main stream) on the copy stream, one event per segment.
+    // VLBA (the blanker probes past an FFT) and framed input (frame-addressed
+    // reads) wait for the whole input.
+    size_t seg = (size_t)16 << 20;
+    if (const char* e = getenv("SWSPEC_H2D_SEG_MB")) seg = (size_t)std::max(0L, atol(e)) << 20;
+    c->seg_bytes = 0;
+    c->nseg = 0;
+    c->seg_waited = c->seg_waited_p = -1;
+    if (seg == 0 || c->vlba || (c->cfg.flags & SWSPEC_FLAG_MK5B_FRAMED) || inmax == 0) {
+        for (int s = 0; s < c->nsrc; s++)
+            HIPCHK(c, hipMemcpyAsync(c->d_in[s], in[s], in_len[s], hipMemcpyHostToDevice, c->stream));
+    } else {
+        if (!c->cstream) {
+            HIPCHK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));   // default priority
+            HIPCHK(c, hipEventCreateWithFlags(&c->ev_cs, hipEventDisableTiming));
+        }
+        const int nseg = (int)((inmax + seg - 1) / seg);
+        while ((int)c->ev_seg.size() < nseg) {
+            hipEvent_t e = nullptr;
+            HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            c->ev_seg.push_back(e);
+        }
+        // the staging buffers are free once the main stream's earlier work is
+        HIPCHK(c, hipEventRecord(c->ev_cs, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->cstream, c->ev_cs, 0));
+        for (int k = 0; k < nseg; k++) {
+            const size_t lo = (size_t)k * seg;
+            for (int s = 0; s < c->nsrc; s++)
+                if (lo < in_len[s])
+                    HIPCHK(c, hipMemcpyAsync(c->d_in[s] + lo, (const uint8_t*)in[s] + lo,
+                                             std::min(seg, in_len[s] - lo), hipMemcpyHostToDevice, c->cstream));
+            HIPCHK(c, hipEventRecord(c->ev_seg[k], c->cstream));
+        }
+        c->seg_bytes = seg;
+        c->nseg = nseg;
+    }
     float* autos[kMaxJobs];
     float* pcals[kMaxJobs];
     float* base = c->d_out;
@@ -804,6 +866,13 @@ extern "C" int swspec_hip_run(swspec_hip_core* c, const void* const* in, const s
     const uint8_t* ins[2] = {c->d_in[0], c->d_in[1]};
     rc = run_common(c, ins, len, autos, xp, pcals, c->d_tones);
     if (rc) return rc;
+    // every input copy is done before join can return (the caller may then
+    // refill its buffer)
+    if (c->seg_bytes && c->nseg > 0 && c->seg_waited < c->nseg - 1) {
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_seg[c->nseg - 1], 0));
+        c->seg_waited = c->nseg - 1;
+    }
+    c->seg_bytes = 0;
     // results back to the caller's host arrays
     for (int j = 0; j < c->njobs; j++) {
         if (nsp) HIPCHK(c, hipMemcpyAsync(out_auto[j], autos[j], nsp * ssb * sizeof(float), hipMemcpyDeviceToHost, c->stream));

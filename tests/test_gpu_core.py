@@ -217,6 +217,66 @@ def test_host_mode_matches_device_mode(hip, oracle):
     c2.finalize()
 
 
+@pytest.mark.parametrize("seg_mb", ["1", "0", None])
+def test_host_segmented_input_matches_device_mode(hip, oracle, seg_mb):
+    """Host Buffers in pinned memory, copied in SWSPEC_H2D_SEG_MB segments on
+    the copy stream while earlier chunks compute (0: one copy; None: the
+    default 16 MB): spectra, cross spectrum and phase-cal bit-identical to the
+    same run on device-resident input.  4-channel Mk5B with phase-cal over 8
+    chunks, and a 2-source cross-pol pair over 4 chunks."""
+    import os
+    say("host segmented input", seg_mb)
+    old = os.environ.pop("SWSPEC_H2D_SEG_MB", None)
+    if seg_mb is not None:
+        os.environ["SWSPEC_H2D_SEG_MB"] = seg_mb
+    try:
+        fs = 32e6
+        cases = []
+        n, nf = 320000, 30
+        s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=1,
+                                WindowType="Cosine2", BandwidthHz=16e6, BitsPerSample=2, SourceChannels=4,
+                                SourceFormat="Mk5B", ExtractPCal=True, MaxSourceBufferMB=4096)
+        rng = np.random.default_rng(8)
+        tones = [synth.pcal_comb(fs, fs / 2, amp=0.141, rng=rng) for _ in range(4)]
+        cases.append((s, [synth.raw_multi(3, 4, n * nf, tones_per_ch=tones, fs=fs)], [0, 1, 2, 3]))
+        n, nf = 1 << 20, 8
+        s2 = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / 64e6, FFToverlapFactor=1,
+                                 WindowType="Hann", BandwidthHz=32e6, BitsPerSample=2, SourceChannels=1,
+                                 DoCrossPolarization=True, MaxSourceBufferMB=4096, num_sources=2)
+        cases.append((s2, [synth.raw_1ch(31, n * nf), synth.raw_1ch(32, n * nf)], None))
+        for st, raws, chans in cases:
+            m, auto, xp, pc, core = run_device(st, raws, channels=chans, ffts_per_chunk=2 if chans is None else 4)
+            core.finalize()
+            c2 = hip.TaskCoreHIP(1)
+            c2.prepare(st, channels=chans, ffts_per_chunk=2 if chans is None else 4)
+            ins = []
+            for r in raws:
+                b = hip.Buffer(len(r), pinned=True)
+                b.getData()[:] = r
+                b.setLength(len(r))
+                ins.append(b)
+            nj = c2.njobs
+            outs = [hip.Buffer(st.fft_bytes_ssb) for _ in range(nj)]
+            xb = [hip.Buffer(2 * st.fft_bytes_ssb)] if st.num_xpols else []
+            T = st.pcal_tonebins if st.extract_pcal else 0
+            pb = [hip.Buffer(8 * T) for _ in range(nj)] if T else []
+            c2.run(ins, outs, xb, pb)
+            assert c2.join() == m == 1
+            for j in range(nj):
+                assert np.array_equal(outs[j].getData().view(np.float32), auto[j][0]), (seg_mb, j)
+                if T:
+                    v = pb[j].getData().view(np.float32)
+                    assert np.array_equal(v[0::2] + 1j * v[1::2], pc[j][0].astype(np.complex64)), (seg_mb, j)
+            if xb:
+                v = xb[0].getData().view(np.float32)
+                assert np.array_equal(v[0::2] + 1j * v[1::2], xp[0][0].astype(np.complex64)), seg_mb
+            c2.finalize()
+    finally:
+        os.environ.pop("SWSPEC_H2D_SEG_MB", None)
+        if old is not None:
+            os.environ["SWSPEC_H2D_SEG_MB"] = old
+
+
 def test_combine_device_matches_host_combine(hip):
     """combineResults on device arrays (auto, cross and phase-cal outputs of
     two sources, two runs added up) equals the host combineResults of the
