@@ -234,7 +234,7 @@ def host_cfg2_leg(torch, swspec_amd, raw, s, reps=4):
     res = {"input": "pinned host Buffer, %d bytes (Mk5B payload)" % SAMPLES}
     old = os.environ.pop("SWSPEC_H2D_SEG_MB", None)
     try:
-        for name, seg in (("segmented_16MB", None), ("one_copy", "0")):
+        for name, seg in (("segmented_32MB", None), ("one_copy", "0")):
             if seg is not None:
                 os.environ["SWSPEC_H2D_SEG_MB"] = seg
             core = TaskCoreHIP(0)
@@ -391,6 +391,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain per-channel leg (profiling runs: only the measured launches)")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="Poisson arrival window")
+    ap.add_argument("--serve-rate-hi", type=float, default=550.0,
+                    help="arrival rate of the near-capacity serving point (jobs/s, 3 s window)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (3x): the reference point that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -507,6 +509,10 @@ def main():
     if not args.no_serve:
         log("serving replay ...")
         serve = serve_cfg5(torch, args.serve_seconds, plain_seconds=min(args.serve_seconds, 3.0))
+        # the same job mix near the device's capacity: continuous batching
+        # keeps up where one job at a time falls behind
+        log("serving near capacity ...")
+        serve["near_capacity"] = serve_cfg5(torch, 3.0, rate=args.serve_rate_hi, seed=6, plain_seconds=3.0)
 
     xpol = None
     if not args.no_xpol:

@@ -819,11 +819,11 @@ extern "C" int swspec_hip_run(swspec_hip_core* c, const void* const* in, const s
     }
     int rc = ensure_tones(c, nsp);
     if (rc) return rc;
-    // Input copies: segments of SWSPEC_H2D_SEG_MB (default 16 MB; 0 = one
+    // Input copies: segments of SWSPEC_H2D_SEG_MB (default 32 MB; 0 = one
     // copy on the main stream) on the copy stream, one event per segment.
     // VLBA (the blanker probes past an FFT) and framed input (frame-addressed
     // reads) wait for the whole input.
-    size_t seg = (size_t)16 << 20;
+    size_t seg = (size_t)32 << 20;
     if (const char* e = getenv("SWSPEC_H2D_SEG_MB")) seg = (size_t)std::max(0L, atol(e)) << 20;
     c->seg_bytes = 0;
     c->nseg = 0;
