@@ -222,7 +222,7 @@ def xpol_cfg3_leg(torch, swspec_amd, nfft=64, reps=5):
 def host_cfg2_leg(torch, swspec_amd, raw, s, reps=4):
     """The drop-in path as TaskDispatcher drives it: the headline 4-channel
     job from a pinned host Buffer through run()/join(), outputs to host
-    Buffers, PCIe copies included.  Input copied in 16 MB segments on a copy
+    Buffers, PCIe copies included.  Input copied in 32 MB segments on a copy
     stream while earlier chunks compute, and as one copy before the passes
     (SWSPEC_H2D_SEG_MB=0)."""
     from swspec_amd import Buffer, TaskCoreHIP

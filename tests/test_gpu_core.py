@@ -221,7 +221,7 @@ def test_host_mode_matches_device_mode(hip, oracle):
 def test_host_segmented_input_matches_device_mode(hip, oracle, seg_mb):
     """Host Buffers in pinned memory, copied in SWSPEC_H2D_SEG_MB segments on
     the copy stream while earlier chunks compute (0: one copy; None: the
-    default 16 MB): spectra, cross spectrum and phase-cal bit-identical to the
+    default 32 MB): spectra, cross spectrum and phase-cal bit-identical to the
     same run on device-resident input.  4-channel Mk5B with phase-cal over 8
     chunks, and a 2-source cross-pol pair over 4 chunks."""
     import os
