@@ -401,6 +401,7 @@ def main():
     ap.add_argument("--no-xpol", action="store_true", help="skip the 2^20-point cross-pol leg (FP32 vs MFMA stage)")
     ap.add_argument("--only-xpol", action="store_true", help="run only the 2^20-point cross-pol leg")
     ap.add_argument("--no-host", action="store_true", help="skip the host-Buffer (PCIe-inclusive) leg")
+    ap.add_argument("--only-host", action="store_true", help="run only the host-Buffer (PCIe-inclusive) leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the workload is one GPU's share; "
@@ -432,6 +433,9 @@ def main():
     log("generating input on the device ...")
     raw, _ = make_input(torch)
     s = settings(swspec_amd)
+    if args.only_host:
+        print(json.dumps({"host_buffers_pcie_inclusive": host_cfg2_leg(torch, swspec_amd, raw, s)}), flush=True)
+        return
     assert s.core_overlapped_ffts == NFFT and s.extract_pcal == 1
     core = TaskCoreHIP(0)
     core.prepare(s, channels=list(range(NCH)), ffts_per_chunk=args.chunk)

@@ -222,8 +222,9 @@ def test_host_segmented_input_matches_device_mode(hip, oracle, seg_mb):
     """Host Buffers in pinned memory, copied in SWSPEC_H2D_SEG_MB segments on
     the copy stream while earlier chunks compute (0: one copy; None: the
     default 32 MB): spectra, cross spectrum and phase-cal bit-identical to the
-    same run on device-resident input.  4-channel Mk5B with phase-cal over 8
-    chunks, and a 2-source cross-pol pair over 4 chunks."""
+    same run on device-resident input (the chunking, and so the summation
+    order, is the same).  4-channel Mk5B with phase-cal over 8 chunks, and a
+    2-source cross-pol pair over 4 chunks."""
     import os
     say("host segmented input", seg_mb)
     old = os.environ.pop("SWSPEC_H2D_SEG_MB", None)
