@@ -287,11 +287,24 @@ def batch_cfg4(torch, replays=3, seed=4):
         srv.close()
         return dt, tot
 
-    graph_run(1)                         # warm (plans, pools)
-    t1, total = graph_run(1)
-    tn, _ = graph_run(1 + replays)
-    dt_graph = max(tn - t1, 1e-9) / replays   # per replay; capture cost cancels
-    capture_s = max(t1 - dt_graph, 0.0)
+    import swspec_amd
+    sizes = sorted(set(s["fft_points"] for s in streams))
+
+    def graph_rate():
+        graph_run(1)                         # warm (plans, pools)
+        t1, tot = graph_run(1)
+        tn, _ = graph_run(1 + replays)
+        dtg = max(tn - t1, 1e-9) / replays   # per replay; capture cost cancels
+        return dtg, max(t1 - dtg, 0.0), tot
+
+    # the planner's own plans, then the autotuner's choice per size
+    swspec_amd.autotune_clear()
+    dt_graph_planner, _, total = graph_rate()
+    tuned = {}
+    for n in sizes:
+        r = swspec_amd.autotune(n, window="Cosine2", nfft=max(4, min(256, (1 << 23) // n)))
+        tuned[str(n)] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+    dt_graph, capture_s, total = graph_rate()
     # plain: one stream at a time
     srv = Server(max_active=1)
     torch.cuda.synchronize()
@@ -301,14 +314,17 @@ def batch_cfg4(torch, replays=3, seed=4):
         srv.run_batch(use_graph=False)
     dt_plain = time.perf_counter() - t0
     srv.close()
+    swspec_amd.autotune_clear()
     # graph timing includes one capture + instantiate; report per replay from the difference
     return {"streams": len(streams), "samples": total, "graph_replays": replays,
             "batched_gsps": round(total / dt_graph / 1e9, 3),
+            "batched_gsps_planner_plans": round(total / dt_graph_planner / 1e9, 3),
+            "autotune": tuned,
             "plain_gsps": round(total / dt_plain / 1e9, 3),
             "batched_s_per_replay": round(dt_graph, 4), "graph_capture_instantiate_s": round(capture_s, 3),
             "plain_wall_s": round(dt_plain, 3),
             "speedup_vs_plain": round(dt_plain / dt_graph, 2),
-            "sizes": sorted(set(s["fft_points"] for s in streams))}
+            "sizes": sizes}
 
 
 def serve_cfg5(torch, seconds, rate=200.0, seed=5, plain_seconds=None):
@@ -402,6 +418,7 @@ def main():
     ap.add_argument("--only-xpol", action="store_true", help="run only the 2^20-point cross-pol leg")
     ap.add_argument("--no-host", action="store_true", help="skip the host-Buffer (PCIe-inclusive) leg")
     ap.add_argument("--only-host", action="store_true", help="run only the host-Buffer (PCIe-inclusive) leg")
+    ap.add_argument("--only-batch", action="store_true", help="run only the 64-stream batch leg (with autotuning)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the workload is one GPU's share; "
@@ -428,6 +445,9 @@ def main():
 
     if args.only_xpol:
         print(json.dumps({"xpol_2p20": xpol_cfg3_leg(torch, swspec_amd)}), flush=True)
+        return
+    if args.only_batch:
+        print(json.dumps({"batch_64_streams": batch_cfg4(torch)}), flush=True)
         return
 
     log("generating input on the device ...")

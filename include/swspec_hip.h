@@ -334,6 +334,22 @@ int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d_raw, int c
  * stages, MFMA radix-64 stage on, first pass-1 radix. */
 int swspec_hip_plan_info(int64_t fft_points, uint32_t flags, int32_t* info, int ninfo);
 
+/* Autotuner (no counterpart in the reference, which has one FFT plan from
+ * IPP).  Measures every distinct plan the planner's choices give for
+ * fft_points -- column / row points per thread 16 or 32, composite radices
+ * per pass, a cap on the columns per workgroup -- over nfft FFTs of seeded
+ * single-channel 2-bit input on `device` (pass 1 + pass 2, best of 3),
+ * rejects any whose spectrum differs from the planner's own plan by more
+ * than 1e-5, and records the fastest: every later plan for that length
+ * (cores, server jobs, plan_info) uses it.  knobs_out[4] = column points per
+ * thread, column-count cap (0: none), composite mask, row points per thread;
+ * times in ms for the chosen and for the planner's own plan; *ncand = plans
+ * measured.  Allocates device memory while it runs. */
+int swspec_hip_autotune(int device, int64_t fft_points, int32_t window, int32_t nfft, int32_t* knobs_out,
+                        double* ms_best, double* ms_default, int32_t* ncand);
+/* Forget every recorded choice (plans revert to the planner's own). */
+void swspec_hip_autotune_clear(void);
+
 /*
  * ---- phase-cal extractor classes --------------------------------------------
  * The reference's standalone extractor family (src/IA-32/PhaseCal/PCal.h:51-190,

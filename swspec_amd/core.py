@@ -312,3 +312,23 @@ def plan_info(fft_points, flags=0):
     keys = ["N1", "N2", "B", "p1_threads", "p2_threads", "npairs", "p1_maxpts", "p2_maxpts",
             "p1_cx", "p2_cx", "p1_stages", "p2_stages", "mfma64", "p1_radix0"]
     return dict(zip(keys, list(info)))
+
+
+def autotune(fft_points, window="Cosine2", nfft=8, device=0):
+    """Measure the planner's distinct plans for fft_points on the device and
+    keep the fastest for every later plan of that length (cores, server jobs,
+    plan_info).  Returns the chosen knobs and the times (ms per pass 1 +
+    pass 2 over nfft FFTs) of the chosen and of the planner's own plan."""
+    from ._lib import WINDOWS
+    w = WINDOWS[window] if isinstance(window, str) else int(window)
+    k = (C.c_int32 * 4)()
+    best, dflt, n = C.c_double(0), C.c_double(0), C.c_int32(0)
+    check(load().swspec_hip_autotune(device, int(fft_points), w, int(nfft), k, C.byref(best), C.byref(dflt),
+                                     C.byref(n)), None, "swspec_hip_autotune")
+    return {"p1_maxpts": k[0], "p1_bmax": k[1], "cx_mask": k[2], "p2_maxpts": k[3], "ms_best": best.value,
+            "ms_default": dflt.value, "candidates": n.value}
+
+
+def autotune_clear():
+    """Forget every autotuned choice."""
+    load().swspec_hip_autotune_clear()

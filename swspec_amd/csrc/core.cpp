@@ -1099,3 +1099,158 @@ extern "C" int swspec_hip_plan_info(int64_t fft_points, uint32_t flags, int32_t*
     for (int i = 0; i < ninfo && i < 14; i++) info[i] = v[i];
     return SWSPEC_OK;
 }
+
+// ---------------------------------------------------------------------------
+// autotuner: measured plan choice per FFT length
+// ---------------------------------------------------------------------------
+
+// Every distinct plan the planner's knobs give for fft_points (column / row
+// points per thread 16 or 32, composite radices per pass, a cap on the
+// columns per workgroup) runs pass 1 + pass 2 over nfft FFTs of seeded
+// single-channel 2-bit input on `device`; a candidate whose spectrum differs
+// from the planner's own plan by more than 1e-5 (normwise) is rejected.  The
+// fastest (minimum of 3 timed repetitions) is recorded for plan_create.
+extern "C" int swspec_hip_autotune(int device, int64_t fft_points, int32_t window, int32_t nfft, int32_t* knobs_out,
+                                   double* ms_best, double* ms_default, int32_t* ncand)
+{
+    if (fft_points < 16 || nfft < 1 || nfft > 1024 || window < 0 || window > 5)
+        return fail(nullptr, SWSPEC_EINVAL, "autotune arguments");
+    if (hipSetDevice(device) != hipSuccess) return fail(nullptr, SWSPEC_EDEVICE, "hipSetDevice failed");
+    std::vector<PlanKnobs> cands(1);   // [0]: the planner's own choice (explicit: no tuned entry)
+    for (int p1 : {16, 32})
+        for (int cx = 0; cx < 4; cx++)
+            for (int bm : {0, 8, 4, 2})
+                for (int p2 : {16, 32}) {
+                    PlanKnobs k;
+                    k.p1_maxpts = p1;
+                    k.cx_mask = cx;
+                    k.p1_bmax = bm;
+                    k.p2_maxpts = p2;
+                    cands.push_back(k);
+                }
+    auto sig = [](const Plan& p) {
+        std::string s = std::to_string(p.N1) + "/" + std::to_string(p.B) + "/" + std::to_string(p.p1_threads) + "/" +
+                        std::to_string(p.p1_maxpts) + "/" + std::to_string(p.p2_maxpts) + "/" +
+                        std::to_string(p.p2_threads) + ":";
+        for (int i = 0; i < p.sp1.nst; i++) s += std::to_string(p.sp1.radix[i]) + ",";
+        s += ":";
+        for (int i = 0; i < p.sp2.nst; i++) s += std::to_string(p.sp2.radix[i]) + ",";
+        return s;
+    };
+    std::vector<PlanKnobs> uniq;
+    std::vector<std::string> seen;
+    for (size_t i = 0; i < cands.size(); i++) {
+        Plan g;
+        if (!plan_geometry(g, fft_points, false, &cands[i]).empty()) {
+            if (i == 0) return fail(nullptr, SWSPEC_EUNSUPPORTED, "no plan for this FFT length");
+            continue;
+        }
+        const std::string sg = sig(g);
+        if (std::find(seen.begin(), seen.end(), sg) != seen.end()) continue;
+        seen.push_back(sg);
+        uniq.push_back(cands[i]);
+    }
+    const long long N = fft_points, M = N / 2;
+    const size_t rawfft = (size_t)N / 4;   // 4 two-bit samples per byte
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    uint8_t* d_raw = nullptr;
+    float2* d_Y = nullptr;
+    float* d_acc = nullptr;
+    float* d_out = nullptr;
+    hipError_t he = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    if (he == hipSuccess) he = hipMalloc(&d_raw, rawfft * (size_t)nfft);
+    if (he == hipSuccess) he = hipMalloc(&d_Y, (size_t)nfft * (size_t)M * sizeof(float2));
+    if (he == hipSuccess) he = hipMalloc(&d_out, (size_t)(M + 1) * sizeof(float));
+    if (he == hipSuccess) {   // seeded bytes (xorshift), copied once
+        std::vector<uint8_t> h(rawfft * (size_t)nfft);
+        uint64_t x = 0x9E3779B97F4A7C15ull ^ (uint64_t)N;
+        for (auto& b : h) { x ^= x << 13; x ^= x >> 7; x ^= x << 17; b = (uint8_t)(x >> 32); }
+        he = hipMemcpy(d_raw, h.data(), h.size(), hipMemcpyHostToDevice);
+    }
+    std::vector<float> ref, got((size_t)M + 1);
+    double best = 1e300, tdef = 0.0;
+    int ibest = -1, tried = 0;
+    size_t acc_cap = 0;
+    for (size_t i = 0; he == hipSuccess && i < uniq.size(); i++) {
+        Plan p;
+        if (!plan_create(p, N, window, st, false, &uniq[i]).empty()) { plan_destroy(p); continue; }
+        const size_t accb = (size_t)p.smax * 2 * (size_t)(M + 1) * sizeof(float);
+        if (accb > acc_cap) {
+            if (d_acc) (void)hipFree(d_acc);
+            d_acc = nullptr;
+            he = hipMalloc(&d_acc, accb);
+            if (he != hipSuccess) { plan_destroy(p); break; }
+            acc_cap = accb;
+        }
+        JobDev J;
+        J.raw = d_raw;
+        J.fft_stride = (long long)rawfft;
+        J.raw_len = (long long)(rawfft * (size_t)nfft);
+        J.up.kind = kUnpackSingleChannel;
+        J.Y = d_Y;
+        J.acc = d_acc;
+        J.nf = nfft;
+        bool ok = launch_zero(d_acc, accb, st) == hipSuccess;
+        float* accs[1] = {d_acc};
+        float* outs[1] = {d_out};
+        // one checked run, then 3 timed repetitions (accumulators cleared by finalize)
+        ok = ok && launch_pass1(p, &J, 1, nfft, st) == hipSuccess &&
+             launch_pass2(p, &J, 1, nullptr, nfft, 1, p.smax, st) == hipSuccess &&
+             launch_finalize_jobs(p, accs, outs, 1, 1.0, p.smax, false, true, st) == hipSuccess &&
+             hipMemcpyAsync(got.data(), d_out, got.size() * sizeof(float), hipMemcpyDeviceToHost, st) == hipSuccess &&
+             hipStreamSynchronize(st) == hipSuccess;
+        if (ok && i == 0) ref = got;
+        if (ok && i > 0) {
+            double num = 0.0, den = 0.0;
+            for (size_t k = 0; k < got.size(); k++) {
+                const double d = (double)got[k] - (double)ref[k];
+                num += d * d;
+                den += (double)ref[k] * ref[k];
+            }
+            ok = den > 0.0 && std::sqrt(num / den) <= 1e-5;
+        }
+        double tmin = 1e300;
+        for (int r = 0; ok && r < 3; r++) {
+            float ms = 0.f;
+            ok = hipEventRecord(e0, st) == hipSuccess && launch_pass1(p, &J, 1, nfft, st) == hipSuccess &&
+                 launch_pass2(p, &J, 1, nullptr, nfft, 1, p.smax, st) == hipSuccess &&
+                 hipEventRecord(e1, st) == hipSuccess &&
+                 launch_finalize_jobs(p, accs, outs, 1, 1.0, p.smax, false, true, st) == hipSuccess &&
+                 hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            if (ok) tmin = std::min(tmin, (double)ms);
+        }
+        (void)hipStreamSynchronize(st);
+        plan_destroy(p);
+        if (!ok) {
+            if (i == 0) { he = hipErrorUnknown; break; }
+            (void)hipGetLastError();
+            continue;
+        }
+        tried++;
+        if (i == 0) tdef = tmin;
+        if (tmin < best) { best = tmin; ibest = (int)i; }
+    }
+    for (void* q : {(void*)d_raw, (void*)d_Y, (void*)d_acc, (void*)d_out})
+        if (q) (void)hipFree(q);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (he != hipSuccess || ibest < 0) return fail(nullptr, SWSPEC_EDEVICE, "autotune: the planner's own plan failed");
+    const PlanKnobs k = uniq[(size_t)ibest];
+    plan_set_tuned(N, k);
+    if (knobs_out) {
+        knobs_out[0] = k.p1_maxpts;
+        knobs_out[1] = k.p1_bmax;
+        knobs_out[2] = k.cx_mask;
+        knobs_out[3] = k.p2_maxpts;
+    }
+    if (ms_best) *ms_best = best;
+    if (ms_default) *ms_default = tdef;
+    if (ncand) *ncand = tried;
+    return SWSPEC_OK;
+}
+
+extern "C" void swspec_hip_autotune_clear(void) { plan_clear_tuned(); }

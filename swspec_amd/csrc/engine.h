@@ -125,12 +125,26 @@ struct Plan {
 
 // Build a plan (host tables in FP64, rounded to FP32 once).  Returns an
 // error string (empty on success).
+// Planner choices the autotuner varies (0 / -1: the planner's own choice).
+struct PlanKnobs {
+    int p1_maxpts = 0;   // column pass points per thread: 16 or 32
+    int p1_bmax = 0;     // upper bound on the columns per workgroup
+    int cx_mask = -1;    // composite radices: bit 0 column pass, bit 1 row pass
+    int p2_maxpts = 0;   // row pass points per thread: 16 or 32
+};
+// Measured choices per FFT length (swspec_hip_autotune); plan_geometry uses
+// them when no knobs are passed.  Process-wide, thread-safe.
+void plan_set_tuned(long long N, const PlanKnobs& k);
+bool plan_get_tuned(long long N, PlanKnobs& k);
+void plan_clear_tuned();
+
 // mfma64: open the column FFT with a radix-64 stage on the matrix cores
 // (needs a column length that is a multiple of 64).
-std::string plan_create(Plan& p, long long N, int window, hipStream_t stream, bool mfma64 = false);
+std::string plan_create(Plan& p, long long N, int window, hipStream_t stream, bool mfma64 = false,
+                        const PlanKnobs* knobs = nullptr);
 void plan_destroy(Plan& p);
 // Factorisation only (no device work): used by tests and the tuner.
-std::string plan_geometry(Plan& p, long long N, bool mfma64 = false);
+std::string plan_geometry(Plan& p, long long N, bool mfma64 = false, const PlanKnobs* knobs = nullptr);
 
 // One job as the kernels see it.
 struct JobDev {

@@ -22,6 +22,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 namespace swspec {
@@ -2135,8 +2137,34 @@ static int subplan_threads(const SubPlan& sp, int maxpts)
     return radices_threads(sp.L, sp.radix, sp.nst, maxpts);
 }
 
-std::string plan_geometry(Plan& p, long long N, bool mfma64)
+static std::mutex g_tuned_mu;
+static std::map<long long, PlanKnobs> g_tuned;
+
+void plan_set_tuned(long long N, const PlanKnobs& k)
 {
+    std::lock_guard<std::mutex> l(g_tuned_mu);
+    g_tuned[N] = k;
+}
+
+bool plan_get_tuned(long long N, PlanKnobs& k)
+{
+    std::lock_guard<std::mutex> l(g_tuned_mu);
+    auto it = g_tuned.find(N);
+    if (it == g_tuned.end()) return false;
+    k = it->second;
+    return true;
+}
+
+void plan_clear_tuned()
+{
+    std::lock_guard<std::mutex> l(g_tuned_mu);
+    g_tuned.clear();
+}
+
+std::string plan_geometry(Plan& p, long long N, bool mfma64, const PlanKnobs* knobs)
+{
+    PlanKnobs tuned;
+    if (!knobs && plan_get_tuned(N, tuned)) knobs = &tuned;
     if (N < 16 || (N % 2) != 0) return "FFT length must be even and >= 16";
     const long long M = N / 2;
     if (!smooth235(M)) return "FFT length must have prime factors 2, 3, 5 only";
@@ -2174,6 +2202,7 @@ std::string plan_geometry(Plan& p, long long N, bool mfma64)
     if (p.N2 > 16384) return "FFT length too large for two passes";
     // composite radices per pass: SWSPEC_COMPOSITE bit 0 = pass 1, bit 1 = pass 2
     int cxmask = 3;
+    if (knobs && knobs->cx_mask >= 0) cxmask = knobs->cx_mask & 3;
     if (const char* e = getenv("SWSPEC_COMPOSITE")) cxmask = atoi(e) & 3;
     // sub-plan for (maxpts, composite); composite only where it saves a stage
     static const bool p1_order = [] {
@@ -2234,7 +2263,15 @@ std::string plan_geometry(Plan& p, long long N, bool mfma64)
             p.p1_cx = cx32;
         }
     }
-    // tuning overrides: SWSPEC_P1_MAXPTS (16|32), SWSPEC_P1_BMAX (upper bound on B)
+    // tuning overrides: knobs, then SWSPEC_P1_MAXPTS (16|32), SWSPEC_P1_BMAX (upper bound on B)
+    if (knobs && (knobs->p1_maxpts == 16 || knobs->p1_maxpts == 32)) {
+        const int mp = knobs->p1_maxpts;
+        p.p1_maxpts = mp;
+        p.sp1 = mp == 16 ? c16 : c32;
+        p.p1_cx = mp == 16 ? cx16 : cx32;
+        T1 = subplan_threads(p.sp1, mp);
+        B = pick_B(T1);
+    }
     if (const char* e = getenv("SWSPEC_P1_MAXPTS")) {
         const int mp = atoi(e);
         if (mp == 16 || mp == 32) {
@@ -2255,6 +2292,8 @@ std::string plan_geometry(Plan& p, long long N, bool mfma64)
             B = pick_B(T1);
         }
     }
+    if (knobs && knobs->p1_bmax > 0)
+        while (B > 1 && B > knobs->p1_bmax) B >>= 1;
     if (const char* e = getenv("SWSPEC_P1_BMAX")) {
         const int bm = atoi(e);
         while (B > 1 && B > bm) B >>= 1;
@@ -2345,6 +2384,7 @@ std::string plan_geometry(Plan& p, long long N, bool mfma64)
     if (!subplan(p.N2, 16, cxmask & 2, r16, rx16) || !subplan(p.N2, 32, cxmask & 2, r32, rx32))
         return "radix decomposition failed";
     p.p2_maxpts = 16;
+    if (knobs && (knobs->p2_maxpts == 16 || knobs->p2_maxpts == 32)) p.p2_maxpts = knobs->p2_maxpts;
     if (const char* e = getenv("SWSPEC_P2_MAXPTS")) {
         const int mp = atoi(e);
         if (mp == 16 || mp == 32) p.p2_maxpts = mp;
@@ -2469,9 +2509,9 @@ static std::vector<uint4> mfma64_table()
     return t;
 }
 
-std::string plan_create(Plan& p, long long N, int window, hipStream_t stream, bool mfma64)
+std::string plan_create(Plan& p, long long N, int window, hipStream_t stream, bool mfma64, const PlanKnobs* knobs)
 {
-    std::string err = plan_geometry(p, N, mfma64);
+    std::string err = plan_geometry(p, N, mfma64, knobs);
     if (!err.empty()) return err;
     if (p.mfma64 && upload(&p.d_mft, mfma64_table(), stream) != hipSuccess)
         return "device allocation failed (MFMA stage table)";
