@@ -269,6 +269,37 @@ def host_cfg2_leg(torch, swspec_amd, raw, s, reps=4):
     return res
 
 
+def framed_cfg2_leg(torch, swspec_amd, raw, s, reps=4):
+    """The headline job fed as Mk5B frames as stored on disk (16-byte header
+    + 10000-byte payload, SWSPEC_FLAG_MK5B_FRAMED): headers skipped on the
+    device in the decode and the phase-cal fold, device-resident input."""
+    from swspec_amd import TaskCoreHIP
+    from swspec_amd._lib import FLAG_MK5B_FRAMED
+    nfr = SAMPLES // 10000
+    body = raw[: nfr * 10000].view(nfr, 10000)
+    hdr = torch.zeros((nfr, 16), dtype=torch.uint8, device=body.device)
+    hdr[:, 0:4] = torch.tensor([0xED, 0xDE, 0xAD, 0xAB], dtype=torch.uint8, device=body.device)
+    frames = torch.cat([hdr, body], dim=1).reshape(-1).contiguous()
+    ssb = s.fft_ssb_points
+    d_auto = [torch.empty(ssb, device=body.device) for _ in range(NCH)]
+    d_pcal = [torch.empty(2 * s.pcal_tonebins, device=body.device) for _ in range(NCH)]
+    core = TaskCoreHIP(0)
+    core.prepare(s, channels=list(range(NCH)), flags=FLAG_MK5B_FRAMED)
+    for _ in range(2):
+        core.run_device([frames], [frames.numel()], d_auto, None, d_pcal)
+        assert core.join() == 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        core.run_device([frames], [frames.numel()], d_auto, None, d_pcal)
+        assert core.join() == 1
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    core.finalize()
+    return {"frames": nfr, "bytes": int(frames.numel()), "gsamples_per_s": round(NCH * SAMPLES / dt / 1e9, 2),
+            "ms_per_step": round(dt * 1e3, 3)}
+
+
 def batch_cfg4(torch, replays=3, seed=4):
     """64 streams of mixed FFT sizes: one hipGraph replay of the whole batch
     vs plain per-stream launches (each stream alone, step by step)."""
@@ -517,6 +548,9 @@ def main():
     assert np.isfinite(a0).all()
     core.finalize()
 
+    log("Mk5B frames as on disk (device-side header skip) ...")
+    framed = framed_cfg2_leg(torch, swspec_amd, raw, s)
+
     host = None
     if not args.no_host:
         log("host Buffers through run()/join() (PCIe included) ...")
@@ -575,6 +609,7 @@ def main():
         "pcal_extractor": pcalx,
         "xpol_2p20": xpol,
         "host_buffers_pcie_inclusive": host,
+        "mk5b_frames_on_disk_layout": framed,
     }
     if cpu:
         out["gpu_vs_cpu"] = round(value / cpu["value"], 1)

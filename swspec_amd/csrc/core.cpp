@@ -465,9 +465,10 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
         c->pg.T = T;
         c->pg.L = L;
         pcal_geometry(c->pg, cfg->fft_points);
-        // the integer-count fold reads whole words of contiguous samples: VLBA
-        // (modulated, blanked samples) and framed input take the generic fold
-        if (c->vlba || (cfg->flags & SWSPEC_FLAG_MK5B_FRAMED)) c->pg.fast = 0;
+        // the integer-count fold reads whole words of samples: VLBA (modulated,
+        // blanked samples) takes the generic fold; framed input maps each
+        // word's payload offset to its frame (launch_pcal checks alignment)
+        if (c->vlba) c->pg.fast = 0;
         std::vector<float2> rot(L);
         const double dphi = 2 * M_PI * (-cfg->pcaloffsethz / cfg->samplingfreq);   // :130-135
         for (int i = 0; i < L; i++) {

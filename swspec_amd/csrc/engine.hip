@@ -1418,6 +1418,17 @@ __global__ void __launch_bounds__(1024) k_pcal_fast(PcArgs a)
     double accr[4] = {0.0, 0.0, 0.0, 0.0}, acci[4] = {0.0, 0.0, 0.0, 0.0};
     if ((J.pcal_mask >> f) & 1ull) {
         const uint8_t* __restrict__ raw = job_fft_ptr(J, f);
+        // framed Mk5B: byte P of the FFT's payload is at mk5b_frame_addr(pay + P)
+        // of the first frame; a word (STEP 1) or 8-byte group (STEP 2) at an
+        // aligned payload offset never straddles a frame (10000 = 8 * 1250), and
+        // offsets stay below 2^32 (checked by the launcher)
+        const bool framed = J.up.framed != 0;
+        const unsigned pay = (unsigned)(J.pay0 + (long long)f * J.fft_stride);
+        auto at = [&](unsigned P) -> const uint8_t* {
+            if (!framed) return raw + P;
+            const unsigned Q = pay + P;   // payload index
+            return J.raw + (Q + 16u * (Q / 10000u + 1u));
+        };
         const int nper = (int)(a.N / L);
         const int p0 = blk * a.g.ppb;
         const int p1 = min(p0 + a.g.ppb, nper);
@@ -1439,7 +1450,7 @@ __global__ void __launch_bounds__(1024) k_pcal_fast(PcArgs a)
                     unsigned cb = 0, cn1 = 0, cn0 = 0;
 #pragma unroll 8
                     for (int p = pc; p < pe; p++) {
-                        const unsigned w = *(const uint32_t*)(raw + (long long)p * L + 4 * g + cbyte) >> shift;
+                        const unsigned w = *(const uint32_t*)at((unsigned)(p * L + 4 * g + cbyte)) >> shift;
                         const unsigned x = w >> 1;
                         cb += x & M;
                         cn1 += w & x & M;
@@ -1462,7 +1473,7 @@ __global__ void __launch_bounds__(1024) k_pcal_fast(PcArgs a)
 #pragma unroll
                     for (int u = 0; u < 4; u++) fld[u] = (b >> (6 - 2 * u)) & 3u;
                 } else {
-                    const uint32_t* w = (const uint32_t*)(raw + j0 * STEP);
+                    const uint32_t* w = (const uint32_t*)at((unsigned)(j0 * STEP));
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
                         const int bi = u * STEP + cbyte;
@@ -1890,7 +1901,12 @@ hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int n
         a.jobs[j] = jobs[j];
         aligned &= ((uintptr_t)jobs[j].raw % 4 == 0) && (jobs[j].fft_stride % 4 == 0);
         const int st = jobs[j].up.kind == kUnpackSingleChannel ? 0 : jobs[j].up.step;
-        if (jobs[j].up.kind == kUnpackVLBA || jobs[j].up.framed) step = -2;   // generic fold
+        // framed Mk5B (1 or 2 bytes per time sample): payload offsets 8-byte
+        // aligned and below 2^32 (32-bit frame arithmetic in the kernel)
+        const bool framed_ok = jobs[j].up.framed && (st == 1 || st == 2) && jobs[j].pay0 % 8 == 0 &&
+                               jobs[j].fft_stride % 8 == 0 && jobs[j].raw_len < (1LL << 32) &&
+                               jobs[j].pay0 + (long long)nfft * jobs[j].fft_stride + N * st < (1LL << 32);
+        if (jobs[j].up.kind == kUnpackVLBA || (jobs[j].up.framed && !framed_ok)) step = -2;   // generic fold
         else if (step == -1) step = st;
         else if (step != st) step = -2;
     }
