@@ -1404,7 +1404,9 @@ __global__ void __launch_bounds__(1024) k_pcal(PcArgs a)
 // and the rotator is applied once per position in FP64.  With 4*nth % T == 0
 // every group of a thread folds into the same four bins, reduced in LDS in a
 // fixed order.  STEP = bytes per time sample (multi-channel), 0 = 1-channel.
-template <int STEP, int GPT>
+// FR: framed Mk5B input (its own instantiation, so the payload path's loop
+// carries no frame arithmetic)
+template <int STEP, int GPT, bool FR = false>
 __global__ void __launch_bounds__(1024) k_pcal_fast(PcArgs a)
 {
     extern __shared__ double2 red4[];   // [nth][4]
@@ -1422,12 +1424,14 @@ __global__ void __launch_bounds__(1024) k_pcal_fast(PcArgs a)
         // of the first frame; a word (STEP 1) or 8-byte group (STEP 2) at an
         // aligned payload offset never straddles a frame (10000 = 8 * 1250), and
         // offsets stay below 2^32 (checked by the launcher)
-        const bool framed = J.up.framed != 0;
-        const unsigned pay = (unsigned)(J.pay0 + (long long)f * J.fft_stride);
-        auto at = [&](unsigned P) -> const uint8_t* {
-            if (!framed) return raw + P;
-            const unsigned Q = pay + P;   // payload index
-            return J.raw + (Q + 16u * (Q / 10000u + 1u));
+        const unsigned pay = FR ? (unsigned)(J.pay0 + (long long)f * J.fft_stride) : 0u;
+        auto at = [&](auto P) -> const uint8_t* {
+            if constexpr (!FR) {
+                return raw + P;
+            } else {
+                const unsigned Q = pay + (unsigned)P;   // payload index
+                return J.raw + (Q + 16u * (Q / 10000u + 1u));
+            }
         };
         const int nper = (int)(a.N / L);
         const int p0 = blk * a.g.ppb;
@@ -1450,7 +1454,7 @@ __global__ void __launch_bounds__(1024) k_pcal_fast(PcArgs a)
                     unsigned cb = 0, cn1 = 0, cn0 = 0;
 #pragma unroll 8
                     for (int p = pc; p < pe; p++) {
-                        const unsigned w = *(const uint32_t*)at((unsigned)(p * L + 4 * g + cbyte)) >> shift;
+                        const unsigned w = *(const uint32_t*)at((long long)p * L + 4 * g + cbyte) >> shift;
                         const unsigned x = w >> 1;
                         cb += x & M;
                         cn1 += w & x & M;
@@ -1473,7 +1477,7 @@ __global__ void __launch_bounds__(1024) k_pcal_fast(PcArgs a)
 #pragma unroll
                     for (int u = 0; u < 4; u++) fld[u] = (b >> (6 - 2 * u)) & 3u;
                 } else {
-                    const uint32_t* w = (const uint32_t*)at((unsigned)(j0 * STEP));
+                    const uint32_t* w = (const uint32_t*)at(j0 * STEP);
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
                         const int bi = u * STEP + cbyte;
@@ -1897,6 +1901,7 @@ hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int n
     a.g = g; a.N = N;
     bool aligned = true;
     int step = -1;
+    bool framed = false;
     for (int j = 0; j < njobs; j++) {
         a.jobs[j] = jobs[j];
         aligned &= ((uintptr_t)jobs[j].raw % 4 == 0) && (jobs[j].fft_stride % 4 == 0);
@@ -1906,10 +1911,14 @@ hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int n
         const bool framed_ok = jobs[j].up.framed && (st == 1 || st == 2) && jobs[j].pay0 % 8 == 0 &&
                                jobs[j].fft_stride % 8 == 0 && jobs[j].raw_len < (1LL << 32) &&
                                jobs[j].pay0 + (long long)nfft * jobs[j].fft_stride + N * st < (1LL << 32);
+        framed |= jobs[j].up.framed != 0;
         if (jobs[j].up.kind == kUnpackVLBA || (jobs[j].up.framed && !framed_ok)) step = -2;   // generic fold
         else if (step == -1) step = st;
         else if (step != st) step = -2;
     }
+    int nfr = 0;
+    for (int j = 0; j < njobs; j++) nfr += jobs[j].up.framed != 0;
+    if (nfr != 0 && nfr != njobs) step = -2;   // framed and payload jobs mixed: generic fold
     const bool fast = g.fast && aligned && (step == 0 || step == 1 || step == 2 || step == 4);
     if (fast) {
         // one job (channel) per blockIdx.z: folding the 4 channels of one source
@@ -1918,17 +1927,22 @@ hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int n
         // work per thread; the re-reads hit in cache)
         const dim3 grid(g.nblk_fast, nfft, njobs), block(g.threads_fast);
         const size_t lds = (size_t)g.threads_fast * 4 * sizeof(double2);
-#define SWSPEC_PCAL_FAST(ST)                                                                            \
+#define SWSPEC_PCAL_FAST(ST, FRM)                                                                       \
     switch (g.gpt) {                                                                                  \
-    case 1: hipLaunchKernelGGL((k_pcal_fast<ST, 1>), grid, block, lds, s, a); break;                   \
-    case 2: hipLaunchKernelGGL((k_pcal_fast<ST, 2>), grid, block, lds, s, a); break;                   \
-    default: hipLaunchKernelGGL((k_pcal_fast<ST, 4>), grid, block, lds, s, a); break;                  \
+    case 1: hipLaunchKernelGGL((k_pcal_fast<ST, 1, FRM>), grid, block, lds, s, a); break;              \
+    case 2: hipLaunchKernelGGL((k_pcal_fast<ST, 2, FRM>), grid, block, lds, s, a); break;              \
+    default: hipLaunchKernelGGL((k_pcal_fast<ST, 4, FRM>), grid, block, lds, s, a); break;             \
     }
-        switch (step) {
-        case 0: SWSPEC_PCAL_FAST(0) break;
-        case 1: SWSPEC_PCAL_FAST(1) break;
-        case 2: SWSPEC_PCAL_FAST(2) break;
-        default: SWSPEC_PCAL_FAST(4) break;
+        if (framed) {   // every job framed (one core's flags), step 1 or 2
+            if (step == 1) SWSPEC_PCAL_FAST(1, true)
+            else SWSPEC_PCAL_FAST(2, true)
+        } else {
+            switch (step) {
+            case 0: SWSPEC_PCAL_FAST(0, false) break;
+            case 1: SWSPEC_PCAL_FAST(1, false) break;
+            case 2: SWSPEC_PCAL_FAST(2, false) break;
+            default: SWSPEC_PCAL_FAST(4, false) break;
+            }
         }
 #undef SWSPEC_PCAL_FAST
     } else {
