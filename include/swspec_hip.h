@@ -315,6 +315,10 @@ int swspec_hip_server_sync(swspec_hip_server* s);
  * replays); otherwise step-by-step launches, once. */
 int swspec_hip_server_run_batch(swspec_hip_server* s, int use_graph, int replays);
 int swspec_hip_server_destroy(swspec_hip_server* s);
+/* Counters: out[0..5] = steps enqueued, cross-stream waits (a job's batch
+ * moved to the other stream and waited for its previous one), jobs admitted,
+ * jobs completed, jobs active, device bytes of resident job state. */
+int swspec_hip_server_stats(const swspec_hip_server* s, int64_t* out, int n);
 const char* swspec_hip_server_last_error(const swspec_hip_server* s);
 
 /* Pinned host memory for Buffer rings (hipHostMalloc) and its release. */

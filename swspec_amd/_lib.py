@@ -28,7 +28,7 @@ EXPORTS = [
     "swspec_hip_unpack_device", "swspec_hip_plan_info", "swspec_hip_autotune", "swspec_hip_autotune_clear", "swspec_hip_set_profiling", "swspec_hip_kernel_stats",
     "swspec_hip_server_create", "swspec_hip_server_submit", "swspec_hip_server_step", "swspec_hip_server_poll",
     "swspec_hip_server_sync", "swspec_hip_server_run_batch", "swspec_hip_server_destroy",
-    "swspec_hip_server_last_error",
+    "swspec_hip_server_last_error", "swspec_hip_server_stats",
     "swspec_hip_pcal_create", "swspec_hip_pcal_destroy", "swspec_hip_pcal_info", "swspec_hip_pcal_clear",
     "swspec_hip_pcal_adjust_offset", "swspec_hip_pcal_extract", "swspec_hip_pcal_final",
     "swspec_hip_pcal_copied_tones", "swspec_hip_pcal_seconds", "swspec_hip_pcal_delay",
@@ -147,6 +147,7 @@ def load():
     L.swspec_hip_server_destroy.argtypes = [P]
     L.swspec_hip_server_last_error.argtypes = [P]
     L.swspec_hip_server_last_error.restype = C.c_char_p
+    L.swspec_hip_server_stats.argtypes = [P, C.POINTER(C.c_int64), C.c_int]
     L.swspec_hip_pcal_create.argtypes = [C.c_double, C.c_double, C.c_int, C.c_size_t, C.c_int, C.c_int,
                                          C.POINTER(P)]
     L.swspec_hip_pcal_destroy.argtypes = [P]

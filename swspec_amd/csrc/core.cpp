@@ -598,14 +598,21 @@ static std::vector<SpecWork> schedule(const swspec_hip_core* c, size_t len)
     return out;
 }
 
+// Bytes the doMaths loop walks: framed input counts the payload bytes of its
+// whole frames (headers and a trailing partial frame excluded).
+static size_t payload_len(const swspec_hip_core* c, size_t len)
+{
+    if (c->cfg.flags & SWSPEC_FLAG_MK5B_FRAMED) return (len / (size_t)kMk5bFrame) * (size_t)kMk5bPayload;
+    return len;
+}
+
 static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len, float* const* d_auto,
                       float* d_xpol, float* const* d_pcal, double* d_tones)
 {
     const Plan& p = c->plan;
     const swspec_hip_cfg& g = c->cfg;
     const bool framed = (g.flags & SWSPEC_FLAG_MK5B_FRAMED) != 0;
-    // framed input: the doMaths loop runs over the payload bytes of whole frames
-    const size_t plen = framed ? (len / (size_t)kMk5bFrame) * (size_t)kMk5bPayload : len;
+    const size_t plen = payload_len(c, len);
     std::vector<SpecWork> sched = schedule(c, plen);
     const size_t M1 = (size_t)p.M + 1;
     const int ssb = g.fft_ssb_points;
@@ -767,9 +774,7 @@ extern "C" int swspec_hip_run_device(swspec_hip_core* c, const void* const* d_in
     if (c->pending) return fail(c, SWSPEC_ESTATE, "run while a run is pending");
     if (c->nx && (!d_out_xpol || !d_out_xpol[0])) return fail(c, SWSPEC_EINVAL, "cross-power output missing");
     const size_t len = min_len(c, in_len);
-    int nfft = 0;
-    size_t nsp = schedule(c, len).size();
-    (void)nfft;
+    const size_t nsp = schedule(c, payload_len(c, len)).size();
     int rc = ensure_tones(c, nsp);
     if (rc) return rc;
     const uint8_t* ins[2] = {(const uint8_t*)d_in[0], c->nsrc > 1 ? (const uint8_t*)d_in[1] : nullptr};
@@ -798,7 +803,7 @@ extern "C" int swspec_hip_run(swspec_hip_core* c, const void* const* in, const s
     if (c->pending) return fail(c, SWSPEC_ESTATE, "run while a run is pending");
     if (c->nx && (!out_xpol || !out_xpol[0])) return fail(c, SWSPEC_EINVAL, "cross-power output missing");
     const size_t len = min_len(c, in_len);
-    const size_t nsp = schedule(c, len).size();
+    const size_t nsp = schedule(c, payload_len(c, len)).size();   // what run_common emits
     const size_t nalloc = std::max<size_t>(nsp, 1);
     const size_t ssb = (size_t)c->cfg.fft_ssb_points;
     const size_t T = c->cfg.extract_pcal ? (size_t)c->pg.T : 0;

@@ -1504,12 +1504,15 @@ __global__ void __launch_bounds__(1024) k_pcal_fast(PcArgs a)
 #pragma unroll
     for (int u = 0; u < 4; u++) red4[4 * tid + u] = make_double2(accr[u], acci[u]);
     __syncthreads();
-    if (tid < T) {
-        // bin t collects threads h with (4h mod T) == t - t%4, column t%4
-        const int q = T >> 2, u = tid & 3;
+    // bin t collects threads h with (4h mod T) == t - t%4, column t%4; the
+    // thread count is a multiple of T/4 but may be below T (e.g. L = T = 256
+    // runs 64 threads), so each thread writes bins t = tid, tid + nth, ...
+    const int q = T >> 2;
+    for (int t = tid; t < T; t += nth) {
+        const int u = t & 3;
         double sr = 0.0, si = 0.0;
-        for (int h = tid >> 2; h < nth; h += q) { sr += red4[4 * h + u].x; si += red4[4 * h + u].y; }
-        double* part = J.pcal_part + 2 * (((long long)f * a.g.nblk_max + blk) * T + tid);
+        for (int h = t >> 2; h < nth; h += q) { sr += red4[4 * h + u].x; si += red4[4 * h + u].y; }
+        double* part = J.pcal_part + 2 * (((long long)f * a.g.nblk_max + blk) * T + t);
         part[0] = sr;
         part[1] = si;
     }

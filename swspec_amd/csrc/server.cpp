@@ -45,6 +45,7 @@ struct SJob {
     double* pcal_acc = nullptr;
     double* pcal_part = nullptr;
     int ntones = 0, tone_step = 0;
+    int last_stream = -1;      // stream of the job's last batch (-1: none yet)
     size_t mem = 0;
     std::vector<void*> bufs;
     std::vector<size_t> buf_sizes;
@@ -54,6 +55,7 @@ struct Pending {
     hipEvent_t ev[kStreams];
     bool used[kStreams];
     std::vector<SJob*> finished;
+    std::vector<hipEvent_t> marks;   // start-of-step marks, reusable once the step is done
 };
 
 }  // namespace
@@ -62,6 +64,7 @@ struct swspec_hip_server {
     int device = 0;
     hipStream_t st[kStreams] = {nullptr, nullptr};
     std::map<std::pair<long long, int>, Plan*> plans;
+    std::map<Plan*, int> plan_seq;      // creation order: the order groups launch in each step
     std::map<std::tuple<long long, long long, int, int>, float2*> rots;
     std::vector<SJob*> active;
     std::deque<Pending> pending;
@@ -72,7 +75,10 @@ struct swspec_hip_server {
     int64_t next_id = 1;
     std::map<size_t, std::vector<void*>> pool;
     std::vector<hipEvent_t> free_events;
+    std::vector<hipEvent_t> capture_events;   // recorded during a capture; freed after it
     std::string err;
+    // counters (swspec_hip_server_stats)
+    int64_t n_steps = 0, n_cross_waits = 0, n_admitted = 0, n_completed = 0;
 };
 
 static int sfail(swspec_hip_server* s, int code, const std::string& m)
@@ -196,6 +202,8 @@ extern "C" int swspec_hip_server_submit(swspec_hip_server* s, const swspec_hip_j
         std::string e = plan_create(*p, jd->fft_points, jd->window, s->st[0], mfma);
         if (!e.empty()) { delete p; delete j; return sfail(s, SWSPEC_EUNSUPPORTED, e); }
         it = s->plans.emplace(key, p).first;
+        const int seq = (int)s->plan_seq.size();
+        s->plan_seq[p] = seq;
     }
     j->plan = it->second;
     const long long M = j->plan->M;
@@ -257,6 +265,7 @@ extern "C" int swspec_hip_server_submit(swspec_hip_server* s, const swspec_hip_j
     s->used += (int64_t)mem;
     j->id = s->next_id++;
     s->active.push_back(j);
+    s->n_admitted++;
     if (id) *id = j->id;
     return SWSPEC_OK;
 }
@@ -265,24 +274,47 @@ static void retire(swspec_hip_server* s, Pending& p)
 {
     for (SJob* j : p.finished) {
         s->done_ids.push_back(j->id);
+        s->n_completed++;
         job_release(s, j);
     }
     for (int i = 0; i < kStreams; i++)
         if (p.used[i]) s->free_events.push_back(p.ev[i]);
+    for (hipEvent_t e : p.marks) s->free_events.push_back(e);
 }
 
 // Enqueue one step.  capture: no events, no waits (stream capture).
 static int step_enqueue(swspec_hip_server* s, bool capture, std::vector<SJob*>* finished_out)
 {
-    // group by plan
-    std::map<Plan*, std::vector<SJob*>> groups;
-    for (SJob* j : s->active) groups[j->plan].push_back(j);
+    // group by plan, groups in plan creation order (deterministic across runs)
+    std::map<int, std::pair<Plan*, std::vector<SJob*>>> groups;
+    for (SJob* j : s->active) {
+        auto& g = groups[s->plan_seq[j->plan]];
+        g.first = j->plan;
+        g.second.push_back(j);
+    }
+    s->n_steps++;
     int launch = 0;
     bool used[kStreams] = {false, false};
     std::vector<SJob*> finished;
+    // A job's batch may land on another stream than its previous batch (the
+    // batch order shifts as jobs arrive and finish).  Its accumulator and
+    // phase-cal read-modify-writes must stay ordered, so each stream's tail at
+    // the start of this step is marked, and a batch that takes a job from
+    // stream o waits for that mark (it covers every earlier launch on o).
+    hipEvent_t mark[kStreams] = {nullptr, nullptr};
+    std::vector<hipEvent_t> marks;
+    for (int i = 0; i < kStreams; i++) {
+        mark[i] = ev_get(s);
+        if (!mark[i]) return sfail(s, SWSPEC_EDEVICE, "hipEventCreate failed");
+        marks.push_back(mark[i]);
+        SCHK(s, hipEventRecord(mark[i], s->st[i]));
+    }
+    auto keep_marks = [&]() {
+        if (capture) s->capture_events.insert(s->capture_events.end(), marks.begin(), marks.end());
+    };
     for (auto& g : groups) {
-        Plan* p = g.first;
-        std::vector<SJob*>& js = g.second;
+        Plan* p = g.second.first;
+        std::vector<SJob*>& js = g.second.second;
         const double per = (double)p->M * sizeof(float2);
         // jobs per launch: keep >= 8 FFTs per job per chunk when the scratch allows
         int nj = (int)std::max(1.0, std::min((double)kMaxJobs, std::floor((double)kYScratch / (8.0 * per))));
@@ -295,6 +327,23 @@ static int step_enqueue(swspec_hip_server* s, bool capture, std::vector<SJob*>* 
             const int si = launch++ % kStreams;
             hipStream_t st = s->st[si];
             used[si] = true;
+            {
+                bool wait_other[kStreams] = {false, false};
+                for (int k = 0; k < n; k++) {
+                    SJob* j = js[b + k];
+                    if (j->last_stream >= 0 && j->last_stream != si) wait_other[j->last_stream] = true;
+                    j->last_stream = si;
+                }
+                for (int o = 0; o < kStreams; o++)
+                    if (wait_other[o]) {
+                        s->n_cross_waits++;
+                        hipError_t we = hipStreamWaitEvent(st, mark[o], 0);
+                        if (we != hipSuccess) {
+                            keep_marks();
+                            return sfail(s, SWSPEC_EDEVICE, std::string("hipStreamWaitEvent: ") + hipGetErrorString(we));
+                        }
+                    }
+            }
             JobDev jd[kMaxJobs];
             int maxnf = 0;
             for (int k = 0; k < n; k++) {
@@ -362,11 +411,13 @@ static int step_enqueue(swspec_hip_server* s, bool capture, std::vector<SJob*>* 
         s->active.swap(keep);
     }
     if (capture) {
+        keep_marks();
         if (finished_out) finished_out->insert(finished_out->end(), finished.begin(), finished.end());
         return SWSPEC_OK;
     }
     Pending pd;
     pd.finished = finished;
+    pd.marks = marks;
     for (int i = 0; i < kStreams; i++) {
         pd.used[i] = used[i];
         if (used[i]) {
@@ -478,10 +529,22 @@ extern "C" int swspec_hip_server_run_batch(swspec_hip_server* s, int use_graph, 
     (void)hipGraphDestroy(graph);
     s->free_events.push_back(fork);
     s->free_events.push_back(join);
+    s->free_events.insert(s->free_events.end(), s->capture_events.begin(), s->capture_events.end());
+    s->capture_events.clear();
     for (SJob* j : finished) {
         s->done_ids.push_back(j->id);
+        s->n_completed++;
         job_release(s, j);
     }
+    return SWSPEC_OK;
+}
+
+extern "C" int swspec_hip_server_stats(const swspec_hip_server* s, int64_t* out, int n)
+{
+    if (!s || !out || n < 0) return SWSPEC_EINVAL;
+    const int64_t v[6] = {s->n_steps, s->n_cross_waits, s->n_admitted, s->n_completed,
+                          (int64_t)s->active.size(), s->used};
+    for (int i = 0; i < n && i < 6; i++) out[i] = v[i];
     return SWSPEC_OK;
 }
 

@@ -92,6 +92,16 @@ class Server:
         if rc != 0:
             raise self._err("swspec_hip_server_run_batch", rc)
 
+    STATS = ("steps", "cross_stream_waits", "admitted", "completed", "active", "resident_bytes")
+
+    def stats(self):
+        """Counters of swspec_hip_server_stats as a dict (names in STATS)."""
+        v = (C.c_int64 * len(self.STATS))()
+        rc = load().swspec_hip_server_stats(self._s, v, len(self.STATS))
+        if rc != 0:
+            raise self._err("swspec_hip_server_stats", rc)
+        return dict(zip(self.STATS, list(v)))
+
     def close(self):
         if self._s:
             load().swspec_hip_server_destroy(self._s)

@@ -455,20 +455,22 @@ def test_unpack_mk5b_framed_bitexact(hip, oracle):
             assert np.array_equal(d_out.cpu().numpy(), ref), (nch, ch)
 
 
-def test_mk5b_framed_run_matches_stripped(hip, oracle):
+@pytest.mark.parametrize("nch", [4, 8])
+def test_mk5b_framed_run_matches_stripped(hip, oracle, nch):
     """A core fed the on-disk frames gives the spectra of the core fed the
     stripped payload bit for bit (same decoded samples, same FFT), and both
     match the oracle; phase-cal bit for bit too (the same fold reads each
-    payload word through its frame address)."""
+    payload word -- 4 channels -- or 8-byte group -- 8 channels -- through
+    its frame address)."""
     from swspec_amd._lib import FLAG_MK5B_FRAMED
     n, nf, fs = 320000, 3, 32e6
-    say("mk5b framed run", n, nf)
+    say("mk5b framed run", n, nf, nch)
     s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=2, WindowType="Cosine2",
-                            BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=4, SourceFormat="Mk5B",
-                            ExtractPCal=True, MaxSourceBufferMB=4096, UseFile1Channel=2)
+                            BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=nch, SourceFormat="Mk5B",
+                            ExtractPCal=True, MaxSourceBufferMB=4096, UseFile1Channel=nch - 2)
     rng = np.random.default_rng(33)
     comb = synth.pcal_comb(fs, fs / 2, amp=0.2, rng=rng)
-    payload = synth.raw_multi(7, 4, n * (nf + 2), tones_per_ch=[comb] * 4, fs=fs)
+    payload = synth.raw_multi(7, nch, n * (nf + 2), tones_per_ch=[comb] * nch, fs=fs)
     payload = payload[: (len(payload) // 10000) * 10000]
     frames = synth.mk5b_frames(payload)
     m1, a1, _, p1, c1 = run_device(s, [payload])
@@ -497,3 +499,79 @@ def test_mk5b_framed_run_matches_stripped(hip, oracle):
     # both take the integer-count fold (framed words mapped to their frames):
     # the same counts in the same order, so the phase-cal is bit-identical
     assert e <= TOL and np.array_equal(p2, p1[0])
+
+
+def test_framed_host_run_counts_payload_spectra(hip):
+    """Host-mode run on whole Mk5B frames whose frame bytes would hold one
+    FFT more than their payload: 400 frames = 4,000,000 payload bytes = 625
+    FFTs of 6400 4-channel time samples (5 spectra of 125), while 4,006,400
+    frame bytes would give 626.  The core reports and copies back exactly the
+    5 payload spectra (a guard after the host arrays stays untouched) and
+    they equal the run on the stripped payload bit for bit."""
+    from swspec_amd._lib import FLAG_MK5B_FRAMED
+    n, fs, K = 6400, 32e6, 125
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * (K + 0.25) / fs, FFToverlapFactor=1,
+                            WindowType="Hann", BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=4,
+                            SourceFormat="Mk5B", ExtractPCal=True, MaxSourceBufferMB=4096, UseFile1Channel=2)
+    assert s.core_overlapped_ffts == K and s.extract_pcal == 1
+    payload = np.random.default_rng(5).integers(0, 256, 400 * 10000, dtype=np.uint8)
+    frames = synth.mk5b_frames(payload)
+    assert len(frames) // n == 626 and len(payload) // n == 625
+    m, auto, _, pc, c1 = run_device(s, [payload])
+    c1.finalize()
+    assert m == 5
+    core = hip.TaskCoreHIP(1)
+    core.prepare(s, flags=FLAG_MK5B_FRAMED)
+    inb = hip.Buffer(len(frames), pinned=True)
+    inb.getData()[:] = frames
+    inb.setLength(len(frames))
+    ssb, T, guard = s.fft_ssb_points, s.pcal_tonebins, 4096
+    out = hip.Buffer(m * 4 * ssb + guard)
+    pcb = hip.Buffer(m * 8 * T + guard)
+    for b in (out, pcb):
+        b.getData()[:] = 0xAB
+    core.run([inb], [out], [], [pcb])
+    assert core.join() == m
+    core.finalize()
+    assert out.getLength() == m * 4 * ssb and pcb.getLength() == m * 8 * T
+    assert (out.getData()[m * 4 * ssb:] == 0xAB).all() and (pcb.getData()[m * 8 * T:] == 0xAB).all()
+    got = out.getData()[: m * 4 * ssb].view(np.float32).reshape(m, ssb)
+    assert np.array_equal(got, auto[0])
+    v = pcb.getData()[: m * 8 * T].view(np.float32)
+    assert np.array_equal((v[0::2] + 1j * v[1::2]).reshape(m, T), pc[0].astype(np.complex64))
+
+
+@pytest.mark.parametrize("bw,off,nch,fmt", [(64e6, 500e3, 1, "RawSigned"), (64e6, 500e3, 4, "Mk5B"),
+                                            (128e6, 1e6, 1, "RawSigned")])
+def test_pcal_fold_bins_beyond_thread_count(hip, oracle, bw, off, nch, fmt):
+    """Phase-cal geometries whose integer-count fold runs fewer threads than
+    tone bins: fs = 128 Ms/s with a 500 kHz offset (T = 128 bins, rotator
+    L = 256, 64 threads) and fs = 256 Ms/s with a 1 MHz offset (T = L = 256,
+    64 threads).  Every bin is written: the folded vector matches the FP64
+    fold with the FP32 rotator (<= 1e-6) and the oracle's doMaths."""
+    fs = 2 * bw
+    n, nf = 25600, 3
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * (nf + 0.25) / fs, FFToverlapFactor=1,
+                            WindowType="Hann", BandwidthHz=bw, BitsPerSample=2, SourceChannels=nch, SourceFormat=fmt,
+                            ExtractPCal=True, PCalOffsetHz=off, MaxSourceBufferMB=4096, UseFile1Channel=nch)
+    T, L = s.pcal_tonebins, s.pcal_rotatorlen
+    say("pcal T=%d L=%d %s %d ch" % (T, L, fmt, nch))
+    assert s.extract_pcal == 1 and T == int(fs / 1e6) and L == int(fs / np.gcd(int(fs), int(off)))
+    rng = np.random.default_rng(int(off) % 997 + nch)
+    comb = synth.pcal_comb(fs, bw, offset=off, amp=0.2, rng=rng)
+    if nch == 1:
+        raw = synth.raw_1ch(9, n * nf, tones=comb, fs=fs)
+    else:
+        raw = synth.raw_multi(9, nch, n * nf, tones_per_ch=[comb] * nch, fs=fs)
+    m, _, _, pc, core = run_device(s, [raw])
+    core.finalize()
+    os_ = oracle_settings(oracle, s)
+    _, _, _, p64 = oracle.domaths(os_, [raw], 64)
+    x = oracle.unpack(os_, raw, n * nf, nch - 1).astype(np.float64)
+    acc = np.zeros(2 * T)
+    for f in range(nf):
+        acc = oracle.pcal_fold64(x[f * n:(f + 1) * n], fs, off, T, L, rot32=True, acc=acc)
+    ep = relerr(pc[0][0], acc[0::2] + 1j * acc[1::2])
+    e64 = relerr(pc[0][0], p64[0][0])
+    say("  pcal %.3e vs fp64 rotator %.3e" % (ep, e64))
+    assert m == 1 and np.isfinite(pc[0][0]).all() and ep <= 1e-6 and e64 <= 1e-5

@@ -24,6 +24,23 @@ def _oracle_mean(oracle, raw, n, nffts, fmt, nch, ch, window, fs, pcal=False):
     return auto[0][0], (pc[0][0] if pcal else None), s
 
 
+def _oracle_ring_mean(oracle, ring, n, nffts, fmt, nch, ch, window, fs):
+    """A job reading a ring of R FFTs cyclically (FFT k from slot k mod R),
+    averaged over nffts FFTs: sum over slots of (uses of the slot / nffts) x
+    the slot's spectrum, each slot's spectrum from the FP64 oracle's doMaths
+    with K = 1 (the combine identity, TaskCoreIPP.cpp:124)."""
+    per = n // 4 if nch == 1 else n * (nch // 4)
+    R = len(ring) // per
+    s = oracle.derive(fft_points=n, fft_integ_seconds=n * 1.25 / fs, fft_overlap_factor=1, window=window,
+                      bandwidth_hz=fs / 2, bits_per_sample=2, source_channels=nch, source_format=fmt,
+                      use_channel_file1=ch + 1, max_source_buffer_mb=1 << 16)
+    assert s.core_overlapped_ffts == 1
+    m, auto, _, _ = oracle.domaths(s, [np.ascontiguousarray(ring[: R * per])], 64)
+    assert m == R
+    uses = np.bincount(np.arange(nffts) % R, minlength=R) / nffts
+    return (uses[:, None] * auto[0]).sum(axis=0)
+
+
 def _mkjob(kind, n, seed, nffts_data, fs):
     if kind == "raw1":
         return synth.raw_1ch(seed, n * nffts_data, tones=[(0.2, 0.21 * fs, 0.1)], fs=fs), "RawSigned", 1
@@ -115,7 +132,12 @@ def test_server_graph_replay_matches_steps(hip):
 
 
 def test_result_same_alone_and_in_mix(hip, oracle):
-    """Continuous batching: a job's result is the same alone and mixed with others."""
+    """Continuous batching: a job's result is the same alone and mixed with
+    others, and every other job of the mix matches the oracle too.  Groups
+    launch in plan-creation order on alternating streams, so when the
+    4096-point group finishes (after 2 steps) the 65536-point job's batch
+    moves to the other stream mid-run and must wait for its previous batch
+    (counted in the server's cross-stream waits)."""
     import torch
     from swspec_amd.server import Server
     fs, n, nffts = 32e6, 320000, 12
@@ -125,14 +147,15 @@ def test_result_same_alone_and_in_mix(hip, oracle):
     alone = torch.empty(n // 2 + 1, device="cuda")
     srv.submit(n, nffts, d_raw, len(raw), alone, source_format=fmt, source_channels=4, channel=0, samplingfreq=fs)
     srv.run_batch(use_graph=False)
-    # in a mix: same job admitted mid-run while other jobs of several sizes are active
+    # in a mix: same job admitted mid-run while other jobs of several sizes are
+    # active; the inputs are rings of <= 8 FFTs read cyclically
     others = []
-    for i, (m, k) in enumerate([(4096, 300), (320000, 30), (65536, 50), (320000, 7)]):
+    for i, (m, k) in enumerate([(4096, 1500), (320000, 300), (65536, 3000), (320000, 7)]):
         r, f, c = _mkjob("raw1", m, 300 + i, min(k, 8), fs)
         d = torch.from_numpy(r).cuda()
-        o = torch.empty(m // 2 + 1, device="cuda")
+        o = torch.full((m // 2 + 1,), float("nan"), device="cuda")
         srv.submit(m, k, d, len(r), o, source_format=f, source_channels=c, samplingfreq=fs, keep=d)
-        others.append(o)
+        others.append((r, m, k, o))
     srv.step()
     srv.step()
     mixed = torch.empty(n // 2 + 1, device="cuda")
@@ -140,13 +163,56 @@ def test_result_same_alone_and_in_mix(hip, oracle):
     while srv.step() > 0:
         pass
     srv.sync()
+    st = srv.stats()
+    say("server stats", st)
+    assert st["cross_stream_waits"] >= 1 and st["completed"] == st["admitted"] == 6
     ref, _, _ = _oracle_mean(oracle, raw, n, nffts, fmt, nch, 0, "Cosine2", fs)
     a, m = alone.cpu().numpy(), mixed.cpu().numpy()
     say("alone vs mix %.3e, alone vs oracle %.3e, mix vs oracle %.3e" % (relerr(m, a), relerr(a, ref), relerr(m, ref)))
     assert relerr(m, a) <= TOL and relerr(a, ref) <= TOL and relerr(m, ref) <= TOL
-    for o in others:
-        assert np.isfinite(o.cpu().numpy()).all()
+    for (r, mm, k, o) in others:
+        e = relerr(o.cpu().numpy(), _oracle_ring_mean(oracle, r, mm, k, "RawSigned", 1, 0, "Cosine2", fs))
+        say("  other job %d points x %d FFTs vs oracle %.3e" % (mm, k, e))
+        assert e <= TOL
     srv.close()
+
+
+def test_batch_64_streams_every_stream_vs_oracle(hip, oracle):
+    """The 64-stream mixed-size batch (bench.py's workload: sizes 4096 ...
+    3.2M, 10-5000 FFTs, raw 1-channel and Mk5B 4-channel, rings of 8 FFTs)
+    captured into one hipGraph and replayed twice: every stream's spectrum
+    within 1e-5 of the FP64 oracle (ring-weighted doMaths)."""
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+    from swspec_amd import workloads as W
+    from swspec_amd.server import Server
+    fs = 32e6
+    streams = W.batch_workload(seed=4, nstreams=64)
+    kinds = {(s["fft_points"], s["fmt"]) for s in streams}
+    for need in [(1 << 20, "raw1"), (1 << 20, "mk5b4"), (3200000, "raw1"), (3200000, "mk5b4")]:
+        assert need in kinds
+    pool = W.RingPool(torch, ring_ffts=8)
+    srv = Server(max_active=1024)
+    outs, total = W.submit_batch(torch, srv, streams, pool)
+    srv.run_batch(use_graph=True, replays=2)
+    got = [o.cpu().numpy() for o in outs]
+    srv.close()
+    say("batch of %d streams, %.1f G channel-samples per replay" % (len(streams), total / 1e9))
+
+    def ref(i):
+        s = streams[i]
+        ring = pool.get(s["fmt"], s["fft_points"], fs, round(s["tone"]), None, s["seed"]).cpu().numpy()
+        if s["fmt"] == "raw1":
+            return _oracle_ring_mean(oracle, ring, s["fft_points"], s["nffts"], "RawSigned", 1, 0, "Cosine2", fs)
+        return _oracle_ring_mean(oracle, ring, s["fft_points"], s["nffts"], "Mk5B", 4, s["channel"], "Cosine2", fs)
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        refs = list(ex.map(ref, range(len(streams))))
+    worst = 0.0
+    for s, g, r in zip(streams, got, refs):
+        e = relerr(g, r)
+        worst = max(worst, e)
+        assert e <= TOL, (s, e)
+    say("  worst stream vs FP64 oracle %.3e" % worst)
 
 
 def test_server_mfma64_jobs_in_mix(hip, oracle):
