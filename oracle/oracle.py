@@ -59,14 +59,18 @@ def build():
 
 REF_SRC = "/root/reference/src"
 REF_FILESINK = os.path.join(HERE, "_ref", "filesink_ref")
+REF_ADAPTER_DRV = os.path.join(HERE, "_ref", "taskcorehip_drv")
 
 
 def build_ref():
     """Compile oracle/_ref/ from the reference's own sources when they are
-    present (the reference FileSink and Buffer with our driver); returns
-    whether the reference FileSink binary is available."""
+    present: the reference FileSink and Buffer with our sink driver, and the
+    C++ TaskCoreHIP adapter driver (needs swspec_amd/libswspec_hip.so built
+    first).  Returns whether the reference FileSink binary is available."""
     if os.path.isdir(REF_SRC):
         subprocess.check_call(["make", "-s", "-C", HERE, "ref", "REFSRC=" + REF_SRC])
+        if os.path.exists(os.path.join(HERE, "..", "swspec_amd", "libswspec_hip.so")):
+            subprocess.check_call(["make", "-s", "-C", HERE, "adapter", "REFSRC=" + REF_SRC])
     return os.path.exists(REF_FILESINK)
 
 

@@ -59,6 +59,63 @@ def make_ring(torch, seed, fmt, nsamples, fs, tones_per_ch, dev="cuda"):
     return out
 
 
+def comb_tones(rng, nch, ntones=16, amp=0.141, offset=10_000, spacing=1_000_000):
+    """Per channel: a phase-cal comb of `ntones` tones at offset + k*spacing Hz
+    (amplitude 0.141 = -20 dB of unit noise power), seeded phases."""
+    return [[(amp, offset + k * spacing, float(rng.uniform(0, 2 * np.pi))) for k in range(ntones)]
+            for _ in range(nch)]
+
+
+def mk5b_payload(torch, seed, nsamples, fs=32e6, nch=4, ntones=16, dev="cuda"):
+    """Seeded 4-channel 2-bit payload on the device: one byte per time sample,
+    channel c in bits 2c..2c+1 (Mk5BUnpacker's increasing order), each channel
+    unit Gaussian noise plus a phase-cal comb; returns (uint8 tensor, tones)."""
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    raw = torch.zeros(nsamples, dtype=torch.uint8, device=dev)
+    tones = comb_tones(np.random.default_rng(seed), nch, ntones)
+    ifs = int(fs)
+    slab = 40_000_000
+    for ch in range(nch):
+        for s0 in range(0, nsamples, slab):
+            n = min(slab, nsamples - s0)
+            x = torch.randn(n, generator=g, device=dev, dtype=torch.float32)
+            idx = torch.arange(s0, s0 + n, device=dev, dtype=torch.int64)
+            for (a, f, ph) in tones[ch]:
+                r = (idx * f) % ifs          # exact phase numerator
+                x += a * torch.cos(r.to(torch.float32) * (2 * math.pi / fs) + ph)
+            raw[s0:s0 + n] |= _fields(torch, x) << (2 * ch)
+            del x, idx
+    torch.cuda.synchronize()
+    return raw, tones
+
+
+def xpol_pair(torch, seed, n, nfft, rho=0.2, phase_deg=30.0, dev="cuda"):
+    """Cross-pol pair as two 1-channel 2-bit streams (4 samples per byte,
+    oldest in the MSB): per FFT block of n samples, X ~ N(0,1) and
+    Y = rho (cos(phi) X + sin(phi) H{X}) + sqrt(1 - rho^2) Z, H the Hilbert
+    transform of the block (FP64 FFT), Z independent noise."""
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    ph = math.radians(phase_deg)
+    out = [torch.empty(n // 4 * nfft, dtype=torch.uint8, device=dev) for _ in range(2)]
+    f = torch.fft.fftfreq(n, device=dev, dtype=torch.float64)
+    hmul = -1j * torch.sign(f)                  # Hilbert transform in frequency
+    blk = max(1, (1 << 25) // n)
+    for b0 in range(0, nfft, blk):
+        nb = min(blk, nfft - b0)
+        x = torch.randn(nb, n, generator=g, device=dev, dtype=torch.float64)
+        z = torch.randn(nb, n, generator=g, device=dev, dtype=torch.float64)
+        h = torch.fft.ifft(torch.fft.fft(x, dim=1) * hmul, dim=1).real
+        y = rho * (math.cos(ph) * x + math.sin(ph) * h) + math.sqrt(1 - rho * rho) * z
+        for k, v in enumerate((x, y)):
+            q = _fields(torch, v.reshape(-1)).view(-1, 4)
+            out[k][b0 * n // 4:(b0 + nb) * n // 4] = (q[:, 0] << 6) | (q[:, 1] << 4) | (q[:, 2] << 2) | q[:, 3]
+        del x, z, h, y
+    torch.cuda.synchronize()
+    return out
+
+
 def batch_workload(seed=4, nstreams=64, fs=32e6):
     rng = np.random.default_rng(seed)
     out = []

@@ -76,3 +76,35 @@ def oracle_settings(oracle, s):
         source_channels=int(k["SourceChannels"]), use_channel_file1=int(k["UseFile1Channel"]),
         use_channel_file2=int(k["UseFile2Channel"]), extract_pcal=1 if k["ExtractPCal"] else 0,
         num_sources=s.num_sources, do_xpol=1 if k["DoCrossPolarization"] else 0)
+
+
+def oracle_split(oracle, mk, raws, nfft, bytes_per_fft, parts=8, precision=64):
+    """The oracle's doMaths over `nfft` non-overlapped FFTs (one spectrum,
+    K = nfft) computed as `parts` contiguous sub-integrations in threads
+    (ctypes releases the GIL while the C oracle runs) and combined by the
+    combine identity of TaskDispatcher (TaskCoreIPP.cpp:124, 311-349): the
+    mean is sum_p (n_p / nfft) mean_p, and phase-cal folds (raw sums) add.
+    mk(n) returns oracle settings integrating n FFTs; raws are the sources'
+    byte arrays.  Returns (auto per source, xpol list, pcal per source), FP64."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    parts = max(1, min(parts, nfft))
+    edges = [round(nfft * i / parts) for i in range(parts + 1)]
+
+    def one(i):
+        a, b = edges[i], edges[i + 1]
+        s = mk(b - a)
+        assert s.core_overlapped_ffts == b - a, (s.core_overlapped_ffts, b - a)
+        sub = [np.ascontiguousarray(r[a * bytes_per_fft: b * bytes_per_fft]) for r in raws]
+        m, auto, xp, pc = oracle.domaths(s, sub, precision)
+        assert m == 1
+        return b - a, auto, xp, pc
+
+    nth = max(1, min(parts, (os.cpu_count() or 1), 16))
+    with ThreadPoolExecutor(max_workers=nth) as ex:
+        res = list(ex.map(one, range(parts)))
+    w = [n / nfft for n, _, _, _ in res]
+    auto = [sum(wi * r[1][k][0].astype(np.float64) for wi, r in zip(w, res)) for k in range(len(res[0][1]))]
+    xp = [sum(wi * r[2][k][0].astype(np.complex128) for wi, r in zip(w, res)) for k in range(len(res[0][2]))]
+    pc = [sum(r[3][k][0].astype(np.complex128) for r in res) for k in range(len(res[0][3]))]
+    return auto, xp, pc
