@@ -331,6 +331,23 @@ void swspec_hip_host_free(void* p);
 int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d_raw, int channel, int64_t count,
                              float* d_out);
 
+/*
+ * swspec_hip_unpack_device without device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * arguments and results as swspec_hip_unpack_device, but asynchronous on
+ * `stream` (a hipStream_t; NULL = the null stream): the output is complete
+ * once work queued after it on that stream has run.  VLBA decode keeps its
+ * scratch (modulation bits, blank-zone table) in a fixed 1 MiB device array
+ * per GPU, so a VLBA call decodes at most 1748938752 samples (130752 payload
+ * segments) and fails with SWSPEC_EINVAL beyond that; other formats use no
+ * scratch.  Calls that overlap on one GPU (two streams, two threads) share
+ * that array and race; the first call on a device also copies the constant
+ * modulation bits there (synchronously, once).
+ */
+int swspec_hip_unpack_device_noalloc(const swspec_hip_cfg* cfg, const void* d_raw, int channel, int64_t count,
+                                     float* d_out, void* stream);
+
 /* Plan geometry for an FFT length (no device work) under the plan-shaping
  * flags (SWSPEC_FLAG_MFMA64).  info[0..13] = N1, N2, pass-1 columns per
  * workgroup, pass-1 threads, pass-2 threads, row pairs, pass-1 / pass-2

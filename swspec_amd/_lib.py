@@ -25,7 +25,7 @@ EXPORTS = [
     "swspec_hip_combine_device", "swspec_hip_reset_device",
     "swspec_hip_finalize", "swspec_hip_last_error", "swspec_hip_pcal_tones", "swspec_hip_pcal_ntones",
     "swspec_hip_last_run_seconds", "swspec_hip_host_alloc", "swspec_hip_host_free",
-    "swspec_hip_unpack_device", "swspec_hip_plan_info", "swspec_hip_autotune", "swspec_hip_autotune_clear", "swspec_hip_set_profiling", "swspec_hip_kernel_stats",
+    "swspec_hip_unpack_device", "swspec_hip_unpack_device_noalloc", "swspec_hip_plan_info", "swspec_hip_autotune", "swspec_hip_autotune_clear", "swspec_hip_set_profiling", "swspec_hip_kernel_stats",
     "swspec_hip_server_create", "swspec_hip_server_submit", "swspec_hip_server_step", "swspec_hip_server_poll",
     "swspec_hip_server_sync", "swspec_hip_server_run_batch", "swspec_hip_server_destroy",
     "swspec_hip_server_last_error", "swspec_hip_server_stats",
@@ -132,6 +132,7 @@ def load():
     L.swspec_hip_host_free.argtypes = [P]
     L.swspec_hip_host_free.restype = None
     L.swspec_hip_unpack_device.argtypes = [C.POINTER(Cfg), P, C.c_int, C.c_int64, P]
+    L.swspec_hip_unpack_device_noalloc.argtypes = [C.POINTER(Cfg), P, C.c_int, C.c_int64, P, P]
     L.swspec_hip_plan_info.argtypes = [C.c_int64, C.c_uint32, C.POINTER(C.c_int32), C.c_int]
     L.swspec_hip_autotune.argtypes = [C.c_int, C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                       C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32)]

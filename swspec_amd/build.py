@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libswspec_hip.so")
-SOURCES = ["engine.hip", "core.cpp", "server.cpp", "pcal.hip"]
+SOURCES = ["engine.hip", "core.cpp", "server.cpp", "pcal.hip", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -1055,11 +1055,12 @@ extern "C" void swspec_hip_host_free(void* p)
     if (p) (void)hipHostFree(p);
 }
 
-extern "C" int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d_raw, int channel, int64_t count,
-                                        float* d_out)
+namespace swspec {
+// Decoder of `channel` for a standalone decode call (swspec_hip_unpack_device
+// and its no-allocation variant): the factory choice plus the framed flag.
+// Returns 0, or a negative code with the library error text set.
+int decoder_for(const swspec_hip_cfg* cfg, int channel, Unpack& up)
 {
-    if (!cfg || !d_raw || !d_out || count < 0) return fail(nullptr, SWSPEC_EINVAL, "null argument");
-    Unpack up;
     std::string why;
     if (pick_unpack(cfg, channel, up, why) != 0) return fail(nullptr, SWSPEC_EUNSUPPORTED, why);
     if (cfg->flags & SWSPEC_FLAG_MK5B_FRAMED) {
@@ -1067,6 +1068,18 @@ extern "C" int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d
             return fail(nullptr, SWSPEC_EUNSUPPORTED, "SWSPEC_FLAG_MK5B_FRAMED needs Mk5B input");
         up.framed = 1;   // count samples from payload byte 0 of the frames at d_raw
     }
+    return SWSPEC_OK;
+}
+
+int set_library_error(int code, const char* msg) { return fail(nullptr, code, msg); }
+}  // namespace swspec
+
+extern "C" int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d_raw, int channel, int64_t count,
+                                        float* d_out)
+{
+    if (!cfg || !d_raw || !d_out || count < 0) return fail(nullptr, SWSPEC_EINVAL, "null argument");
+    Unpack up;
+    if (int rc = decoder_for(cfg, channel, up)) return rc;
     hipError_t e = hipSuccess;
     uint32_t* d_mod = nullptr;
     int2* d_blank = nullptr;

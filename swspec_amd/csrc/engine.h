@@ -224,4 +224,11 @@ hipError_t launch_unpack(const uint8_t* raw, Unpack up, long long count, float* 
 // (blanker_mark5.c:126-200), written to each job's up.vblank.
 hipError_t launch_vlba_blank(const JobDev* jobs, int njobs, int nfft, long long N, hipStream_t s);
 
+// Decoder for a standalone decode call of the C ABI (core.cpp): the
+// factory choice for `channel` plus SWSPEC_FLAG_MK5B_FRAMED.  Returns 0 or a
+// negative SWSPEC_* code with the library error text set.
+int decoder_for(const struct swspec_hip_cfg* cfg, int channel, Unpack& up);
+// Sets the library error text (swspec_hip_last_error(NULL)); returns code.
+int set_library_error(int code, const char* msg);
+
 }  // namespace swspec
