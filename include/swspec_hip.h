@@ -90,7 +90,8 @@ typedef struct swspec_hip_cfg {
     uint32_t ffts_per_chunk;       /* 0 = automatic */
     char     mark5_format[32];     /* cfg->sourceformat_str for the mark5access formats
                                       (VLBA: "VLBA1_<fanout>-<Mbps>-<nchan>-<nbit>[/<decim>]",
-                                      mark5_stream.c:556-571); NUL-terminated */
+                                      mark5_stream.c:556-571; decoded: fanout 1/2/4 with
+                                      4 channels x 2 bits or 8 x 1 bit); NUL-terminated */
 } swspec_hip_cfg;
 
 typedef struct swspec_hip_core swspec_hip_core;
@@ -339,9 +340,9 @@ int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d_raw, int c
  * `stream` (a hipStream_t; NULL = the null stream): the output is complete
  * once work queued after it on that stream has run.  VLBA decode keeps its
  * scratch (modulation bits, blank-zone table) in a fixed 1 MiB device array
- * per GPU, so a VLBA call decodes at most 1748938752 samples (130752 payload
- * segments) and fails with SWSPEC_EINVAL beyond that; other formats use no
- * scratch.  Calls that overlap on one GPU (two streams, two threads) share
+ * per GPU, so a VLBA call decodes at most 1748938752 samples at fanout 1 or
+ * 2 and 2331918336 at fanout 4 (130752 blank-zone entries) and fails with
+ * SWSPEC_EINVAL beyond that; other formats use no scratch.  Calls that overlap on one GPU (two streams, two threads) share
  * that array and race; the first call on a device also copies the constant
  * modulation bits there (synchronously, once).
  */

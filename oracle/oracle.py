@@ -25,7 +25,7 @@ class Ini(C.Structure):
         ("pcal_harmonics_hz", C.c_float), ("source_format", C.c_int), ("bits_per_sample", C.c_int),
         ("channelorder_increasing", C.c_int), ("source_channels", C.c_int),
         ("use_channel_file1", C.c_int), ("use_channel_file2", C.c_int), ("extract_pcal", C.c_int),
-        ("num_sources", C.c_int), ("do_xpol", C.c_int),
+        ("num_sources", C.c_int), ("do_xpol", C.c_int), ("vlba_fanout", C.c_int),
     ]
 
 
@@ -45,7 +45,7 @@ class Settings(C.Structure):
         ("max_specffts_per_buffer", C.c_int), ("max_buffers_per_spectrum", C.c_int),
         ("core_overlapped_ffts", C.c_int), ("core_averaged_ffts", C.c_int), ("pcal_tonebins", C.c_int),
         ("pcal_rotatorlen", C.c_int), ("pcal_pulses", C.c_int), ("pcal_pulses_per_fft", C.c_int),
-        ("pcal_result_bytes", C.c_size_t),
+        ("pcal_result_bytes", C.c_size_t), ("vlba_fanout", C.c_int),
     ]
 
     def as_dict(self):
@@ -141,6 +141,11 @@ def lib():
         L.so_vlba_blank.restype = None
         L.so_vlba_unpack.argtypes = [P, C.c_long, C.c_size_t, C.c_int, P]
         L.so_vlba_unpack.restype = C.c_size_t
+        L.so_vlba_unpack_mode.argtypes = [P, C.c_long, C.c_size_t, C.c_int, C.c_int, C.c_int, P]
+        L.so_vlba_unpack_mode.restype = C.c_size_t
+        L.so_vlba_blank_zone.argtypes = [P, C.c_long, C.c_long, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int)]
+        L.so_vlba_blank_zone.restype = None
         _lib = L
     return _lib
 
@@ -155,12 +160,15 @@ def make_ini(**kw):
              fft_integ_seconds=20.0, fft_overlap_factor=2, window=WINDOWS["Cosine2"], bandwidth_hz=8e6,
              pcal_offset_hz=10e3, pcal_harmonics_hz=1e6, source_format=FORMATS["RawSigned"],
              bits_per_sample=8, channelorder_increasing=1, source_channels=1, use_channel_file1=1,
-             use_channel_file2=1, extract_pcal=0, num_sources=1, do_xpol=0)
+             use_channel_file2=1, extract_pcal=0, num_sources=1, do_xpol=0, vlba_fanout=1)
     for k, v in kw.items():
         if k == "window" and isinstance(v, str):
             v = WINDOWS[v]
         if k == "source_format" and isinstance(v, str):
-            # FileSource::open (src/FileSource.cpp:41-106): mark5access names by prefix
+            # FileSource::open (src/FileSource.cpp:41-106): mark5access names by prefix;
+            # "VLBA1_<fanout>-<Mbps>-<nchan>-<nbit>" carries the fanout
+            if v.upper().startswith("VLBA1_"):
+                d["vlba_fanout"] = int(v[6:].split("-")[0])
             v = FORMATS.get(v, FORMATS["VLBA"] if v.upper().startswith("VLBA") else -1)
         d[k] = v
     ini = Ini()
@@ -368,9 +376,22 @@ def vlba_blank(buf, seg, avail=None):
     return vs.value, ve.value
 
 
-def vlba_unpack(raw, count, channel, avail=None):
-    """VLBAUnpacker::extract_samples for VLBA1_1-*-4-2 (one byte per time sample)."""
+def vlba_unpack(raw, count, channel, avail=None, fanout=1, nbit=2):
+    """VLBAUnpacker::extract_samples for the one-byte-per-time-sample modes
+    VLBA1_<fanout>-*-4-2 (nbit 2) and VLBA1_<fanout>-*-8-1 (nbit 1)."""
     raw = np.ascontiguousarray(raw, dtype=np.uint8)
     out = np.zeros(count, dtype=np.float32)
-    lib().so_vlba_unpack(_p(raw), len(raw) if avail is None else avail, count, channel, _p(out))
+    n = lib().so_vlba_unpack_mode(_p(raw), len(raw) if avail is None else avail, count, channel, fanout, nbit,
+                                  _p(out))
+    if n != count:
+        raise ValueError("VLBA mode fanout %d nbit %d not restated" % (fanout, nbit))
     return out
+
+
+def vlba_blank_zone(buf, seg, payload, zone, avail=None):
+    """Blank zone `zone` of the `payload`-byte payload at byte `seg` (payload-relative bounds)."""
+    buf = np.ascontiguousarray(buf, dtype=np.uint8)
+    vs, ve = C.c_int(), C.c_int()
+    lib().so_vlba_blank_zone(_p(buf), len(buf) if avail is None else avail, seg, payload, zone, C.byref(vs),
+                             C.byref(ve))
+    return vs.value, ve.value

@@ -49,6 +49,7 @@ int so_derive(const so_ini* ini, so_settings* s)
 {
     memset(s, 0, sizeof(*s));
     s->num_cores = ini->num_cores > 0 ? ini->num_cores : 1;
+    s->vlba_fanout = ini->vlba_fanout;
     /* :70 default, :104-106 INI key (MByte, double-buffered: *1024*1024/2) */
     if (ini->max_source_buffer_mb >= 0) {
         s->max_rawbuf_size = (size_t)ini->max_source_buffer_mb;
@@ -172,9 +173,13 @@ int so_pick_unpacker(const so_settings* s)
         return 0;
     }
     if (s->source_format == SO_FMT_VLBA) {
-        /* VLBAUnpacker::canHandleConfig (:530-533); only 8 tracks, fanout 1,
-         * 2-bit (4 channels) is restated */
-        if (s->bits_per_sample == 2 && s->source_channels == 4) return 4;
+        /* VLBAUnpacker::canHandleConfig (:530-533); restated for the modes
+         * with one byte per time sample: 4 channels x 2 bits or 8 x 1 bit,
+         * fanout 1, 2 or 4 */
+        const int f = s->vlba_fanout;
+        if (f != 1 && f != 2 && f != 4) return 0;
+        if ((s->bits_per_sample == 2 && s->source_channels == 4) ||
+            (s->bits_per_sample == 1 && s->source_channels == 8)) return 4;
         return 0;
     }
     return 0;
@@ -211,7 +216,7 @@ size_t so_unpack(const so_settings* s, const uint8_t* src, float* dst, size_t co
         }
         return smp;
     }
-    if (kind == 4) return so_vlba_unpack(src, (long)count, count, channel, dst);
+    if (kind == 4) return so_vlba_unpack_mode(src, (long)count, count, channel, s->vlba_fanout, s->bits_per_sample, dst);
     return 0;
 }
 
@@ -885,7 +890,8 @@ int so_domaths(const so_settings* s, int precision,
         for (int rs = 0; rs < nsrc; rs++) {
             int ch = rs == 0 ? s->use_channel_file1 : s->use_channel_file2;
             if (so_pick_unpacker(s) == 4)   /* the blanker may read past the FFT's bytes */
-                so_vlba_unpack(src[rs], (long)(len[rs] - (size_t)(src[rs] - src_in[rs])), (size_t)N, ch, xs);
+                so_vlba_unpack_mode(src[rs], (long)(len[rs] - (size_t)(src[rs] - src_in[rs])), (size_t)N, ch,
+                                    s->vlba_fanout, s->bits_per_sample, xs);
             else
                 so_unpack(s, src[rs], xs, (size_t)N, ch);
             src[rs] += s->raw_overlap_bytes;
@@ -999,28 +1005,34 @@ static int vl_fill(const uint8_t* p, long off, int bytes, long avail)
     return 1;
 }
 
-/* blanker_mark5 (blanker_mark5.c:126-200) for one 20000-byte VLBA payload
- * (8 tracks: 2500 words, one 32 KiB zone) at byte `seg` of p: bytes i with
- * vs <= i < ve are data, the rest decode to 0.  Both boundary searches
- * (findfirstinvalid :48-81, findfirstvalid :83-116) are the same bisection
- * over 64-bit words plus a 32-bit straggler check. */
-void so_vlba_blank(const uint8_t* p, long avail, long seg, int* vs, int* ve)
+/* blanker_mark5 (blanker_mark5.c:126-200) for blank zone z of a `payload`-
+ * byte VLBA payload at byte `seg` of p.  The blanker cuts the payload's
+ * 64-bit words into zones of 4096 words (32 KiB, log2blankzonesize 15):
+ * 20000-byte payloads (8 tracks) have one zone, 40000 (16 tracks) two,
+ * 80000 (32 tracks) three.  Bytes i with vs <= i < ve (payload-relative) are
+ * data, the rest decode to 0.  Both boundary searches (findfirstinvalid
+ * :48-81, findfirstvalid :83-116) are the same bisection over the zone's
+ * 64-bit words plus a 32-bit straggler check. */
+void so_vlba_blank_zone(const uint8_t* p, long avail, long seg, int payload, int z, int* vs, int* ve)
 {
-    const int nword = SO_VLBA_PAYLOAD / 8;
-    const int start_ok = !vl_fill(p, seg, 8, avail);
-    const int end_ok = !vl_fill(p, seg + 8L * (nword - 1), 8, avail);
+    const int nword = payload / 8;
+    const int b = 4096 * z;
+    const int e = (b + 4096 < nword) ? b + 4096 : nword;
+    const int start_ok = !vl_fill(p, seg + 8L * b, 8, avail);
+    const int end_ok = !vl_fill(p, seg + 8L * (e - 1), 8, avail);
     if (start_ok && end_ok) { *vs = 0; *ve = 1 << 30; return; }
     if (!start_ok && !end_ok) { *vs = 1 << 30; *ve = 0; return; }
     /* bisection for the boundary word: `want_fill` = the kind of word that
      * moves the boundary down */
     const int want_fill = start_ok;   /* data then fill: find first fill word */
-    int i = nword / 2, r = nword - 1;
-    for (int s = nword / 4; s > 0; s >>= 1) {
-        const int isfill = vl_fill(p, seg + 8L * i, 8, avail);
+    const int n = e - b;
+    int i = n / 2, r = n - 1;
+    for (int s = n / 4; s > 0; s >>= 1) {
+        const int isfill = vl_fill(p, seg + 8L * (b + i), 8, avail);
         if (isfill == want_fill) { r = i; i -= s; }
         else i += s;
     }
-    int byte = 8 * r;
+    int byte = 8 * (b + r);
     if (start_ok) {
         if (vl_fill(p, seg + byte - 4, 4, avail)) byte -= 4;   /* data32[r/4 - 1] */
         *vs = 0;
@@ -1032,42 +1044,110 @@ void so_vlba_blank(const uint8_t* p, long avail, long seg, int* vs, int* ve)
     }
 }
 
-/* VLBAUnpacker::extract_samples (src/IA-32/DataUnpackers.cpp:500-528) for
- * VLBA1_1-<Mbps>-4-2 (8 tracks, fanout 1, 2-bit, mark5_format_vlba.c
- * vlba_decode_2bit_8track_fanout1_decimation1 :4537-4583):
+/* One 20000-byte payload (8 tracks): its single zone. */
+void so_vlba_blank(const uint8_t* p, long avail, long seg, int* vs, int* ve)
+{
+    so_vlba_blank_zone(p, avail, seg, SO_VLBA_PAYLOAD, 0, vs, ve);
+}
+
+/* mark5access's 32-track reordering of a little-endian 32-bit word
+ * (mark5_format_vlba.c:74-76): bits of the 0xAA55AA55 mask stay, bits of
+ * 0x55005500 move down 7 places, bits of 0x00AA00AA up 7. */
+static uint32_t vl_reorder32(uint32_t x)
+{
+    return (x & 0xAA55AA55u) | ((x & 0x55005500u) >> 7) | ((x & 0x00AA00AAu) << 7);
+}
+
+/* VLBAUnpacker::extract_samples (src/IA-32/DataUnpackers.cpp:500-528) for the
+ * VLBA modes with one byte per time sample, ntrack = 8 x fanout:
  *  - the request is cut into chunks of 2*(framebytes - payloadoffset) =
- *    2*(20160 - 96) = 40128 bytes; each chunk is one mark5_unpack call on an
- *    unpacker stream without headers (mark5_stream_unpacker.c:112-129), so
- *    the payload index and the modulation restart at every chunk (and at
- *    every call, i.e. every FFT);
- *  - inside a chunk a payload is 20000 bytes; the blanker runs at the start
- *    of each payload (mark5_stream.c:147-196);
- *  - byte b of payload index i gives channel c the level
- *    {-HiMag, +1, -1, +HiMag}[(b >> 2c) & 3] (initluts :90-165), negated when
- *    modulate[i] is 1, or 0 inside a blank zone. */
-size_t so_vlba_unpack(const uint8_t* src, long avail, size_t count, int channel, float* dst)
+ *    2*(2520 - 12) * ntrack bytes (40128, 80256, 160512); each chunk is one
+ *    mark5_unpack call on an unpacker stream without headers
+ *    (mark5_stream_unpacker.c:112-129), so the payload index and the
+ *    modulation restart at every chunk (and at every call, i.e. every FFT);
+ *  - inside a chunk a payload is 2500 * ntrack bytes (databytes,
+ *    mark5_format_vlba.c:9533); the blanker runs at the start of each
+ *    payload (mark5_stream.c:147-196), the third payload of a chunk being its
+ *    16 * ntrack-byte tail;
+ *  - a frame word of `fanout` bytes carries `fanout` time samples of every
+ *    channel; the word index m (payload byte / fanout) selects modulate[m],
+ *    which negates the word's samples; a word whose first byte lies outside
+ *    its zone's data range gives zeros;
+ *  - levels (initluts :90-165): 2-bit {-HiMag, +1, -1, +HiMag}[sign bit +
+ *    2 x magnitude bit], 1-bit {+1, -1}[bit].  Per mode, sample u of a word
+ *    (u < fanout) of channel c reads:
+ *      2-bit fanout 1 (vlba_decode_2bit_8track_fanout1_decimation1 :4537):
+ *        byte 0, sign bit 2c, magnitude bit 2c+1;
+ *      2-bit fanout 2 (_2bit_16track_fanout2 :5147, lut2bit2): byte c/2,
+ *        sign bit 4(c%2)+u, magnitude bit that + 2;
+ *      2-bit fanout 4 (_2bit_32track_fanout4 :5907): byte k of the word
+ *        reordered by reorder32, k = 0, 2, 1, 3 for c = 0..3, sign bit 2u,
+ *        magnitude bit 2u+1 (lut2bit1);
+ *      1-bit fanout 1 (_1bit_8track_fanout1 :1362): byte 0, bit c;
+ *      1-bit fanout 2 (_1bit_16track_fanout2 :2021): byte c/4, bit 2(c%4)+u;
+ *      1-bit fanout 4 (_1bit_32track_fanout4 :2891): byte c%4, bit 2u + c/4. */
+size_t so_vlba_unpack_mode(const uint8_t* src, long avail, size_t count, int channel, int fanout, int nbit,
+                           float* dst)
 {
     static uint8_t mod[SO_VLBA_PAYLOAD];
     static int mod_ready = 0;
     if (!mod_ready) { so_vlba_modulate(mod); mod_ready = 1; }
+    if ((fanout != 1 && fanout != 2 && fanout != 4) || (nbit != 1 && nbit != 2)) return 0;
+    const int nch = nbit == 2 ? 4 : 8;
+    const int c = channel % nch;
+    const int ntrack = 8 * fanout;
+    const long payload = 2500L * ntrack;
+    const long chunk = 2L * (2520 - 12) * ntrack;
     const float himag = 3.3359f;   /* OPTIMAL_2BIT_HIGH, mark5_stream.h:72 */
     const float lvl[4] = {-himag, 1.0f, -1.0f, himag};
-    const long chunk = SO_VLBA_CHUNK;
     for (size_t c0 = 0; c0 < count; c0 += (size_t)chunk) {
         const size_t n = (count - c0 < (size_t)chunk) ? count - c0 : (size_t)chunk;
-        int vs = 0, ve = 0;
+        int zs[3] = {0, 0, 0}, ze[3] = {0, 0, 0};
         for (size_t o = 0; o < n; o++) {
-            const int i = (int)(o % SO_VLBA_PAYLOAD);
-            if (i == 0) so_vlba_blank(src, avail, (long)(c0 + o), &vs, &ve);
+            const long i = (long)(o % (size_t)payload);     /* payload byte = time sample */
+            if (i == 0) {
+                for (int z = 0; z < (int)((payload / 8 + 4095) / 4096); z++)
+                    so_vlba_blank_zone(src, avail, (long)(c0 + o), (int)payload, z, &zs[z], &ze[z]);
+            }
+            const long m = i / fanout;                      /* frame word */
+            const long i0 = m * fanout;                     /* its first byte */
+            const int u = (int)(i - i0);
+            const int z = (int)(i0 >> 15);
             float v = 0.0f;
-            if (i >= vs && i < ve) {
-                v = lvl[(src[c0 + o] >> (2 * (channel & 3))) & 3u];
-                if (mod[i]) v = -v;
+            if (i0 >= zs[z] && i0 < ze[z]) {
+                const uint8_t* w = src + c0 + (o - (size_t)u);   /* the word's first byte */
+                if (nbit == 2) {
+                    int sb, byte;
+                    if (fanout == 1) { byte = w[0]; sb = 2 * c; }
+                    else if (fanout == 2) { byte = w[c / 2]; sb = 4 * (c % 2) + u; }
+                    else {
+                        const uint32_t x = (uint32_t)w[0] | ((uint32_t)w[1] << 8) | ((uint32_t)w[2] << 16) |
+                                           ((uint32_t)w[3] << 24);
+                        const int k = (c == 1) ? 2 : (c == 2) ? 1 : c;
+                        byte = (int)((vl_reorder32(x) >> (8 * k)) & 0xFFu);
+                        sb = 2 * u;
+                    }
+                    const int mb = (fanout == 2) ? sb + 2 : sb + 1;
+                    v = lvl[((byte >> sb) & 1) + 2 * ((byte >> mb) & 1)];
+                } else {
+                    int bit, byte;
+                    if (fanout == 1) { byte = w[0]; bit = c; }
+                    else if (fanout == 2) { byte = w[c / 4]; bit = 2 * (c % 4) + u; }
+                    else { byte = w[c % 4]; bit = 2 * u + c / 4; }
+                    v = ((byte >> bit) & 1) ? -1.0f : 1.0f;
+                }
+                if (mod[m]) v = -v;
             }
             dst[c0 + o] = v;
         }
     }
     return count;
+}
+
+/* 8 tracks, fanout 1, 2-bit (VLBA1_1-<Mbps>-4-2). */
+size_t so_vlba_unpack(const uint8_t* src, long avail, size_t count, int channel, float* dst)
+{
+    return so_vlba_unpack_mode(src, avail, count, channel, 1, 2, dst);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -48,6 +48,7 @@ typedef struct so_ini {
     int    extract_pcal;
     int    num_sources;            /* 1 or 2 input files */
     int    do_xpol;
+    int    vlba_fanout;            /* VLBA formats: <fanout> of "VLBA1_<fanout>-..." (1, 2, 4) */
 } so_ini;
 
 /* Derived settings (src/Settings.h:59-159, derivation src/swspectrometer.cpp:161-231). */
@@ -89,6 +90,7 @@ typedef struct so_settings {
     int    pcal_pulses;
     int    pcal_pulses_per_fft;
     size_t pcal_result_bytes;
+    int    vlba_fanout;            /* copied from so_ini */
 } so_settings;
 
 /* Settings derivation, float arithmetic as in the reference
@@ -109,7 +111,8 @@ extern const float so_map_rev[4];
 
 /* Which decoder the factory picks (src/IA-32/DataUnpackerFactory.h:38-108):
  * 1 = TwoBitUnpacker, 2 = TwoBitSinglechannelUnpacker, 3 = Mk5BUnpacker (4/8 ch),
- * 4 = VLBAUnpacker for VLBA1_1-*-4-2 (see so_vlba_unpack),
+ * 4 = VLBAUnpacker for the modes with one byte per time sample (4 channels x
+ *     2 bits or 8 channels x 1 bit at fanout 1, 2 or 4; see so_vlba_unpack_mode),
  * 0 = none supported here (8/16-bit, MKIV/Mark5B via mark5access, other VLBA modes). */
 int so_pick_unpacker(const so_settings* s);
 
@@ -237,13 +240,16 @@ uint64_t so_pcalx_final(so_pcalx* p, double* tones_out);
 double so_pcal_delay64(const double* tones, int ntones, double spacing_hz, int upsample);
 
 /* ---- VLBA through mark5access (VLBAUnpacker, DataUnpackers.cpp:459-535) ----
- * Restated from the vendored mark5access 1.5.0 sources for the 8-track,
- * fanout-1, 2-bit case ("VLBA1_1-<Mbps>-4-2"; 4 channels, one byte per time
- * sample).  mark5access cannot be built here (its sources include an
- * autotools-generated config.h), so these functions are pinned by closed
- * forms and invariants only: the register's period and balance, literal LUT
- * bytes, blank zones of synthetic fill patterns against a linear scan, and
- * the per-chunk modulation restart (tests/test_oracle_vlba.py). */
+ * Restated from the vendored mark5access 1.5.0 sources for the modes whose
+ * time sample is one byte, the only ones VLBAUnpacker::extract_samples walks
+ * consistently (it advances its source by the sample count, :515-522):
+ * "VLBA1_<fanout>-<Mbps>-4-2" and "VLBA1_<fanout>-<Mbps>-8-1" at fanout 1, 2
+ * and 4 (8, 16 and 32 tracks).  mark5access cannot be built here (its
+ * sources include an autotools-generated config.h), so these functions are
+ * pinned by closed forms and invariants only: the register's period and
+ * balance, literal LUT bytes per mode, blank zones of synthetic fill patterns
+ * against a linear scan and hand traces, and the per-chunk modulation
+ * restart (tests/test_oracle_vlba.py). */
 #define SO_VLBA_PAYLOAD 20000   /* data bytes per frame, 8 tracks */
 #define SO_VLBA_CHUNK 40128     /* 2 * (framebytes 20160 - payloadoffset 96) */
 /* modulate[0..19999] (mark5_format_vlba.c:167-196), values 0/1. */
@@ -253,8 +259,16 @@ void so_vlba_modulate(uint8_t* out);
 void so_vlba_blank(const uint8_t* p, long avail, long seg, int* vs, int* ve);
 /* extract_samples of `count` samples of `channel` (0..3) from src; `avail`
  * bytes are readable from src (the blanker looks up to 20000 bytes past the
- * last chunk's payload start). */
+ * last chunk's payload start).  8 tracks, fanout 1, 2-bit. */
 size_t so_vlba_unpack(const uint8_t* src, long avail, size_t count, int channel, float* dst);
+/* Blank zone z (words [4096 z, min(4096 z + 4096, payload/8))) of the
+ * `payload`-byte payload at byte `seg` of p; bounds in payload bytes. */
+void so_vlba_blank_zone(const uint8_t* p, long avail, long seg, int payload, int z, int* vs, int* ve);
+/* extract_samples for any of the six one-byte modes: fanout 1/2/4, nbit 2
+ * (4 channels) or 1 (8 channels); `count` a multiple of the fanout.
+ * Returns count, or 0 for a mode outside the six. */
+size_t so_vlba_unpack_mode(const uint8_t* src, long avail, size_t count, int channel, int fanout, int nbit,
+                           float* dst);
 
 /* ---- Mk5B framing -----------------------------------------------------------
  * FileSource::read framed mode (src/FileSource.cpp:184-215): strip the 16-byte

@@ -287,8 +287,11 @@ extern "C" int swspec_hip_cfg_from_ini(const swspec_hip_ini* ini, const swspec_h
 static int pick_unpack(const swspec_hip_cfg* c, int channel, Unpack& up, std::string& why)
 {
     up = Unpack();
-    if (c->bits_per_sample != 2) { why = "only 2-bit samples are implemented"; return -1; }
     if (channel < 0 || channel >= c->source_channels) { why = "channel out of range"; return -1; }
+    if (c->bits_per_sample != 2 && !(c->bits_per_sample == 1 && c->source_format == SWSPEC_FMT_VLBA)) {
+        why = "only 2-bit samples (and 1-bit VLBA) are implemented";
+        return -1;
+    }
     const bool raw = c->source_format == SWSPEC_FMT_RAWSIGNED || c->source_format == SWSPEC_FMT_RAWUNSIGNED;
     if (raw && (c->source_channels % 4) == 0) {
         // TwoBitUnpacker (DataUnpackers.cpp:191-244)
@@ -326,14 +329,26 @@ static int pick_unpack(const swspec_hip_cfg* c, int channel, Unpack& up, std::st
         const bool prefix = strncasecmp(fmt, "VLBA1_", 6) == 0;
         const int r = prefix ? sscanf(fmt + 6, "%d-%d-%d-%d/%d", &fan, &mbps, &nch, &nbit, &dec) : 0;
         if (r < 4) { why = "VLBA format string not understood: '" + std::string(fmt) + "'"; return -1; }
-        if (fan != 1 || nch != 4 || nbit != 2 || dec != 1) {
-            why = "VLBA decode implemented for VLBA1_1-<Mbps>-4-2 (8 tracks, fanout 1, 2-bit) only";
+        // VLBAUnpacker advances its source by the samples it decodes, not the
+        // bytes (:515-522, "TODO: multiply payloadbytes by fanout"): only the
+        // modes with one byte per time sample (nchan x nbit = 8) decode
+        // consistently, at fanout 1, 2, 4 and without decimation
+        if ((fan != 1 && fan != 2 && fan != 4) || dec != 1 || !((nch == 4 && nbit == 2) || (nch == 8 && nbit == 1))) {
+            why = "VLBA decode implemented for VLBA1_<1|2|4>-<Mbps>-4-2 and VLBA1_<1|2|4>-<Mbps>-8-1 "
+                  "(one byte per time sample, no decimation) only";
             return -1;
         }
-        if (c->source_channels != 4) { why = "VLBA1_1-*-4-2 needs SourceChannels = 4"; return -1; }
+        if (c->source_channels != nch || c->bits_per_sample != nbit) {
+            why = "VLBA format needs SourceChannels = " + std::to_string(nch) + " and BitsPerSample = " +
+                  std::to_string(nbit);
+            return -1;
+        }
+        if (c->fft_points % fan != 0) { why = "VLBA FFT length must be a multiple of the fanout"; return -1; }
         up.kind = kUnpackVLBA;
         up.step = 1;
-        up.shift = 2 * (channel % 4);   // allchannels[channel % nchan] (:518)
+        up.vfsh = fan == 1 ? 0 : (fan == 2 ? 1 : 2);
+        up.vnbit = nbit;
+        up.cbyte = channel % nch;   // allchannels[channel % nchan] (:518)
         return 0;
     }
     why = "decoder not implemented for this format/channel count";
@@ -366,7 +381,10 @@ static int core_alloc(swspec_hip_core* c)
     if (c->vlba) {
         uint32_t bits[kVlbaModWords];
         vlba_modulation_bits(bits);
-        const int nseg = vlba_segments(c->cfg.fft_points);
+        int fsh = 0;
+        for (int j = 0; j < c->njobs; j++)
+            if (c->up[j].kind == kUnpackVLBA) fsh = c->up[j].vfsh;
+        const int nseg = (int)vlba_segments(c->cfg.fft_points, fsh);
         HIPCHK(c, hipMalloc(&c->d_vmod, sizeof(bits)));
         HIPCHK(c, hipMemcpy(c->d_vmod, bits, sizeof(bits), hipMemcpyHostToDevice));
         HIPCHK(c, hipMalloc(&c->d_vblank, (size_t)c->njobs * c->G * nseg * sizeof(int2)));
@@ -1087,7 +1105,7 @@ extern "C" int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d
         // one extract_samples call over `count` bytes: its blank zones see only these bytes
         uint32_t bits[kVlbaModWords];
         vlba_modulation_bits(bits);
-        up.vnseg = vlba_segments(count);
+        up.vnseg = (int)vlba_segments(count, up.vfsh);
         e = hipMalloc(&d_mod, sizeof(bits));
         if (e == hipSuccess) e = hipMemcpy(d_mod, bits, sizeof(bits), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMalloc(&d_blank, (size_t)std::max(up.vnseg, 1) * sizeof(int2));

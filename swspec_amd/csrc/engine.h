@@ -24,16 +24,20 @@ enum UnpackKind : int {
     kUnpackNone = 0,
     kUnpackMultiChannel = 1,   // TwoBitUnpacker / Mk5BUnpacker: one byte group per time sample
     kUnpackSingleChannel = 2,  // TwoBitSinglechannelUnpacker: 4 samples per byte, oldest in MSB
-    kUnpackVLBA = 3,           // VLBAUnpacker, VLBA1_1-*-4-2: 8 tracks, fanout 1, 2-bit, 1 byte per sample
+    kUnpackVLBA = 3,           // VLBAUnpacker, one byte per time sample: VLBA1_<1|2|4>-*-4-2 / -8-1
 };
 
-// VLBA framing seen by VLBAUnpacker::extract_samples (DataUnpackers.cpp:500-528):
-// every call (one FFT) is cut into chunks of 2*(20160 - 96) bytes, each an
-// unpacker stream without headers whose 20000-byte payloads restart the
-// modulation index and get their own blank zone.
-constexpr int kVlbaPayload = 20000;
-constexpr int kVlbaChunk = 40128;
-constexpr int kVlbaSegPerChunk = 3;    // payloads at 0, 20000 and the 128-byte tail at 40000
+// VLBA framing seen by VLBAUnpacker::extract_samples (DataUnpackers.cpp:500-528)
+// for the modes with one byte per time sample (ntrack = 8 x fanout, fanout
+// 2^fsh): every call (one FFT) is cut into chunks of 2*(2520 - 12)*ntrack
+// bytes, each an unpacker stream without headers whose payloads of
+// 2500*ntrack bytes restart the modulation index; each payload is cut into
+// blank zones of 32 KiB (fsh + 1 of them), and the chunk's third payload is
+// its 16*ntrack-byte tail.  The modulation index counts frame words of
+// `fanout` bytes, 20000 per payload.
+constexpr int kVlbaPayload = 20000;    // payload bytes at 8 tracks; frame words per payload
+constexpr int kVlbaChunk = 40128;      // chunk bytes at 8 tracks
+constexpr int kVlbaSegPerChunk = 3;    // payloads at 0, P and the tail at 2P
 constexpr int kVlbaModWords = (kVlbaPayload + 31) / 32;
 
 struct Unpack {
@@ -44,8 +48,10 @@ struct Unpack {
     // Mk5B frames as stored on disk (multi-channel layout only): payload byte P
     // is at mk5b_frame_addr(P) of the input
     int framed = 0;
-    // VLBA only
-    int vnseg = 0;                       // payload segments per FFT
+    // VLBA only (cbyte holds the channel, 0..nchan-1)
+    int vfsh = 0;                        // log2 fanout: 0, 1, 2 (8, 16, 32 tracks)
+    int vnbit = 2;                       // 2 (4 channels) or 1 (8 channels)
+    int vnseg = 0;                       // blank-zone entries per FFT (vlba_segments)
     const uint32_t* vmod = nullptr;      // modulation bits, payload index i at bit i
     int2* vblank = nullptr;              // [FFT of the chunk][segment] data zone (start, end)
 };
@@ -59,10 +65,12 @@ __host__ __device__ inline long long mk5b_frame_addr(long long P)
     return fr * kMk5bFrame + kMk5bHeader + (P - fr * kMk5bPayload);
 }
 
-// Segments of an N-sample VLBA FFT.
-inline int vlba_segments(long long N)
+// Blank-zone entries of an N-sample VLBA FFT: chunks x 3 payload segments x
+// (fsh + 1) zones.
+inline long long vlba_segments(long long N, int fsh)
 {
-    return (int)(((N + kVlbaChunk - 1) / kVlbaChunk) * kVlbaSegPerChunk);
+    const long long chunk = (long long)kVlbaChunk << fsh;
+    return ((N + chunk - 1) / chunk) * kVlbaSegPerChunk * (fsh + 1);
 }
 // modulate[] of mark5_format_vlba.c:167-196 packed as bits.
 void vlba_modulation_bits(uint32_t* words /* kVlbaModWords */);

@@ -370,31 +370,84 @@ __device__ __forceinline__ unsigned field2(const uint8_t* __restrict__ raw, cons
     return (raw[j * up.step + up.cbyte] >> up.shift) & 3u;
 }
 
-// VLBA level of 2-bit code l = (byte >> 2c) & 3 (mark5access initluts,
-// mark5_format_vlba.c:90-165): {-HiMag, +1, -1, +HiMag}[l], HiMag = 3.3359f
-// (mark5_stream.h:72).  Note the order differs from swspec's own map above.
+// VLBA level of 2-bit code l = sign bit + 2 x magnitude bit (mark5access
+// initluts, mark5_format_vlba.c:90-165): {-HiMag, +1, -1, +HiMag}[l],
+// HiMag = 3.3359f (mark5_stream.h:72).  Note the order differs from swspec's
+// own map above.
 __device__ __forceinline__ float vlba_level(unsigned l)
 {
     const float mag = ((l ^ (l >> 1)) & 1u) ? 1.0f : 3.3359f;
     return (l & 1u) ? mag : -mag;
 }
 
-// Sample j of one extract_samples call (one FFT) for VLBA1_1-*-4-2: chunk
-// c = j / 40128, payload segment s (0, 1, or the tail 2) of the chunk, payload
-// index i; negated where modulate[i] is set, zero outside the segment's data
-// zone (vlba_decode_2bit_8track_fanout1_decimation1, mark5_format_vlba.c:
-// 4537-4583).  brow: the FFT's row of segment zones.
+// Sample j of one extract_samples call (one FFT) in a one-byte-per-sample
+// VLBA mode (fanout F = 2^fsh, ntrack = 8F): chunk c = j / (40128 F), payload
+// segment s (0, 1, or the tail 2) of the chunk, payload byte t; frame word
+// m = t / F starting at byte i0 = m F selects modulate[m] (negation) and the
+// blank zone of i0 (zero outside its data range).  The word's bits per mode
+// (vlba_decode_*_decimation1 of mark5_format_vlba.c: 2-bit fanout 1 :4537,
+// 2 :5147, 4 :5907; 1-bit fanout 1 :1362, 2 :2021, 4 :2891), for channel c
+// and sample u = t - i0 of the word:
+//   2-bit F=1: byte 0, sign bit 2c, magnitude 2c+1
+//   2-bit F=2: byte c/2, sign bit 4(c%2)+u, magnitude that + 2
+//   2-bit F=4: byte k of the 32-bit word reordered (reorder32 :74-76),
+//              k = 0, 2, 1, 3 for c = 0..3, sign bit 2u, magnitude 2u+1
+//   1-bit F=1: bit c of byte 0;  F=2: byte c/4, bit 2(c%4)+u;
+//   F=4: byte c%4, bit 2u + c/4;  levels {+1, -1}[bit].
+// brow: the FFT's row of zones, (fsh+1) per payload segment.
 __device__ __forceinline__ float vlba_sample(const uint8_t* __restrict__ raw, const Unpack& up,
                                              const int2* __restrict__ brow, unsigned j)
 {
-    const unsigned c = j / (unsigned)kVlbaChunk;
-    const unsigned r = j - c * (unsigned)kVlbaChunk;
-    const unsigned s = r >= 2u * kVlbaPayload ? 2u : (r >= (unsigned)kVlbaPayload ? 1u : 0u);
-    const int i = (int)(r - s * (unsigned)kVlbaPayload);
-    const int2 z = brow[c * kVlbaSegPerChunk + s];
-    const float lv = vlba_level((raw[j] >> up.shift) & 3u);
-    const bool flip = (up.vmod[i >> 5] >> (i & 31)) & 1u;
-    return (i >= z.x && i < z.y) ? (flip ? -lv : lv) : 0.0f;
+    const int fsh = up.vfsh;
+    const unsigned P = (unsigned)kVlbaPayload << fsh, CH = (unsigned)kVlbaChunk << fsh;
+    const unsigned c = j / CH;
+    const unsigned r = j - c * CH;
+    const unsigned s = r >= 2u * P ? 2u : (r >= P ? 1u : 0u);
+    const unsigned t = r - s * P;
+    const unsigned m = t >> fsh;
+    const unsigned i0 = m << fsh;
+    const unsigned u = t - i0;
+    const int2 z = brow[(c * kVlbaSegPerChunk + s) * (unsigned)(fsh + 1) + (i0 >> 15)];
+    if (!((int)i0 >= z.x && (int)i0 < z.y)) return 0.0f;
+    const uint8_t* __restrict__ w = raw + (j - u);   // the frame word's first byte
+    const unsigned ch = (unsigned)up.cbyte;
+    float lv;
+    if (up.vnbit == 2) {
+        unsigned byte, sb, mb;
+        if (fsh == 0) {
+            byte = w[0];
+            sb = 2u * ch;
+            mb = sb + 1u;
+        } else if (fsh == 1) {
+            byte = w[ch >> 1];
+            sb = 4u * (ch & 1u) + u;
+            mb = sb + 2u;
+        } else {
+            const unsigned x = (unsigned)w[0] | ((unsigned)w[1] << 8) | ((unsigned)w[2] << 16) |
+                               ((unsigned)w[3] << 24);
+            const unsigned bits = (x & 0xAA55AA55u) | ((x & 0x55005500u) >> 7) | ((x & 0x00AA00AAu) << 7);
+            const unsigned k = (ch >> 1) | ((ch & 1u) << 1);   // 0, 2, 1, 3
+            byte = (bits >> (8u * k)) & 0xFFu;
+            sb = 2u * u;
+            mb = sb + 1u;
+        }
+        lv = vlba_level(((byte >> sb) & 1u) | (((byte >> mb) & 1u) << 1));
+    } else {
+        unsigned byte, bit;
+        if (fsh == 0) {
+            byte = w[0];
+            bit = ch;
+        } else if (fsh == 1) {
+            byte = w[ch >> 2];
+            bit = 2u * (ch & 3u) + u;
+        } else {
+            byte = w[ch & 3u];
+            bit = 2u * u + (ch >> 2);
+        }
+        lv = ((byte >> bit) & 1u) ? -1.0f : 1.0f;
+    }
+    const bool flip = (up.vmod[m >> 5] >> (m & 31)) & 1u;
+    return flip ? -lv : lv;
 }
 
 // Framed Mk5B: sample j of the FFT whose payload starts at byte `pay` of the
@@ -1655,6 +1708,9 @@ __device__ __forceinline__ bool vl_fill(const uint8_t* __restrict__ p, long long
     return f;
 }
 
+// One thread per (chunk, payload segment, zone) entry g of the FFT's row:
+// zone z of a payload covers its 64-bit words [4096 z, min(4096 z + 4096,
+// payload / 8)); bounds are in payload bytes.
 __global__ void k_vlba_blank(VbArgs a)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1663,10 +1719,15 @@ __global__ void k_vlba_blank(VbArgs a)
     if (g >= a.nseg || f >= J.nf || J.up.kind != kUnpackVLBA) return;
     const uint8_t* __restrict__ p = job_fft_ptr(J, f);
     const long long avail = J.raw_len - (long long)(p - J.raw);
-    const long long seg = (long long)(g / kVlbaSegPerChunk) * kVlbaChunk + (long long)(g % kVlbaSegPerChunk) * kVlbaPayload;
-    constexpr int nword = kVlbaPayload / 8;
-    const bool start_ok = !vl_fill(p, seg, 8, avail);
-    const bool end_ok = !vl_fill(p, seg + 8LL * (nword - 1), 8, avail);
+    const int fsh = J.up.vfsh, nz = fsh + 1;
+    const int cs = g / nz, zi = g - cs * nz;
+    const long long seg = (long long)(cs / kVlbaSegPerChunk) * ((long long)kVlbaChunk << fsh) +
+                          (long long)(cs % kVlbaSegPerChunk) * ((long long)kVlbaPayload << fsh);
+    const int nword = (kVlbaPayload << fsh) / 8;
+    const int b = 4096 * zi;
+    const int e = min(b + 4096, nword);
+    const bool start_ok = !vl_fill(p, seg + 8LL * b, 8, avail);
+    const bool end_ok = !vl_fill(p, seg + 8LL * (e - 1), 8, avail);
     int vs, ve;
     if (start_ok && end_ok) {
         vs = 0;
@@ -1675,16 +1736,17 @@ __global__ void k_vlba_blank(VbArgs a)
         vs = 1 << 30;
         ve = 0;
     } else {
-        int i = nword / 2, r = nword - 1;
-        for (int s = nword / 4; s > 0; s >>= 1) {
-            if (vl_fill(p, seg + 8LL * i, 8, avail) == start_ok) {
+        const int n = e - b;
+        int i = n / 2, r = n - 1;
+        for (int s = n / 4; s > 0; s >>= 1) {
+            if (vl_fill(p, seg + 8LL * (b + i), 8, avail) == start_ok) {
                 r = i;
                 i -= s;
             } else {
                 i += s;
             }
         }
-        int byte = 8 * r;
+        int byte = 8 * (b + r);
         if (start_ok) {
             if (vl_fill(p, seg + byte - 4, 4, avail)) byte -= 4;
             vs = 0;
@@ -2007,12 +2069,15 @@ hipError_t launch_vlba_blank(const JobDev* jobs, int njobs, int nfft, long long 
 {
     if (njobs < 1 || njobs > kMaxJobs || nfft < 1) return hipErrorInvalidValue;
     VbArgs a;
-    a.nseg = vlba_segments(N);
+    a.nseg = -1;
     bool any = false;
     for (int j = 0; j < njobs; j++) {
         a.jobs[j] = jobs[j];
         if (jobs[j].up.kind == kUnpackVLBA) {
-            if (!jobs[j].up.vblank || jobs[j].up.vnseg != a.nseg) return hipErrorInvalidValue;
+            const long long ns = vlba_segments(N, jobs[j].up.vfsh);
+            if (!jobs[j].up.vblank || jobs[j].up.vnseg != ns || (a.nseg >= 0 && a.nseg != ns))
+                return hipErrorInvalidValue;
+            a.nseg = (int)ns;
             any = true;
         }
     }

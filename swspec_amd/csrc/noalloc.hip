@@ -68,10 +68,11 @@ extern "C" int swspec_hip_unpack_device_noalloc(const swspec_hip_cfg* cfg, const
     const hipStream_t st = (hipStream_t)stream;
     if (up.kind == kUnpackVLBA) {
         // one extract_samples call over `count` bytes: its blank zones see only these bytes
-        const long long nseg = ((count + kVlbaChunk - 1) / kVlbaChunk) * kVlbaSegPerChunk;
+        const long long nseg = vlba_segments(count, up.vfsh);
         if (nseg > kMaxSegments)
             return set_library_error(SWSPEC_EINVAL, "VLBA decode longer than the no-allocation scratch holds "
-                                                    "(at most 130752 payload segments: 1748938752 samples)");
+                                                    "(at most 130752 blank-zone entries: 1748938752 samples at "
+                                                    "fanout 1 or 2, 2331918336 at fanout 4)");
         up.vnseg = (int)nseg;
         uint8_t* base = nullptr;
         if (int rc = scratch_for_device(dev, &base)) return rc;

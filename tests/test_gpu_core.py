@@ -431,6 +431,63 @@ def test_vlba_spectrum_and_pcal_vs_oracle(hip, oracle):
     core.finalize()
 
 
+VLBA_MODES = [(1, 2), (2, 2), (4, 2), (1, 1), (2, 1), (4, 1)]   # (fanout, nbit): one byte per time sample
+
+
+@pytest.mark.parametrize("fanout,nbit", VLBA_MODES)
+def test_unpack_vlba_modes_bitexact(hip, oracle, fanout, nbit):
+    """Every one-byte-per-sample VLBA mode (VLBA1_<fanout>-*-4-2 and -8-1 at
+    fanout 1, 2, 4: 8, 16, 32 tracks), every channel, bit-exact against the
+    oracle over two chunks and a part: fill at the end of a payload (its last
+    blank zone), fill from a chunk's tail across the chunk boundary, and a
+    fill inside a zone (the blanker only tests zone ends: decoded as data)."""
+    import torch
+    nch = 4 if nbit == 2 else 8
+    T = 8 * fanout
+    P, CH = 2500 * T, 5016 * T
+    count = 2 * CH + 3 * P // 2
+    say("unpack vlba fanout %d nbit %d" % (fanout, nbit))
+    fills = [(P - 5000, P), (2 * P + 8, CH + 4096), (CH + P + 33000, CH + P + 34000)]
+    raw = _vlba_bytes(np.random.default_rng(70 + 3 * fanout + nbit), count, fills)
+    s = hip.derive_settings(FFTpoints=40000, FFTIntegrationTimeSec=0.01, FFToverlapFactor=1, BandwidthHz=8e6,
+                            BitsPerSample=nbit, SourceChannels=nch,
+                            SourceFormat="VLBA1_%d-256-%d-%d" % (fanout, nch, nbit))
+    d_raw = torch.from_numpy(raw).cuda()
+    for ch in range(nch):
+        d_out = torch.empty(count, dtype=torch.float32, device="cuda")
+        hip.unpack_device(s, d_raw, ch, count, d_out)
+        ref = oracle.vlba_unpack(raw, count, ch, avail=count, fanout=fanout, nbit=nbit)
+        got = d_out.cpu().numpy()
+        assert np.array_equal(got, ref), (ch, int(np.argmax(got != ref)))
+        assert np.count_nonzero(ref == 0) >= 5000
+
+
+@pytest.mark.parametrize("fanout,nbit", [(2, 2), (4, 1)])
+def test_vlba_modes_spectrum_and_pcal_vs_oracle(hip, oracle, fanout, nbit):
+    """doMaths over 16-track 2-bit and 32-track 1-bit VLBA bytes: overlapping
+    FFTs, phase-cal (generic fold), blank zones inside FFTs."""
+    nch = 4 if nbit == 2 else 8
+    n, nf, fs = 192000, 3, 32e6      # 60 rotator periods of 3200
+    say("vlba spectrum fanout %d nbit %d" % (fanout, nbit))
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=2, WindowType="Hann",
+                            BandwidthHz=fs / 2, BitsPerSample=nbit, SourceChannels=nch,
+                            SourceFormat="VLBA1_%d-512-%d-%d" % (fanout, nch, nbit), ExtractPCal=True,
+                            MaxSourceBufferMB=4096, UseFile1Channel=nch - 1)
+    assert s.extract_pcal == 1
+    nbytes = n * (nf + 2)
+    raw = _vlba_bytes(np.random.default_rng(23 + fanout), nbytes,
+                      [(30000, 42000), (100000, 180000), (250008, 290000)])
+    m, auto, _, pc, core = run_device(s, [raw])
+    os_ = oracle_settings(oracle, s)
+    m64, a64, _, p64 = oracle.domaths(os_, [raw], 64)
+    assert m == m64
+    e = relerr(auto[0], a64[0])
+    ep = relerr(pc[0], p64[0])
+    say("  spectra %d err %.3e  pcal err %.3e" % (m, e, ep))
+    assert e <= TOL and ep <= 1e-5
+    core.finalize()
+
+
 # ---------------------------------------------------------------- Mk5B frames on the device
 
 def test_unpack_mk5b_framed_bitexact(hip, oracle):

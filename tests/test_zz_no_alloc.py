@@ -2,8 +2,8 @@
 allocation (VLBA scratch in a fixed per-GPU array) -- against
 swspec_hip_unpack_device through the C ABI: bit-identical samples on two
 calls in a row, the second with a smaller input whose blank zones differ (the
-array keeps the first call's table), for VLBA1_1-128-4-2 (scratch used) and
-4-channel Mk5B (none); a VLBA call longer than the array holds is refused
+array keeps the first call's table), for VLBA1_1-128-4-2 and VLBA1_4-256-8-1
+(scratch used) and 4-channel Mk5B (none); a VLBA call longer than the array holds is refused
 before any device work.  Bar: bit-exact."""
 import ctypes as C
 
@@ -53,6 +53,16 @@ def test_noalloc_decode_matches_allocating_decode(hip):
             say("vlba call %d ch %d: %d samples, %d blanked" % (k, ch, count, int(np.count_nonzero(ref == 0))))
             assert np.count_nonzero(ref == 0) > 0
             assert np.array_equal(got, ref), (k, ch, int(np.argmax(got != ref)))
+    # 32 tracks (three blank zones per payload), the second call smaller
+    s4 = hip.derive_settings(FFTpoints=40000, FFTIntegrationTimeSec=0.01, FFToverlapFactor=1, BandwidthHz=8e6,
+                             BitsPerSample=1, SourceChannels=8, SourceFormat="VLBA1_4-256-8-1")
+    for k, (count, fills) in enumerate([(2 * 160512 + 4000, [(70000, 80000), (160512 + 2000, 160512 + 40000)]),
+                                        (160512 + 400, [(0, 36000)])]):
+        raw = _vlba_bytes(80 + k, count, fills)
+        for ch in (0, 5):
+            got, ref = _noalloc(hip, s4, raw, ch, count)
+            assert np.count_nonzero(ref == 0) > 0
+            assert np.array_equal(got, ref), (k, ch)
     m = hip.derive_settings(FFTpoints=4000, FFTIntegrationTimeSec=0.01, FFToverlapFactor=1, BandwidthHz=8e6,
                             BitsPerSample=2, SourceChannels=4, SourceFormat="Mk5B")
     for k, count in enumerate((60000, 20000)):
