@@ -146,6 +146,8 @@ def lib():
         L.so_vlba_blank_zone.argtypes = [P, C.c_long, C.c_long, C.c_int, C.c_int, C.POINTER(C.c_int),
                                          C.POINTER(C.c_int)]
         L.so_vlba_blank_zone.restype = None
+        L.so_mark5b_unpack.argtypes = [P, C.c_long, C.c_size_t, C.c_int, C.c_int, C.c_int, P]
+        L.so_mark5b_unpack.restype = C.c_size_t
         _lib = L
     return _lib
 
@@ -169,7 +171,10 @@ def make_ini(**kw):
             # "VLBA1_<fanout>-<Mbps>-<nchan>-<nbit>" carries the fanout
             if v.upper().startswith("VLBA1_"):
                 d["vlba_fanout"] = int(v[6:].split("-")[0])
-            v = FORMATS.get(v, FORMATS["VLBA"] if v.upper().startswith("VLBA") else -1)
+            if v not in FORMATS:
+                v = next((FORMATS[p] for p in ("MKIV", "VLBA", "Mark5B") if v.lower().startswith(p.lower())), -1)
+            else:
+                v = FORMATS[v]
         d[k] = v
     ini = Ini()
     for k, v in d.items():
@@ -385,6 +390,17 @@ def vlba_unpack(raw, count, channel, avail=None, fanout=1, nbit=2):
                                   _p(out))
     if n != count:
         raise ValueError("VLBA mode fanout %d nbit %d not restated" % (fanout, nbit))
+    return out
+
+
+def mark5b_unpack(raw, count, channel, nchan, nbit, avail=None):
+    """Mark5BUnpacker::extract_samples over mark5access's Mark5B decoders
+    (samples 0..15: the fixed +-3.3359 pattern that stands in for std::rand())."""
+    raw = np.ascontiguousarray(raw, dtype=np.uint8)
+    out = np.zeros(count, dtype=np.float32)
+    n = lib().so_mark5b_unpack(_p(raw), len(raw) if avail is None else avail, count, channel, nchan, nbit, _p(out))
+    if n != count:
+        raise ValueError("Mark5B configuration %d channels x %d bits not restated" % (nchan, nbit))
     return out
 
 

@@ -182,6 +182,14 @@ int so_pick_unpacker(const so_settings* s)
             (s->bits_per_sample == 1 && s->source_channels == 8)) return 4;
         return 0;
     }
+    if (s->source_format == SO_FMT_MARK5B) {
+        /* Mark5BUnpacker::canHandleConfig (:755-758): 1 or 2 bits, even
+         * channel count; mark5access takes 1..32 bitstreams (decimation 1) */
+        const int B = s->source_channels * s->bits_per_sample;
+        if ((s->bits_per_sample == 1 || s->bits_per_sample == 2) && (s->source_channels % 2) == 0 &&
+            (B == 2 || B == 4 || B == 8 || B == 16 || B == 32)) return 5;
+        return 0;
+    }
     return 0;
 }
 
@@ -217,6 +225,10 @@ size_t so_unpack(const so_settings* s, const uint8_t* src, float* dst, size_t co
         return smp;
     }
     if (kind == 4) return so_vlba_unpack_mode(src, (long)count, count, channel, s->vlba_fanout, s->bits_per_sample, dst);
+    if (kind == 5) {
+        const long bytes = (long)((count * (size_t)s->source_channels * (size_t)s->bits_per_sample + 7) / 8);
+        return so_mark5b_unpack(src, bytes, count, channel, s->source_channels, s->bits_per_sample, dst);
+    }
     return 0;
 }
 
@@ -892,6 +904,9 @@ int so_domaths(const so_settings* s, int precision,
             if (so_pick_unpacker(s) == 4)   /* the blanker may read past the FFT's bytes */
                 so_vlba_unpack_mode(src[rs], (long)(len[rs] - (size_t)(src[rs] - src_in[rs])), (size_t)N, ch,
                                     s->vlba_fanout, s->bits_per_sample, xs);
+            else if (so_pick_unpacker(s) == 5)
+                so_mark5b_unpack(src[rs], (long)(len[rs] - (size_t)(src[rs] - src_in[rs])), (size_t)N, ch,
+                                 s->source_channels, s->bits_per_sample, xs);
             else
                 so_unpack(s, src[rs], xs, (size_t)N, ch);
             src[rs] += s->raw_overlap_bytes;
@@ -1148,6 +1163,40 @@ size_t so_vlba_unpack_mode(const uint8_t* src, long avail, size_t count, int cha
 size_t so_vlba_unpack(const uint8_t* src, long avail, size_t count, int channel, float* dst)
 {
     return so_vlba_unpack_mode(src, avail, count, channel, 1, 2, dst);
+}
+
+/* Mark5BUnpacker::extract_samples (DataUnpackers.cpp:707-751) over
+ * mark5access's Mark5B decoders (mark5b_decode_<B>bitstream_<nbit>bit_
+ * decimation1, mark5_format_mark5b.c:239-1911): see the header. */
+size_t so_mark5b_unpack(const uint8_t* src, long avail, size_t count, int channel, int nchan, int nbit, float* dst)
+{
+    const int B = nchan * nbit;
+    if ((nbit != 1 && nbit != 2) || (B != 2 && B != 4 && B != 8 && B != 16 && B != 32) || (nchan % 2) != 0) return 0;
+    const int c = channel % nchan;
+    const float himag = 3.3359f;
+    const float lvl[4] = {-himag, 1.0f, -1.0f, himag};
+    long cur = -1;
+    int zs = 0, ze = 0;
+    for (size_t t = 0; t < count; t++) {
+        const unsigned long long p = (unsigned long long)t * (unsigned)B + (unsigned)(c * nbit);
+        const long i0 = (long)(((unsigned long long)t * (unsigned)B) >> 3);   /* the sample's byte group */
+        const long pl = i0 / SO_MARK5B_PAYLOAD;
+        if (pl != cur) {
+            so_vlba_blank_zone(src, avail, pl * SO_MARK5B_PAYLOAD, SO_MARK5B_PAYLOAD, 0, &zs, &ze);
+            cur = pl;
+        }
+        const long ip = i0 - pl * SO_MARK5B_PAYLOAD;
+        float v = 0.0f;
+        if (ip >= zs && ip < ze) {
+            const unsigned byte = src[p >> 3];
+            const unsigned sh = (unsigned)(p & 7u);
+            if (nbit == 2) v = lvl[((byte >> sh) & 1u) + 2u * ((byte >> (sh + 1u)) & 1u)];
+            else v = ((byte >> sh) & 1u) ? -1.0f : 1.0f;
+        }
+        dst[t] = v;
+    }
+    for (size_t n = 0; n < 16 && n < count; n++) dst[n] = (n & 1) ? -himag : himag;
+    return count;
 }
 
 /* ------------------------------------------------------------------------- */

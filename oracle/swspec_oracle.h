@@ -113,6 +113,7 @@ extern const float so_map_rev[4];
  * 1 = TwoBitUnpacker, 2 = TwoBitSinglechannelUnpacker, 3 = Mk5BUnpacker (4/8 ch),
  * 4 = VLBAUnpacker for the modes with one byte per time sample (4 channels x
  *     2 bits or 8 channels x 1 bit at fanout 1, 2 or 4; see so_vlba_unpack_mode),
+ * 5 = Mark5BUnpacker (mark5access Mark5B, decimation 1; see so_mark5b_unpack),
  * 0 = none supported here (8/16-bit, MKIV/Mark5B via mark5access, other VLBA modes). */
 int so_pick_unpacker(const so_settings* s);
 
@@ -269,6 +270,23 @@ void so_vlba_blank_zone(const uint8_t* p, long avail, long seg, int payload, int
  * Returns count, or 0 for a mode outside the six. */
 size_t so_vlba_unpack_mode(const uint8_t* src, long avail, size_t count, int channel, int fanout, int nbit,
                            float* dst);
+
+/* ---- Mark5B through mark5access (Mark5BUnpacker, DataUnpackers.cpp:660-758) --
+ * "Mark5B-<Mbps>-<nchan>-<nbit>", decimation 1, nchan even (canHandleConfig
+ * :755-758), nbitstream B = nchan x nbit in {2, 4, 8, 16, 32}: one
+ * mark5_unpack over the whole request (no chunking, :728-731) on an unpacker
+ * stream without headers, 10000-byte payloads each with one blank zone
+ * (blanker_mark5), no modulation.  Channel c of time sample t reads bit
+ * t*B + c*nbit of the payload (LSB first; 2-bit: that bit is the sign, the
+ * next the magnitude, {-HiMag, +1, -1, +HiMag}[sign + 2 mag], initluts
+ * mark5_format_mark5b.c:59-100; 1-bit {+1, -1}[bit]); a sample whose byte
+ * group (B/8 bytes, or the byte for B < 8) starts outside its zone's data
+ * range is 0.  The reference then overwrites samples 0..15 of every request
+ * with std::rand() signs x 3.3359 (:747-750), a stream no restatement can
+ * follow: here they are the fixed pattern +3.3359, -3.3359, ... (even,
+ * odd sample).  Returns count, or 0 for a configuration outside the above. */
+#define SO_MARK5B_PAYLOAD 10000
+size_t so_mark5b_unpack(const uint8_t* src, long avail, size_t count, int channel, int nchan, int nbit, float* dst);
 
 /* ---- Mk5B framing -----------------------------------------------------------
  * FileSource::read framed mode (src/FileSource.cpp:184-215): strip the 16-byte

@@ -288,8 +288,9 @@ static int pick_unpack(const swspec_hip_cfg* c, int channel, Unpack& up, std::st
 {
     up = Unpack();
     if (channel < 0 || channel >= c->source_channels) { why = "channel out of range"; return -1; }
-    if (c->bits_per_sample != 2 && !(c->bits_per_sample == 1 && c->source_format == SWSPEC_FMT_VLBA)) {
-        why = "only 2-bit samples (and 1-bit VLBA) are implemented";
+    if (c->bits_per_sample != 2 &&
+        !(c->bits_per_sample == 1 && (c->source_format == SWSPEC_FMT_VLBA || c->source_format == SWSPEC_FMT_MARK5B))) {
+        why = "only 2-bit samples (and 1-bit VLBA / Mark5B) are implemented";
         return -1;
     }
     const bool raw = c->source_format == SWSPEC_FMT_RAWSIGNED || c->source_format == SWSPEC_FMT_RAWUNSIGNED;
@@ -351,6 +352,42 @@ static int pick_unpack(const swspec_hip_cfg* c, int channel, Unpack& up, std::st
         up.cbyte = channel % nch;   // allchannels[channel % nchan] (:518)
         return 0;
     }
+    if (c->source_format == SWSPEC_FMT_MARK5B) {
+        // Mark5BUnpacker (:660-758) over mark5access's format string
+        // "Mark5B-<Mbps>-<nchan>-<nbit>[/<decimation>]" (mark5_stream.c:556-571)
+        char fmt[sizeof(c->mark5_format) + 1];
+        std::memcpy(fmt, c->mark5_format, sizeof(c->mark5_format));
+        fmt[sizeof(c->mark5_format)] = 0;
+        int mbps = 0, nch = 0, nbit = 0, dec = 1;
+        const bool prefix = strncasecmp(fmt, "Mark5B-", 7) == 0;
+        const int r = prefix ? sscanf(fmt + 7, "%d-%d-%d/%d", &mbps, &nch, &nbit, &dec) : 0;
+        if (r < 3) { why = "Mark5B format string not understood: '" + std::string(fmt) + "'"; return -1; }
+        const int B = nch * nbit;
+        // canHandleConfig (:755-758): 1 or 2 bits, even channel count; one
+        // extract_samples call decodes the whole FFT, so the bytes it reads
+        // (N*B/8) are the settings' raw bytes per FFT only without decimation
+        if ((nbit != 1 && nbit != 2) || (nch % 2) != 0 || !(B == 2 || B == 4 || B == 8 || B == 16 || B == 32) ||
+            dec != 1) {
+            why = "Mark5B decode implemented for Mark5B-<Mbps>-<nchan>-<nbit> with an even channel count, "
+                  "1 or 2 bits, 2..32 bitstreams and no decimation";
+            return -1;
+        }
+        if (c->source_channels != nch || c->bits_per_sample != nbit) {
+            why = "Mark5B format needs SourceChannels = " + std::to_string(nch) + " and BitsPerSample = " +
+                  std::to_string(nbit);
+            return -1;
+        }
+        if (B < 8 && (c->fft_points % (8 / B)) != 0) {
+            why = "Mark5B FFT length must fill whole bytes";
+            return -1;
+        }
+        up.kind = kUnpackMark5B;
+        up.step = 1;
+        up.vnbit = nbit;
+        up.vbits = B;
+        up.cbyte = channel % nch;   // allchannels[channel % nchan] (:734)
+        return 0;
+    }
     why = "decoder not implemented for this format/channel count";
     return -1;
 }
@@ -381,15 +418,14 @@ static int core_alloc(swspec_hip_core* c)
     if (c->vlba) {
         uint32_t bits[kVlbaModWords];
         vlba_modulation_bits(bits);
-        int fsh = 0;
+        int nseg = 1;
         for (int j = 0; j < c->njobs; j++)
-            if (c->up[j].kind == kUnpackVLBA) fsh = c->up[j].vfsh;
-        const int nseg = (int)vlba_segments(c->cfg.fft_points, fsh);
+            if (is_m5a(c->up[j])) nseg = (int)m5a_segments(c->up[j], c->cfg.fft_points);
         HIPCHK(c, hipMalloc(&c->d_vmod, sizeof(bits)));
         HIPCHK(c, hipMemcpy(c->d_vmod, bits, sizeof(bits), hipMemcpyHostToDevice));
         HIPCHK(c, hipMalloc(&c->d_vblank, (size_t)c->njobs * c->G * nseg * sizeof(int2)));
         for (int j = 0; j < c->njobs; j++) {
-            if (c->up[j].kind != kUnpackVLBA) continue;
+            if (!is_m5a(c->up[j])) continue;
             c->up[j].vmod = c->d_vmod;
             c->up[j].vnseg = nseg;
             c->up[j].vblank = c->d_vblank + (size_t)j * c->G * nseg;
@@ -417,7 +453,7 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
         std::string why;
         c->job_src[j] = job_src[j];
         if (pick_unpack(cfg, job_ch[j], c->up[j], why) != 0) return fail(c, SWSPEC_EUNSUPPORTED, why);
-        c->vlba |= c->up[j].kind == kUnpackVLBA;
+        c->vlba |= is_m5a(c->up[j]);
         if (cfg->flags & SWSPEC_FLAG_MK5B_FRAMED) {
             if (cfg->source_format != SWSPEC_FMT_MK5B || c->up[j].kind != kUnpackMultiChannel)
                 return fail(c, SWSPEC_EUNSUPPORTED, "SWSPEC_FLAG_MK5B_FRAMED needs Mk5B input");
@@ -427,6 +463,7 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
     // bytes consumed per FFT must match the decoder layout
     {
         const long long per = (c->up[0].kind == kUnpackSingleChannel) ? cfg->fft_points / 4
+                              : (c->up[0].kind == kUnpackMark5B)      ? cfg->fft_points * c->up[0].vbits / 8
                                                                        : cfg->fft_points * c->up[0].step;
         if (per != cfg->raw_fullfft_bytes) return fail(c, SWSPEC_EINVAL, "raw_fullfft_bytes does not match format");
         if (c->up[0].kind == kUnpackSingleChannel && (cfg->raw_overlap_bytes * 4) % 2 != 0)
@@ -1101,11 +1138,11 @@ extern "C" int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d
     hipError_t e = hipSuccess;
     uint32_t* d_mod = nullptr;
     int2* d_blank = nullptr;
-    if (up.kind == kUnpackVLBA) {
-        // one extract_samples call over `count` bytes: its blank zones see only these bytes
+    if (is_m5a(up)) {
+        // one extract_samples call over `count` samples: its blank zones see only its bytes
         uint32_t bits[kVlbaModWords];
         vlba_modulation_bits(bits);
-        up.vnseg = (int)vlba_segments(count, up.vfsh);
+        up.vnseg = (int)m5a_segments(up, count);
         e = hipMalloc(&d_mod, sizeof(bits));
         if (e == hipSuccess) e = hipMemcpy(d_mod, bits, sizeof(bits), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMalloc(&d_blank, (size_t)std::max(up.vnseg, 1) * sizeof(int2));
@@ -1113,7 +1150,7 @@ extern "C" int swspec_hip_unpack_device(const swspec_hip_cfg* cfg, const void* d
         up.vblank = d_blank;
         JobDev J;
         J.raw = (const uint8_t*)d_raw;
-        J.raw_len = count;
+        J.raw_len = m5a_bytes(up, count);
         J.nf = 1;
         J.up = up;
         if (e == hipSuccess && count > 0) e = launch_vlba_blank(&J, 1, 1, count, nullptr);

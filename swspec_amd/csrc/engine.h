@@ -25,7 +25,13 @@ enum UnpackKind : int {
     kUnpackMultiChannel = 1,   // TwoBitUnpacker / Mk5BUnpacker: one byte group per time sample
     kUnpackSingleChannel = 2,  // TwoBitSinglechannelUnpacker: 4 samples per byte, oldest in MSB
     kUnpackVLBA = 3,           // VLBAUnpacker, one byte per time sample: VLBA1_<1|2|4>-*-4-2 / -8-1
+    kUnpackMark5B = 4,         // Mark5BUnpacker: mark5access Mark5B bitstreams, decimation 1
 };
+
+// Mark5B through mark5access (Mark5BUnpacker, DataUnpackers.cpp:660-758): one
+// mark5_unpack over the whole FFT, 10000-byte payloads with one blank zone
+// each, no modulation; channel c of time sample t reads bit t*B + c*nbit.
+constexpr int kMark5BPayload = 10000;
 
 // VLBA framing seen by VLBAUnpacker::extract_samples (DataUnpackers.cpp:500-528)
 // for the modes with one byte per time sample (ntrack = 8 x fanout, fanout
@@ -50,7 +56,8 @@ struct Unpack {
     int framed = 0;
     // VLBA only (cbyte holds the channel, 0..nchan-1)
     int vfsh = 0;                        // log2 fanout: 0, 1, 2 (8, 16, 32 tracks)
-    int vnbit = 2;                       // 2 (4 channels) or 1 (8 channels)
+    int vnbit = 2;                       // VLBA: 2 (4 channels) or 1 (8 channels); Mark5B: 1 or 2
+    int vbits = 0;                       // Mark5B: bitstreams B = nchan x nbit (2..32)
     int vnseg = 0;                       // blank-zone entries per FFT (vlba_segments)
     const uint32_t* vmod = nullptr;      // modulation bits, payload index i at bit i
     int2* vblank = nullptr;              // [FFT of the chunk][segment] data zone (start, end)
@@ -71,6 +78,20 @@ inline long long vlba_segments(long long N, int fsh)
 {
     const long long chunk = (long long)kVlbaChunk << fsh;
     return ((N + chunk - 1) / chunk) * kVlbaSegPerChunk * (fsh + 1);
+}
+// Either mark5access decoder (blank-zone tables, generic phase-cal fold).
+__host__ __device__ inline bool is_m5a(const Unpack& up) { return up.kind == kUnpackVLBA || up.kind == kUnpackMark5B; }
+// Blank-zone entries per FFT of N samples: VLBA as above; Mark5B one per
+// 10000-byte payload of the FFT's N*B/8 bytes.
+inline long long m5a_segments(const Unpack& up, long long N)
+{
+    if (up.kind == kUnpackMark5B) return (N * up.vbits / 8 + kMark5BPayload - 1) / kMark5BPayload;
+    return vlba_segments(N, up.vfsh);
+}
+// Bytes of `count` samples of a mark5access decode (VLBA: one per sample).
+inline long long m5a_bytes(const Unpack& up, long long count)
+{
+    return up.kind == kUnpackMark5B ? count * up.vbits / 8 : count;
 }
 // modulate[] of mark5_format_vlba.c:167-196 packed as bits.
 void vlba_modulation_bits(uint32_t* words /* kVlbaModWords */);

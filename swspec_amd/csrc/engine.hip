@@ -450,6 +450,38 @@ __device__ __forceinline__ float vlba_sample(const uint8_t* __restrict__ raw, co
     return flip ? -lv : lv;
 }
 
+// Sample j of one Mark5BUnpacker::extract_samples call (one FFT) through
+// mark5access's decimation-1 Mark5B decoders (mark5_format_mark5b.c:239-
+// 1911): bit p = j*B + c*nbit of the FFT's bytes (LSB first), 2-bit levels
+// {-HiMag, +1, -1, +HiMag}[bit p + 2 bit p+1] (initluts :59-100), 1-bit
+// {+1, -1}[bit p]; zero when the sample's byte group (byte j*B/8) lies
+// outside its 10000-byte payload's data zone.  Samples 0..15 of the call are
+// the fixed pattern +3.3359, -3.3359, ... where the reference writes
+// std::rand() signs (DataUnpackers.cpp:747-750), a stream no other program
+// reproduces.
+__device__ __forceinline__ float m5b_sample(const uint8_t* __restrict__ raw, const Unpack& up,
+                                            const int2* __restrict__ brow, unsigned j)
+{
+    if (j < 16u) return (j & 1u) ? -3.3359f : 3.3359f;
+    const unsigned long long p = (unsigned long long)j * (unsigned)up.vbits + (unsigned)(up.cbyte * up.vnbit);
+    const unsigned i0 = (unsigned)(((unsigned long long)j * (unsigned)up.vbits) >> 3);
+    const unsigned pl = i0 / (unsigned)kMark5BPayload;
+    const int ip = (int)(i0 - pl * (unsigned)kMark5BPayload);
+    const int2 z = brow[pl];
+    if (!(ip >= z.x && ip < z.y)) return 0.0f;
+    const unsigned byte = raw[p >> 3];
+    const unsigned sh = (unsigned)(p & 7u);
+    if (up.vnbit == 2) return vlba_level(((byte >> sh) & 1u) | (((byte >> (sh + 1u)) & 1u) << 1));
+    return ((byte >> sh) & 1u) ? -1.0f : 1.0f;
+}
+
+// Either mark5access decoder.
+__device__ __forceinline__ float m5a_sample(const uint8_t* __restrict__ raw, const Unpack& up,
+                                            const int2* __restrict__ brow, unsigned j)
+{
+    return up.kind == kUnpackMark5B ? m5b_sample(raw, up, brow, j) : vlba_sample(raw, up, brow, j);
+}
+
 // Framed Mk5B: sample j of the FFT whose payload starts at byte `pay` of the
 // header-stripped stream; raw is the first frame.
 __device__ __forceinline__ unsigned field2_framed(const uint8_t* __restrict__ raw, const Unpack& up, long long pay,
@@ -463,7 +495,7 @@ __device__ __forceinline__ unsigned field2_framed(const uint8_t* __restrict__ ra
 __device__ __forceinline__ float decode_sample(const uint8_t* __restrict__ raw, const Unpack& up,
                                                const int2* __restrict__ brow, long long pay, long long j)
 {
-    if (up.kind == kUnpackVLBA) return vlba_sample(raw, up, brow, (unsigned)j);
+    if (is_m5a(up)) return m5a_sample(raw, up, brow, (unsigned)j);
     if (up.framed) return map2bit(field2_framed(raw, up, pay, j));
     return map2bit(field2(raw, up, j));
 }
@@ -715,8 +747,7 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     const int total = N1 * B;
     const int estep = nth / B;
     // MODE 3 (VLBA): samples decoded through the FFT's row of blank zones
-    const int2* __restrict__ brow =
-        up.kind == kUnpackVLBA ? up.vblank + (size_t)(active ? f : 0) * up.vnseg : nullptr;
+    const int2* __restrict__ brow = is_m5a(up) ? up.vblank + (size_t)(active ? f : 0) * up.vnseg : nullptr;
     auto fill = [&](auto mode, auto wmc) __attribute__((always_inline)) {
         constexpr int MODE = decltype(mode)::value;
         constexpr int WM = decltype(wmc)::value;
@@ -744,8 +775,8 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
                 const int e = FULL ? e0 + u * estep : min(e0 + u * estep, N1 - 1);
                 const unsigned n = __umul24((unsigned)N2, (unsigned)e) + (unsigned)cb;
                 if constexpr (MODE == 3) {
-                    x0[u] = vlba_sample(raw, up, brow, 2u * n);
-                    x1[u] = vlba_sample(raw, up, brow, 2u * n + 1u);
+                    x0[u] = m5a_sample(raw, up, brow, 2u * n);
+                    x1[u] = m5a_sample(raw, up, brow, 2u * n + 1u);
                 } else if constexpr (MODE == 4) {
                     x0[u] = map2bit(field2_framed(raw, up, pay, 2LL * n));
                     x1[u] = map2bit(field2_framed(raw, up, pay, 2LL * n + 1));
@@ -797,7 +828,7 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     };
     if (active) {
         if (up.kind == kUnpackSingleChannel) fill_w(I0());
-        else if (up.kind == kUnpackVLBA) fill(std::integral_constant<int, 3>(), I2());
+        else if (is_m5a(up)) fill(std::integral_constant<int, 3>(), I2());
         else if (up.framed) fill_w(std::integral_constant<int, 4>());   // same window code as mode 1
         else if (up.step == 1 && even) fill_w(I1());
         else fill_w(I2());
@@ -1425,7 +1456,7 @@ __global__ void __launch_bounds__(1024) k_pcal(PcArgs a)
         const Unpack up = J.up;
         const uint8_t* __restrict__ raw = up.framed ? J.raw : job_fft_ptr(J, f);
         const long long pay = J.pay0 + (long long)f * J.fft_stride;   // framed only
-        const int2* __restrict__ brow = up.kind == kUnpackVLBA ? up.vblank + (size_t)f * up.vnseg : nullptr;
+        const int2* __restrict__ brow = is_m5a(up) ? up.vblank + (size_t)f * up.vnseg : nullptr;
         const long long s0 = (long long)blk * a.g.chunk;
         const long long s1 = min(s0 + (long long)a.g.chunk, a.N);
         int ri = (int)((s0 + tid) % L);
@@ -1716,14 +1747,23 @@ __global__ void k_vlba_blank(VbArgs a)
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     const int f = blockIdx.y;
     const JobDev& J = a.jobs[blockIdx.z];
-    if (g >= a.nseg || f >= J.nf || J.up.kind != kUnpackVLBA) return;
+    if (g >= a.nseg || f >= J.nf || !is_m5a(J.up)) return;
     const uint8_t* __restrict__ p = job_fft_ptr(J, f);
     const long long avail = J.raw_len - (long long)(p - J.raw);
-    const int fsh = J.up.vfsh, nz = fsh + 1;
-    const int cs = g / nz, zi = g - cs * nz;
-    const long long seg = (long long)(cs / kVlbaSegPerChunk) * ((long long)kVlbaChunk << fsh) +
-                          (long long)(cs % kVlbaSegPerChunk) * ((long long)kVlbaPayload << fsh);
-    const int nword = (kVlbaPayload << fsh) / 8;
+    long long seg;
+    int nword, zi;
+    if (J.up.kind == kUnpackMark5B) {   // payload g of the FFT, one zone
+        seg = (long long)g * kMark5BPayload;
+        nword = kMark5BPayload / 8;
+        zi = 0;
+    } else {
+        const int fsh = J.up.vfsh, nz = fsh + 1;
+        const int cs = g / nz;
+        zi = g - cs * nz;
+        seg = (long long)(cs / kVlbaSegPerChunk) * ((long long)kVlbaChunk << fsh) +
+              (long long)(cs % kVlbaSegPerChunk) * ((long long)kVlbaPayload << fsh);
+        nword = (kVlbaPayload << fsh) / 8;
+    }
     const int b = 4096 * zi;
     const int e = min(b + 4096, nword);
     const bool start_ok = !vl_fill(p, seg + 8LL * b, 8, avail);
@@ -1977,7 +2017,7 @@ hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int n
                                jobs[j].fft_stride % 8 == 0 && jobs[j].raw_len < (1LL << 32) &&
                                jobs[j].pay0 + (long long)nfft * jobs[j].fft_stride + N * st < (1LL << 32);
         framed |= jobs[j].up.framed != 0;
-        if (jobs[j].up.kind == kUnpackVLBA || (jobs[j].up.framed && !framed_ok)) step = -2;   // generic fold
+        if (is_m5a(jobs[j].up) || (jobs[j].up.framed && !framed_ok)) step = -2;   // generic fold
         else if (step == -1) step = st;
         else if (step != st) step = -2;
     }
@@ -2073,8 +2113,8 @@ hipError_t launch_vlba_blank(const JobDev* jobs, int njobs, int nfft, long long 
     bool any = false;
     for (int j = 0; j < njobs; j++) {
         a.jobs[j] = jobs[j];
-        if (jobs[j].up.kind == kUnpackVLBA) {
-            const long long ns = vlba_segments(N, jobs[j].up.vfsh);
+        if (is_m5a(jobs[j].up)) {
+            const long long ns = m5a_segments(jobs[j].up, N);
             if (!jobs[j].up.vblank || jobs[j].up.vnseg != ns || (a.nseg >= 0 && a.nseg != ns))
                 return hipErrorInvalidValue;
             a.nseg = (int)ns;

@@ -1,8 +1,8 @@
 // noalloc.hip -- the decoder entry point for callers that cannot allocate.
 //
 // swspec_hip_unpack_device_noalloc is swspec_hip_unpack_device without any
-// device allocation: the VLBA decode's scratch (the modulation bits of a
-// 20000-byte payload and the blank-zone table of the call's payload
+// device allocation: the mark5access decodes' scratch (the VLBA modulation
+// bits of a 20000-word payload and the blank-zone table of the call's payload
 // segments, mark5_format_vlba.c:167-196 and blanker_mark5.c:126-200) lives in
 // the file-scope array `scratch` below, one copy per GPU, held until the
 // process ends.  Host code never passes the array's name to a kernel: each
@@ -66,13 +66,14 @@ extern "C" int swspec_hip_unpack_device_noalloc(const swspec_hip_cfg* cfg, const
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return set_library_error(SWSPEC_EDEVICE, "hipGetDevice failed");
     const hipStream_t st = (hipStream_t)stream;
-    if (up.kind == kUnpackVLBA) {
-        // one extract_samples call over `count` bytes: its blank zones see only these bytes
-        const long long nseg = vlba_segments(count, up.vfsh);
+    if (is_m5a(up)) {
+        // one extract_samples call over `count` samples: its blank zones see only its bytes
+        const long long nseg = m5a_segments(up, count);
         if (nseg > kMaxSegments)
-            return set_library_error(SWSPEC_EINVAL, "VLBA decode longer than the no-allocation scratch holds "
-                                                    "(at most 130752 blank-zone entries: 1748938752 samples at "
-                                                    "fanout 1 or 2, 2331918336 at fanout 4)");
+            return set_library_error(SWSPEC_EINVAL, "decode longer than the no-allocation scratch holds "
+                                                    "(at most 130752 blank-zone entries: VLBA 1748938752 samples "
+                                                    "at fanout 1 or 2, 2331918336 at fanout 4; Mark5B "
+                                                    "1307520000 bytes)");
         up.vnseg = (int)nseg;
         uint8_t* base = nullptr;
         if (int rc = scratch_for_device(dev, &base)) return rc;
@@ -80,7 +81,7 @@ extern "C" int swspec_hip_unpack_device_noalloc(const swspec_hip_cfg* cfg, const
         up.vblank = (int2*)(base + kModBytes);
         JobDev J;
         J.raw = (const uint8_t*)d_raw;
-        J.raw_len = count;
+        J.raw_len = m5a_bytes(up, count);
         J.nf = 1;
         J.up = up;
         if (count > 0) {

@@ -488,6 +488,61 @@ def test_vlba_modes_spectrum_and_pcal_vs_oracle(hip, oracle, fanout, nbit):
     core.finalize()
 
 
+# ---------------------------------------------------------------- Mark5B (mark5access bitstreams)
+
+MARK5B_CONFIGS = [(2, 1), (4, 1), (8, 1), (16, 1), (32, 1), (2, 2), (4, 2), (8, 2), (16, 2)]   # (nchan, nbit)
+
+
+@pytest.mark.parametrize("nchan,nbit", MARK5B_CONFIGS)
+def test_unpack_mark5b_bitexact(hip, oracle, nchan, nbit):
+    """Mark5BUnpacker over mark5access (Mark5B-<Mbps>-<nchan>-<nbit>), every
+    bitstream count the reference admits, bit-exact against the oracle over
+    four 10000-byte payloads, with a fill zone at the end of one payload and
+    over the start of the next (blanked from the data side in each zone)."""
+    import torch
+    B = nchan * nbit
+    count = (4 * 10000 * 8) // B + 8 * 16
+    nbytes = count * B // 8
+    say("unpack mark5b %d ch x %d bit" % (nchan, nbit))
+    raw = _vlba_bytes(np.random.default_rng(90 + B + nbit), nbytes, [(17000, 20000), (30000, 33000)])
+    s = hip.derive_settings(FFTpoints=40000, FFTIntegrationTimeSec=0.01, FFToverlapFactor=1, BandwidthHz=8e6,
+                            BitsPerSample=nbit, SourceChannels=nchan,
+                            SourceFormat="Mark5B-%d-%d-%d" % (256 * B // 8, nchan, nbit))
+    d_raw = torch.from_numpy(raw).cuda()
+    for ch in sorted({0, 1, nchan // 2, nchan - 1}):
+        d_out = torch.empty(count, dtype=torch.float32, device="cuda")
+        hip.unpack_device(s, d_raw, ch, count, d_out)
+        ref = oracle.mark5b_unpack(raw, count, ch, nchan, nbit, avail=nbytes)
+        got = d_out.cpu().numpy()
+        assert np.array_equal(got, ref), (ch, int(np.argmax(got != ref)))
+        assert np.count_nonzero(ref == 0) > 0
+
+
+@pytest.mark.parametrize("nchan,nbit", [(4, 2), (16, 1)])
+def test_mark5b_spectrum_and_pcal_vs_oracle(hip, oracle, nchan, nbit):
+    """doMaths over Mark5B bitstream bytes with overlapping FFTs, phase-cal
+    (generic fold) and blank zones inside FFTs."""
+    n, nf, fs = 64000, 3, 32e6
+    B = nchan * nbit
+    say("mark5b spectrum %d ch x %d bit" % (nchan, nbit))
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=2, WindowType="Hann",
+                            BandwidthHz=fs / 2, BitsPerSample=nbit, SourceChannels=nchan,
+                            SourceFormat="Mark5B-%d-%d-%d" % (B * 32, nchan, nbit), ExtractPCal=True,
+                            MaxSourceBufferMB=4096, UseFile1Channel=nchan - 1)
+    assert s.extract_pcal == 1 and s.raw_fullfft_bytes == n * B // 8
+    nbytes = n * B // 8 * (nf + 2)
+    raw = _vlba_bytes(np.random.default_rng(31 + B), nbytes, [(5000, 10000), (26000, 31000), (60008, 70000)])
+    m, auto, _, pc, core = run_device(s, [raw])
+    os_ = oracle_settings(oracle, s)
+    m64, a64, _, p64 = oracle.domaths(os_, [raw], 64)
+    assert m == m64
+    e = relerr(auto[0], a64[0])
+    ep = relerr(pc[0], p64[0])
+    say("  spectra %d err %.3e  pcal err %.3e" % (m, e, ep))
+    assert e <= TOL and ep <= 1e-5
+    core.finalize()
+
+
 # ---------------------------------------------------------------- Mk5B frames on the device
 
 def test_unpack_mk5b_framed_bitexact(hip, oracle):

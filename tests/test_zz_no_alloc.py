@@ -2,8 +2,8 @@
 allocation (VLBA scratch in a fixed per-GPU array) -- against
 swspec_hip_unpack_device through the C ABI: bit-identical samples on two
 calls in a row, the second with a smaller input whose blank zones differ (the
-array keeps the first call's table), for VLBA1_1-128-4-2 and VLBA1_4-256-8-1
-(scratch used) and 4-channel Mk5B (none); a VLBA call longer than the array holds is refused
+array keeps the first call's table), for VLBA1_1-128-4-2, VLBA1_4-256-8-1 and
+Mark5B-1024-8-2 (scratch used) and 4-channel Mk5B (none); a VLBA call longer than the array holds is refused
 before any device work.  Bar: bit-exact."""
 import ctypes as C
 
@@ -63,6 +63,14 @@ def test_noalloc_decode_matches_allocating_decode(hip):
             got, ref = _noalloc(hip, s4, raw, ch, count)
             assert np.count_nonzero(ref == 0) > 0
             assert np.array_equal(got, ref), (k, ch)
+    # Mark5B bitstreams (one blank zone per 10000-byte payload)
+    s5 = hip.derive_settings(FFTpoints=40000, FFTIntegrationTimeSec=0.01, FFToverlapFactor=1, BandwidthHz=8e6,
+                             BitsPerSample=2, SourceChannels=8, SourceFormat="Mark5B-1024-8-2")
+    for k, (count, fills) in enumerate([(20000, [(25000, 30000)]), (6000, [(0, 4096)])]):
+        raw = _vlba_bytes(90 + k, 2 * count, fills)
+        got, ref = _noalloc(hip, s5, raw, 3, count)
+        assert np.count_nonzero(ref == 0) > 0
+        assert np.array_equal(got, ref), k
     m = hip.derive_settings(FFTpoints=4000, FFTIntegrationTimeSec=0.01, FFToverlapFactor=1, BandwidthHz=8e6,
                             BitsPerSample=2, SourceChannels=4, SourceFormat="Mk5B")
     for k, count in enumerate((60000, 20000)):
