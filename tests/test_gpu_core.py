@@ -106,6 +106,39 @@ def test_multichannel_mk5b_and_pcal(hip, oracle):
     core.finalize()
 
 
+@pytest.mark.parametrize("nch,channels,framed", [(8, list(range(8)), False), (4, [1, 3], False),
+                                                 (4, [0, 1, 2, 3], True), (8, [4, 6], True)])
+def test_channel_subsets_vs_oracle(hip, oracle, nch, channels, framed):
+    """Several channels of one stream in one core (jobs sharing the stream's
+    bytes in the same launches): all 8 channels of 8-channel Mk5B (bytes 0
+    and 1 of each time sample), a pair of channels, and framed input; every
+    channel's spectrum and phase-cal against the oracle, and within 1e-6
+    (phase-cal bit for bit) of the channel run alone."""
+    from swspec_amd._lib import FLAG_MK5B_FRAMED
+    n, nf, fs = 320000, 3, 32e6
+    say("channel groups", nch, channels, framed)
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=1,
+                            WindowType="Cosine2", BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=nch,
+                            SourceFormat="Mk5B", ExtractPCal=True, MaxSourceBufferMB=4096)
+    rng = np.random.default_rng(50 + nch + len(channels))
+    tones = [synth.pcal_comb(fs, fs / 2, amp=0.15, rng=rng) for _ in range(nch)]
+    payload = synth.raw_multi(5, nch, n * nf, tones_per_ch=tones, fs=fs)
+    data = synth.mk5b_frames(payload) if framed else payload
+    flags = FLAG_MK5B_FRAMED if framed else 0
+    m, auto, _, pc, core = run_device(s, [data], channels=channels, flags=flags)
+    core.finalize()
+    assert m == 1
+    for j, ch in enumerate(channels):
+        sc = hip.derive_settings(**dict(s.keys, UseFile1Channel=ch + 1))
+        _, a64, _, p64 = oracle.domaths(oracle_settings(oracle, sc), [payload], 64)
+        e, ep = relerr(auto[j], a64[0]), relerr(pc[j], p64[0])
+        m1, a1, _, p1, c1 = run_device(sc, [data], flags=flags)
+        c1.finalize()
+        say("  ch%d spectrum %.3e pcal %.3e" % (ch, e, ep))
+        assert e <= TOL and ep <= 1e-5
+        assert relerr(auto[j], a1[0]) <= 1e-6 and np.array_equal(pc[j], p1[0])
+
+
 @pytest.mark.parametrize("fmt,nch,incr,bw,n", [
     ("RawSigned", 1, 1, 16e6, 32000),     # fast fold, 4 samples per byte
     ("RawSigned", 8, 0, 16e6, 6400),      # fast fold, 2 bytes per time sample
