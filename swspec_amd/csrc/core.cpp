@@ -401,7 +401,8 @@ static int core_alloc(swspec_hip_core* c)
     const Plan& p = c->plan;
     const size_t M1 = (size_t)p.M + 1;
     const size_t S = (size_t)p.smax;   // accumulator slices per job
-    HIPCHK(c, hipMalloc(&c->d_Y, (size_t)c->ybufs * c->njobs * c->G * (size_t)p.M * sizeof(float2)));
+    if (!p.single)   // single-pass plans keep no intermediate
+        HIPCHK(c, hipMalloc(&c->d_Y, (size_t)c->ybufs * c->njobs * c->G * (size_t)p.M * sizeof(float2)));
     HIPCHK(c, hipMalloc(&c->d_acc, (size_t)c->njobs * S * 2 * M1 * sizeof(float)));
     HIPCHK(c, (hipError_t)launch_zero(c->d_acc, (size_t)c->njobs * S * 2 * M1 * sizeof(float), c->stream));
     if (c->nx) {
@@ -485,10 +486,14 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
     if (!e.empty()) return fail(c, SWSPEC_EUNSUPPORTED, e);
 
     const long long M = c->plan.M;
-    // FFTs per chunk: phase-cal masks hold 64 FFTs; without phase-cal up to 1024
-    const int gcap = cfg->extract_pcal ? 64 : 1024;
+    // FFTs per chunk: phase-cal masks hold 64 FFTs; without phase-cal up to
+    // 1024 (single-pass plans, which keep no intermediate: 65536)
+    const bool single = c->plan.single != 0;
+    const int gcap = cfg->extract_pcal ? 64 : (single ? 65536 : 1024);
     if (cfg->ffts_per_chunk > 0) {
         c->G = (int)std::min<uint32_t>(cfg->ffts_per_chunk, (uint32_t)gcap);
+    } else if (single) {
+        c->G = gcap;
     } else {
         // chunk intermediate up to ~1 GB: the 3.2M-point four-channel case
         // (20 FFTs per chunk) measured fastest there; fewer accumulator passes
@@ -505,6 +510,7 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
         // stream.  SWSPEC_ROW_STREAM=1/0 forces it on/off.
         bool on = c->njobs == 1 && cfg->core_overlapped_ffts > c->G;
         if (const char* e = getenv("SWSPEC_ROW_STREAM")) on = atoi(e) != 0;
+        if (single) on = false;   // one pass: nothing to overlap
         if (on) {
             HIPCHK(c, hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking));   // default priority
             for (hipEvent_t* ev : {&c->ev_p1[0], &c->ev_p1[1], &c->ev_p2[0], &c->ev_p2[1]})
@@ -713,7 +719,7 @@ static int run_common(swspec_hip_core* c, const uint8_t* const* d_in, size_t len
                 J.fft_stride = g.raw_overlap_bytes;
                 J.nf = nf;
                 J.up = c->up[j];
-                J.Y = c->d_Y + ((size_t)yb * c->njobs + j) * c->G * (size_t)p.M;
+                J.Y = p.single ? nullptr : c->d_Y + ((size_t)yb * c->njobs + j) * c->G * (size_t)p.M;
                 J.acc = c->d_acc + (size_t)j * p.smax * 2 * M1;
                 if (g.extract_pcal) {
                     J.pcal_part = c->d_pcal_part + (size_t)j * c->G * c->pg.nblk_max * c->pg.T * 2;
@@ -1168,9 +1174,10 @@ extern "C" int swspec_hip_plan_info(int64_t fft_points, uint32_t flags, int32_t*
     Plan p;
     std::string e = plan_geometry(p, fft_points, (flags & SWSPEC_FLAG_MFMA64) != 0);
     if (!e.empty()) return fail(nullptr, SWSPEC_EUNSUPPORTED, e);
-    int32_t v[14] = {p.N1,        p.N2,        p.B,      p.p1_threads, p.sp2.T * 2, p.npairs,    p.p1_maxpts,
-                     p.p2_maxpts, p.p1_cx,     p.p2_cx,  p.sp1.nst,    p.sp2.nst,   p.mfma64,    p.sp1.radix[0]};
-    for (int i = 0; i < ninfo && i < 14; i++) info[i] = v[i];
+    int32_t v[15] = {p.N1,        p.N2,        p.B,      p.p1_threads, p.sp2.T * 2, p.npairs,    p.p1_maxpts,
+                     p.p2_maxpts, p.p1_cx,     p.p2_cx,  p.sp1.nst,    p.sp2.nst,   p.mfma64,    p.sp1.radix[0],
+                     p.single};
+    for (int i = 0; i < ninfo && i < 15; i++) info[i] = v[i];
     return SWSPEC_OK;
 }
 

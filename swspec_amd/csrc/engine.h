@@ -135,6 +135,7 @@ struct Plan {
     int p1_cx = 0, p2_cx = 0;    // sub-plan uses composite register radices (10..32)
     int p1_fpw = 1;          // FFT pipelines per pass-1 workgroup (short columns)
     int mfma64 = 0;          // pass 1 opens with the radix-64 split-FP16 MFMA stage
+    int single = 0;          // short FFTs in one pass (k_short): N1 = M, N2 = 1, sp1 the M-point FFT
     int smax = 1;            // accumulator slices per job (pass-2 FFT parallelism)
     int npairs = 0;          // pass-2 workgroups per FFT (row pairs)
     int twshift = 0;         // four-step twiddle split W_M^e = A[e>>s] * B[e & (2^s-1)]
@@ -160,6 +161,7 @@ struct PlanKnobs {
     int p1_bmax = 0;     // upper bound on the columns per workgroup
     int cx_mask = -1;    // composite radices: bit 0 column pass, bit 1 row pass
     int p2_maxpts = 0;   // row pass points per thread: 16 or 32
+    int single = -1;     // short FFTs: 0 forces two passes (-1: the planner's choice)
 };
 // Measured choices per FFT length (swspec_hip_autotune); plan_geometry uses
 // them when no knobs are passed.  Process-wide, thread-safe.

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <set>
 #include <type_traits>
 
 namespace swspec {
@@ -683,46 +684,27 @@ __device__ __forceinline__ void xcd_remap(unsigned& x, unsigned& y, unsigned& z,
     z = yz / gy;
 }
 
-// MF: the first column stage is radix 64 on the matrix cores (stage_mfma64;
-// one pipeline per workgroup, whole waves)
-template <int MAXPTS, bool CX, bool MF = false>
-__global__ void __launch_bounds__(512) k_pass1(P1Args a)
-{
-    extern __shared__ float2 lds[];
-    unsigned bx, by, bz;
-    xcd_remap(bx, by, bz, a.zfast);
-    const JobDev J = a.jobs[bz];
-    // a workgroup runs `fpw` independent pipelines, one FFT each (short columns)
-    const int fpw = a.fpw, pth = a.pth;
-    if ((int)by * fpw >= J.nf) return;   // no FFT of this job here (whole workgroup)
-    int pipe = 0;
-    while (pipe + 1 < fpw && threadIdx.x >= (unsigned)((pipe + 1) * pth)) pipe++;
-    const int f = (int)by * fpw + pipe;
-    const bool active = f < J.nf;
-    const int B = a.B, N1 = a.N1, N2 = a.N2;
-    const int c0 = (int)bx * B;
-    const int tid = threadIdx.x - pipe * pth, nth = pth;
-    float2* __restrict__ lp = lds + pipe * (N1 * B);
-    const Unpack up = J.up;
-    const uint8_t* __restrict__ raw = up.framed ? J.raw : job_fft_ptr(J, active ? f : 0);
-    const long long pay = J.pay0 + (long long)(active ? f : 0) * J.fft_stride;   // framed only
-    const bool even = ((uintptr_t)raw & 1) == 0;
+// Window on the fly: real sample j = 2(N2 e + c) + r has angle
+// theta = phi_e + psi_r, phi_e = 2 pi (2 N2 e) / D from a per-row table and
+// psi_r = 2 pi (2c + r + delta) / D fixed per thread (its column c never
+// changes: nth is a multiple of B).  Expanding the cosine sums,
+//   w_r = a0 + cos(phi) P_r + sin(phi) Q_r + cos(2 phi) R_r + sin(2 phi) S_r
+//   P = a1 cos psi + a2 sin psi,  Q = a2 cos psi - a1 sin psi,
+//   R = a3 cos 2psi,              S = -a3 sin 2psi
+// (per thread, FP64, rounded once).  phi is linear in the row, so the rows
+// e0 + u*estep of a batch take (cos, sin)(phi) as the product of the table
+// entries of e0 and of u*estep: one product each, nothing accumulates.
+struct WinCoef {
+    float P[2], Q[2], R2[2], S2[2];
+    float a0;
+    int mode;   // 0: no window (w = 1, no arithmetic), 1: no cos(2 theta) term, 2: all terms
+};
 
-    // Window on the fly: real sample j = 2(N2 e + c) + r has angle
-    // theta = phi_e + psi_r, phi_e = 2 pi (2 N2 e) / D from a per-row table and
-    // psi_r = 2 pi (2c + r + delta) / D fixed per thread (its column c never
-    // changes: nth is a multiple of B).  Expanding the cosine sums,
-    //   w_r = a0 + cos(phi) P_r + sin(phi) Q_r + cos(2 phi) R_r + sin(2 phi) S_r
-    //   P = a1 cos psi + a2 sin psi,  Q = a2 cos psi - a1 sin psi,
-    //   R = a3 cos 2psi,              S = -a3 sin 2psi
-    // (per thread, FP64, rounded once).  phi is linear in the row, so the rows
-    // e0 + u*estep of a batch take (cos, sin)(phi) as the product of the table
-    // entries of e0 and of u*estep: one product each, nothing accumulates.
-    const WindowGen wg = a.win;
-    const int et = tid / B;            // the thread's first row (and first butterfly)
-    const int bcol = tid - et * B;     // its column, fixed
-    const int cb = c0 + bcol;
-    float P[2] = {0.f, 0.f}, Q[2] = {0.f, 0.f}, R2[2] = {0.f, 0.f}, S2[2] = {0.f, 0.f};
+__device__ __forceinline__ WinCoef win_coef(const WindowGen& wg, int cb)
+{
+    WinCoef w;
+#pragma unroll
+    for (int r = 0; r < 2; r++) w.P[r] = w.Q[r] = w.R2[r] = w.S2[r] = 0.f;
     if (wg.kind != 0) {
 #pragma unroll
         for (int r = 0; r < 2; r++) {
@@ -730,24 +712,34 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
             double s, c;
             sincospi(2.0 * x, &s, &c);
             const double c2 = c * c - s * s, s2 = 2.0 * s * c;
-            P[r] = (float)((double)wg.a1 * c + (double)wg.a2 * s);
-            Q[r] = (float)((double)wg.a2 * c - (double)wg.a1 * s);
-            R2[r] = (float)((double)wg.a3 * c2);
-            S2[r] = (float)(-(double)wg.a3 * s2);
+            w.P[r] = (float)((double)wg.a1 * c + (double)wg.a2 * s);
+            w.Q[r] = (float)((double)wg.a2 * c - (double)wg.a1 * s);
+            w.R2[r] = (float)((double)wg.a3 * c2);
+            w.S2[r] = (float)(-(double)wg.a3 * s2);
         }
     }
-    const float wa0 = wg.a0;
-    // 0: no window (w = 1, no arithmetic), 1: no cos(2 theta) term, 2: all terms
-    const int wmode = wg.kind == 0 ? 0 : (wg.a3 == 0.f ? 1 : 2);
+    w.a0 = wg.a0;
+    w.mode = wg.kind == 0 ? 0 : (wg.a3 == 0.f ? 1 : 2);
+    return w;
+}
 
-    // lp[e*B + b] = z[N2*e + c0 + b] windowed; idx == e*B + b.  A thread keeps
-    // its column b = bcol; its row e advances by estep per strided step.  Rows
-    // are clamped to N1 - 1 for the loads of a partial batch (values unused).
-    // Point indices are < M <= 2^30 and the factors < 2^14: 24-bit products.
+// Decode + window of one FFT's points into the column layout:
+// lp[e*B + b] = z[N2*e + c0 + b] windowed; idx == e*B + b.  A thread keeps
+// its column b = bcol (point column cb = c0 + bcol); its row e starts at et
+// and advances by estep = nth / B per strided step.  Rows are clamped to
+// N1 - 1 for the loads of a partial batch (values unused).  Point indices
+// are < M <= 2^30 and the factors < 2^14: 24-bit products.  raw is the FFT's
+// first byte (framed Mk5B: the first frame, payload offset pay); brow its
+// row of blank zones (mark5access decodes).
+template <bool MF>
+__device__ __forceinline__ void fill_points(float2* __restrict__ lp, const uint8_t* __restrict__ raw,
+                                            const Unpack& up, long long pay, const int2* __restrict__ brow,
+                                            const WindowGen& wg, const WinCoef& wc, int tid, int nth, int B,
+                                            int N1, int N2, int cb, int et)
+{
     const int total = N1 * B;
     const int estep = nth / B;
-    // MODE 3 (VLBA): samples decoded through the FFT's row of blank zones
-    const int2* __restrict__ brow = is_m5a(up) ? up.vblank + (size_t)(active ? f : 0) * up.vnseg : nullptr;
+    const bool even = ((uintptr_t)raw & 1) == 0;
     auto fill = [&](auto mode, auto wmc) __attribute__((always_inline)) {
         constexpr int MODE = decltype(mode)::value;
         constexpr int WM = decltype(wmc)::value;
@@ -800,12 +792,12 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
                             const float2 r = wg.phi[min(g * estep, N1 - 1)];
                             p = cmul(anc, r);
                         }
-                        w0 = wa0 + p.x * P[0] + p.y * Q[0];
-                        w1 = wa0 + p.x * P[1] + p.y * Q[1];
+                        w0 = wc.a0 + p.x * wc.P[0] + p.y * wc.Q[0];
+                        w1 = wc.a0 + p.x * wc.P[1] + p.y * wc.Q[1];
                         if constexpr (WM == 2) {
                             const float c2 = p.x * p.x - p.y * p.y, s2 = 2.f * p.x * p.y;
-                            w0 += c2 * R2[0] + s2 * S2[0];
-                            w1 += c2 * R2[1] + s2 * S2[1];
+                            w0 += c2 * wc.R2[0] + s2 * wc.S2[0];
+                            w1 += c2 * wc.R2[1] + s2 * wc.S2[1];
                         }
                     }
                     if constexpr (MF) lp[idx] = split16(x0[u] * w0, x1[u] * w1);
@@ -822,17 +814,49 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     using I1 = std::integral_constant<int, 1>;
     using I2 = std::integral_constant<int, 2>;
     auto fill_w = [&](auto mode) __attribute__((always_inline)) {
-        if (wmode == 0) fill(mode, I0());
-        else if (wmode == 1) fill(mode, I1());
+        if (wc.mode == 0) fill(mode, I0());
+        else if (wc.mode == 1) fill(mode, I1());
         else fill(mode, I2());
     };
-    if (active) {
-        if (up.kind == kUnpackSingleChannel) fill_w(I0());
-        else if (is_m5a(up)) fill(std::integral_constant<int, 3>(), I2());
-        else if (up.framed) fill_w(std::integral_constant<int, 4>());   // same window code as mode 1
-        else if (up.step == 1 && even) fill_w(I1());
-        else fill_w(I2());
-    }
+    if (up.kind == kUnpackSingleChannel) fill_w(I0());
+    else if (is_m5a(up)) fill(std::integral_constant<int, 3>(), I2());
+    else if (up.framed) fill_w(std::integral_constant<int, 4>());   // same window code as mode 1
+    else if (up.step == 1 && even) fill_w(I1());
+    else fill_w(I2());
+}
+
+// MF: the first column stage is radix 64 on the matrix cores (stage_mfma64;
+// one pipeline per workgroup, whole waves)
+template <int MAXPTS, bool CX, bool MF = false>
+__global__ void __launch_bounds__(512) k_pass1(P1Args a)
+{
+    extern __shared__ float2 lds[];
+    unsigned bx, by, bz;
+    xcd_remap(bx, by, bz, a.zfast);
+    const JobDev J = a.jobs[bz];
+    // a workgroup runs `fpw` independent pipelines, one FFT each (short columns)
+    const int fpw = a.fpw, pth = a.pth;
+    if ((int)by * fpw >= J.nf) return;   // no FFT of this job here (whole workgroup)
+    int pipe = 0;
+    while (pipe + 1 < fpw && threadIdx.x >= (unsigned)((pipe + 1) * pth)) pipe++;
+    const int f = (int)by * fpw + pipe;
+    const bool active = f < J.nf;
+    const int B = a.B, N1 = a.N1, N2 = a.N2;
+    const int c0 = (int)bx * B;
+    const int tid = threadIdx.x - pipe * pth, nth = pth;
+    float2* __restrict__ lp = lds + pipe * (N1 * B);
+    const Unpack up = J.up;
+    const uint8_t* __restrict__ raw = up.framed ? J.raw : job_fft_ptr(J, active ? f : 0);
+    const long long pay = J.pay0 + (long long)(active ? f : 0) * J.fft_stride;   // framed only
+    const int et = tid / B;            // the thread's first row (and first butterfly)
+    const int bcol = tid - et * B;     // its column, fixed
+    const int cb = c0 + bcol;
+    const int total = N1 * B;
+    const int estep = nth / B;
+    const WinCoef wc = win_coef(a.win, cb);
+    // MODE 3 (VLBA): samples decoded through the FFT's row of blank zones
+    const int2* __restrict__ brow = is_m5a(up) ? up.vblank + (size_t)(active ? f : 0) * up.vnseg : nullptr;
+    if (active) fill_points<MF>(lp, raw, up, pay, brow, a.win, wc, tid, nth, B, N1, N2, cb, et);
     __syncthreads();
 
     // A plain last radix (2, 3, 4, 5, 8) after at least one LDS stage runs
@@ -1253,6 +1277,235 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
                 *(float4*)(xbase + 4 * o1[u]) = kahan4(xc1[u], xacc1[m0 + u]);
                 if (two) *(float4*)(xbase + 4 * o2[u]) = kahan4(xc2[u], xacc2[m0 + u]);
             }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// single pass (short FFTs): decode + window + the whole M-point FFT in LDS +
+// r2c split + detection, each pipeline looping over the FFTs of one
+// accumulator slice with the slice's sums in registers, so no intermediate
+// leaves the workgroup.  The plan is the four-step layout with N1 = M rows
+// and N2 = 1 column: the decode/window fill is pass 1's (column 0) and
+// accumulator entry k is bin k (N1 = M, so the entry of bin k is k).
+// ---------------------------------------------------------------------------
+
+struct SPArgs {
+    long long M;
+    int T;                 // threads per FFT pipeline
+    int fpw;               // pipelines per workgroup: fpw / NSRC groups of NSRC pipelines
+    int S;                 // slices (workgroups) per job: slice s sums FFTs [s*per, (s+1)*per) of the chunk
+    int kahan;
+    long long accstride;   // floats between accumulator slices (2*(M+1))
+    long long xaccstride;  // floats between cross-power slices (4*(M+1))
+    SubPlan sp;
+    const float2* stw;
+    const float2* rA;      // W_N^k, k < M: r2c split twiddles
+    WindowGen win;
+    int zfast;             // jobs share raw bytes: job index fastest in the workgroup order
+    JobDev jobs[kMaxJobs];
+    XpolDev xp[kMaxJobs / 2];
+};
+
+__device__ __forceinline__ float2 kahan_add2(float2 sc, float v, int kahan)
+{
+    if (kahan) {
+        const float y = v - sc.y, t = sc.x + y;
+        sc.y = (t - sc.x) - y;
+        sc.x = t;
+    } else {
+        sc.x += v;
+    }
+    return sc;
+}
+
+__device__ __forceinline__ float4 kahan_add4(float4 x, float2 v, int kahan)
+{
+    if (kahan) {
+        const float y0 = v.x - x.z, y1 = v.y - x.w;
+        const float t0 = x.x + y0, t1 = x.y + y1;
+        x.z = (t0 - x.x) - y0;
+        x.w = (t1 - x.y) - y1;
+        x.x = t0;
+        x.y = t1;
+    } else {
+        x.x += v.x;
+        x.y += v.y;
+    }
+    return x;
+}
+
+// Grid (S, 1, job groups); block fpw * T threads.  A workgroup is one
+// accumulator slice; its G = fpw / NSRC groups of NSRC pipelines take the
+// slice's FFTs in turn (group g: FFTs g, g + G, ...), pipeline `pipe` holding
+// source pipe % NSRC of group pipe / NSRC.  Bin pairs (k, M - k) are split
+// over a group's NSRC * T threads, each thread keeping the same pairs (and
+// both sources of a cross-power pair) for every FFT; at the end the groups'
+// sums meet in LDS, are added in group order (a fixed order: reproducible),
+// and group 0 updates the slice.
+// One source at 16 points per thread is held to 128 VGPRs (4 waves per SIMD;
+// a few dwords spill): 7% faster at 4096 points than 2 waves without
+// spills.  At 32 points per thread the same bound spills ~250 bytes per lane
+// and costs 25-35%, so those take the compiler's allocation.
+template <int MAXPTS, int NSRC, bool CX>
+__global__ void __launch_bounds__(512)
+    __attribute__((amdgpu_waves_per_eu(NSRC == 1 && MAXPTS == 16 ? 4 : 1))) k_short(SPArgs a)
+{
+    extern __shared__ float2 lds[];
+    const int M = (int)a.M, T = a.T;
+    const int pipe = (int)threadIdx.x / T;
+    const int tid = (int)threadIdx.x - pipe * T;
+    const int G = a.fpw / NSRC;
+    const int gi = pipe / NSRC, src = pipe - gi * NSRC;
+    unsigned bx, by, bz;
+    xcd_remap(bx, by, bz, a.zfast);   // grid (S, 1, job groups)
+    const int sl = (int)bx;
+    const int grp = (int)bz;
+    JobDev J = a.jobs[grp * NSRC];
+    if (NSRC == 2 && src == 1) J = a.jobs[grp * NSRC + 1];
+    const int nfj = a.jobs[grp * NSRC].nf;          // FFTs of this job (pair) in the chunk
+    const int per = (nfj + a.S - 1) / a.S;          // FFTs per slice
+    const int fb = sl * per;
+    const int trips = (per + G - 1) / G;            // uniform: barriers inside
+    float2* __restrict__ lp = lds + pipe * M;
+    const float2* __restrict__ lg = lds + gi * NSRC * M;   // the group's NSRC FFTs
+    const int gt = src * T + tid, gth = NSRC * T;
+    const WinCoef wc = win_coef(a.win, 0);
+    constexpr int MP = MAXPTS / (2 * NSRC) + 1;     // pairs per thread: (M/2 + 1) / (NSRC T) <= MP
+    const int npair = M / 2 + 1;
+    float acc1[NSRC][MP], acc2[NSRC][MP];
+    float2 xacc1[MP], xacc2[MP];
+#pragma unroll
+    for (int m = 0; m < MP; m++) {
+#pragma unroll
+        for (int s = 0; s < NSRC; s++) acc1[s][m] = acc2[s][m] = 0.f;
+        xacc1[m] = xacc2[m] = make_float2(0.f, 0.f);
+    }
+    for (int it = 0; it < trips; it++) {
+        const int k = it * G + gi;                  // FFT of the slice
+        const int f = fb + k;
+        const bool act = k < per && f < nfj;
+        if (act) {
+            const uint8_t* __restrict__ raw = J.up.framed ? J.raw : job_fft_ptr(J, f);
+            const long long pay = J.pay0 + (long long)f * J.fft_stride;   // framed only
+            const int2* __restrict__ brow = is_m5a(J.up) ? J.up.vblank + (size_t)f * J.up.vnseg : nullptr;
+            fill_points<false>(lp, raw, J.up, pay, brow, a.win, wc, tid, T, 1, M, 1, 0, tid);
+        }
+        __syncthreads();
+        fft_lds<MAXPTS, CX>(lds, a.sp, a.stw, pipe * M, 1, tid, T, a.sp.nst);
+        if (act) {
+            // X[k] = E + W_N^k O, X[M-k] = conj(E - W_N^k O) with
+            // E = (Z[k] + conj Z[M-k])/2, O = (Z[k] - conj Z[M-k])/(2i);
+            // pair 0 gives DC and Nyquist (accumulator entry M)
+#pragma unroll
+            for (int m = 0; m < MP; m++) {
+                int pi = gt + m * gth;
+                asm volatile("" : "+v"(pi));   // per pair: loads and index math not hoisted together
+                if (pi >= npair) continue;
+                const int mc = pi == 0 ? 0 : M - pi;
+                const float2 w = a.rA[pi];
+                float2 X1[NSRC], X2[NSRC];
+#pragma unroll
+                for (int s = 0; s < NSRC; s++) {
+                    const float2 Z = lg[s * M + pi];
+                    const float2 Zm = lg[s * M + mc];
+                    const float2 E = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+                    const float2 O = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
+                    const float2 wO = cmul(w, O);
+                    X1[s] = make_float2(E.x + wO.x, E.y + wO.y);
+                    X2[s] = make_float2(E.x - wO.x, wO.y - E.y);   // conj(E - wO)
+                    acc1[s][m] += X1[s].x * X1[s].x + X1[s].y * X1[s].y;
+                    acc2[s][m] += X2[s].x * X2[s].x + X2[s].y * X2[s].y;
+                }
+                if (NSRC == 2) {
+                    // X0 * conj(X1) for both bins of the pair
+                    float2 c = xacc1[m];
+                    c.x += X1[0].x * X1[NSRC - 1].x + X1[0].y * X1[NSRC - 1].y;
+                    c.y += X1[0].y * X1[NSRC - 1].x - X1[0].x * X1[NSRC - 1].y;
+                    xacc1[m] = c;
+                    c = xacc2[m];
+                    c.x += X2[0].x * X2[NSRC - 1].x + X2[0].y * X2[NSRC - 1].y;
+                    c.y += X2[0].y * X2[NSRC - 1].x - X2[0].x * X2[NSRC - 1].y;
+                    xacc2[m] = c;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (fb >= nfj) return;   // no FFT in this slice (whole workgroup): nothing to add
+    // groups 1.. hand their sums to group 0 through LDS (the FFT buffers are
+    // free: the loop ended with a barrier).  Group g - 1's region holds, per
+    // pair, NSRC (bin k, bin M-k) sums and with cross-power the two complex
+    // sums: (2 NSRC + 4 [NSRC == 2]) npair floats, which the G >= 2 groups'
+    // NSRC * M * 8 bytes each of LDS hold.
+    if (G > 1) {
+        float* __restrict__ red = (float*)lds;
+        const int gstride = (2 * NSRC + (NSRC == 2 ? 4 : 0)) * npair;
+        if (gi > 0) {
+            float* __restrict__ r = red + (gi - 1) * gstride;
+#pragma unroll
+            for (int m = 0; m < MP; m++) {
+                const int pi = gt + m * gth;
+                if (pi >= npair) continue;
+#pragma unroll
+                for (int s = 0; s < NSRC; s++) {
+                    r[(2 * s) * npair + pi] = acc1[s][m];
+                    r[(2 * s + 1) * npair + pi] = acc2[s][m];
+                }
+                if (NSRC == 2) {
+                    float2* rx = (float2*)(r + 2 * NSRC * npair);
+                    rx[pi] = xacc1[m];
+                    rx[npair + pi] = xacc2[m];
+                }
+            }
+        }
+        __syncthreads();
+        if (gi > 0) return;
+        for (int g = 1; g < G; g++) {
+            const float* __restrict__ r = red + (g - 1) * gstride;
+#pragma unroll
+            for (int m = 0; m < MP; m++) {
+                const int pi = gt + m * gth;
+                if (pi >= npair) continue;
+#pragma unroll
+                for (int s = 0; s < NSRC; s++) {
+                    acc1[s][m] += r[(2 * s) * npair + pi];
+                    acc2[s][m] += r[(2 * s + 1) * npair + pi];
+                }
+                if (NSRC == 2) {
+                    const float2* rx = (const float2*)(r + 2 * NSRC * npair);
+                    const float2 u = rx[pi], v = rx[npair + pi];
+                    xacc1[m].x += u.x; xacc1[m].y += u.y;
+                    xacc2[m].x += v.x; xacc2[m].y += v.y;
+                }
+            }
+        }
+    }
+    // accumulator (sum, compensation) updates, both bins of each pair
+#pragma unroll
+    for (int m = 0; m < MP; m++) {
+        const int pi = gt + m * gth;
+        if (pi >= npair) continue;
+        const long long o1 = pi, o2 = pi == 0 ? M : M - pi;
+        const bool two = !(pi != 0 && 2 * pi == M);   // M/2 is its own mirror
+#pragma unroll
+        for (int s = 0; s < NSRC; s++) {
+            float* __restrict__ accp = a.jobs[grp * NSRC + s].acc + (long long)sl * a.accstride;
+            float2* e1 = (float2*)(accp + 2 * o1);
+            float2* e2 = (float2*)(accp + 2 * o2);
+            const float2 c1 = *e1;
+            const float2 c2 = two ? *e2 : make_float2(0.f, 0.f);
+            *e1 = kahan_add2(c1, acc1[s][m], a.kahan);
+            if (two) *e2 = kahan_add2(c2, acc2[s][m], a.kahan);
+        }
+        if (NSRC == 2) {
+            float* __restrict__ xb = a.xp[grp].acc + (long long)sl * a.xaccstride;
+            float4* e1 = (float4*)(xb + 4 * o1);
+            float4* e2 = (float4*)(xb + 4 * o2);
+            const float4 c1 = *e1;
+            const float4 c2 = two ? *e2 : make_float4(0.f, 0.f, 0.f, 0.f);
+            *e1 = kahan_add4(c1, xacc1[m], a.kahan);
+            if (two) *e2 = kahan_add4(c2, xacc2[m], a.kahan);
         }
     }
 }
@@ -1830,6 +2083,7 @@ static size_t p2_lds(const Plan& p, int nsrc)
 hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, hipStream_t s)
 {
     if (njobs < 1 || njobs > kMaxJobs || nfft < 1) return hipErrorInvalidValue;
+    if (p.single) return hipSuccess;   // single-pass plans: launch_pass2 does the whole FFT
     P1Args a;
     a.N1 = p.N1; a.N2 = p.N2; a.M = p.M; a.B = p.B; a.twshift = p.twshift;
     a.sp = p.sp1; a.stw = p.d_stw1; a.twA = p.d_twA; a.twB = p.d_twB; a.win = p.win; a.mft = p.d_mft;
@@ -1876,11 +2130,84 @@ static int pass2_slices(const Plan& p, int ngroups, int nfft, int smax)
     return std::max(1, std::min(S, std::min(smax, nfft)));
 }
 
+// Slices (workgroups) per job for the single-pass kernel: about 2048
+// workgroups over the device, at least 2 FFTs per pipeline where the chunk
+// allows, at most smax.
+static int short_slices(int ngroups, int nfft, int groups_per_wg, int smax)
+{
+    const int want = std::max(1, (2048 + ngroups - 1) / ngroups);
+    const int S = std::min(want, std::max(1, nfft / (2 * groups_per_wg)));
+    return std::max(1, std::min(S, smax));
+}
+
+static hipError_t launch_short(const Plan& p, const JobDev* jobs, int njobs, const XpolDev* xp, int nfft, int kahan,
+                               int smax, hipStream_t s)
+{
+    const int nsrc = xp ? 2 : 1;
+    SPArgs a;
+    a.M = p.M; a.T = p.sp1.T; a.kahan = kahan; a.sp = p.sp1; a.stw = p.d_stw1; a.rA = p.d_rA; a.win = p.win;
+    for (int j = 0; j < njobs; j++) a.jobs[j] = jobs[j];
+    if (xp) for (int g = 0; g < njobs / 2; g++) a.xp[g] = xp[g];
+    a.accstride = 2 * (p.M + 1);
+    a.xaccstride = 4 * (p.M + 1);
+    const int gthreads = nsrc * a.T;
+    const int MP = p.p1_maxpts / (2 * nsrc) + 1;   // the kernel's register pairs per thread
+    if (gthreads > 512 || (p.p1_maxpts != 16 && p.p1_maxpts != 32) || p.M / 2 + 1 > (long long)MP * gthreads)
+        return hipErrorInvalidValue;
+    const size_t pipe_bytes = (size_t)p.M * sizeof(float2);
+    // groups per workgroup: <= 512 threads, <= 64 KB of LDS (one group may
+    // need more), no more than the chunk's FFTs; with G >= 2 the groups' sums
+    // meet in the LDS of the FFT buffers ((2 nsrc + 4 [nsrc == 2]) (M/2 + 1)
+    // floats per group after the first: within nsrc * M * 8 bytes per group)
+    int gpw = std::max(1, std::min(512 / gthreads, (int)((64 * 1024) / (nsrc * pipe_bytes))));
+    gpw = std::max(1, std::min(gpw, nfft));
+    while (gpw > 1 && (size_t)(gpw - 1) * (2 * nsrc + (nsrc == 2 ? 4 : 0)) * (size_t)(p.M / 2 + 1) * sizeof(float) >
+                          (size_t)gpw * nsrc * pipe_bytes)
+        gpw--;
+    a.fpw = gpw * nsrc;
+    a.S = short_slices(njobs / nsrc, nfft, gpw, smax);
+    const size_t lds = (size_t)a.fpw * pipe_bytes;
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    using K = void (*)(SPArgs);
+    // [maxpts 16|32][nsrc 1|2][composite]
+    static const K table[2][2][2] = {
+        {{k_short<16, 1, false>, k_short<16, 1, true>}, {k_short<16, 2, false>, k_short<16, 2, true>}},
+        {{k_short<32, 1, false>, k_short<32, 1, true>}, {k_short<32, 2, false>, k_short<32, 2, true>}}};
+    K k = table[p.p1_maxpts == 32][nsrc - 1][p.p1_cx ? 1 : 0];
+    if (lds > 64 * 1024) {
+        static std::mutex mu;
+        static std::set<K> raised;
+        std::lock_guard<std::mutex> l(mu);
+        if (!raised.count(k)) {
+            const hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     160 * 1024);
+            if (e != hipSuccess) return e;
+            raised.insert(k);
+        }
+    }
+    // Slices fastest in the workgroup order, jobs slowest.  Channels of one
+    // stream read the same bytes, but putting a slice's channel workgroups
+    // side by side on one XCD (pass 1's order) measured 5-7% slower at
+    // 4096-16384 points, 4 channels (interleaved runs on one device): the
+    // bytes are few and the L2 serves them either way.  SWSPEC_SHORT_ZFAST=1
+    // selects that order.
+    a.zfast = 0;
+    static const int zf_env = [] {
+        const char* e = getenv("SWSPEC_SHORT_ZFAST");
+        return e ? atoi(e) : -1;
+    }();
+    if (zf_env >= 0) a.zfast = zf_env != 0;
+    const dim3 grid(a.S, 1, njobs / nsrc);
+    hipLaunchKernelGGL(k, grid, dim3(a.fpw * a.T), lds, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const XpolDev* xp, int nfft,
                         int kahan, int smax, hipStream_t s)
 {
     const int nsrc = xp ? 2 : 1;
     if (njobs < 1 || njobs > kMaxJobs || (njobs % nsrc) != 0 || nfft < 1 || smax < 1) return hipErrorInvalidValue;
+    if (p.single) return launch_short(p, jobs, njobs, xp, nfft, kahan, smax, s);
     P2Args a;
     a.N1 = p.N1; a.N2 = p.N2; a.M = p.M; a.nfft = nfft; a.kahan = kahan;
     a.sp = p.sp2; a.stw = p.d_stw2; a.rA = p.d_rA; a.rB = p.d_rB;
@@ -2358,6 +2685,57 @@ std::string plan_geometry(Plan& p, long long N, bool mfma64, const PlanKnobs* kn
         }
         return true;
     };
+    // accumulator slices: up to 1024, at most 16 MiB of slices per job (64 MiB
+    // for the single pass, whose slices are its workgroups: 1024 at M = 8192)
+    long long slice_mb = M <= 8192 ? 64 : 16;
+    if (const char* e = getenv("SWSPEC_SLICE_MB")) slice_mb = std::max(1LL, atoll(e));
+    p.smax = (int)std::max<long long>(1, std::min<long long>(1024, (slice_mb << 20) / (8 * (p.M + 1))));
+
+    // Short FFTs (M <= 8192 complex points; SWSPEC_SHORT_MAX moves the bound,
+    // 0 turns it off) run in one pass, k_short: the M-point FFT in LDS with at
+    // most 256 threads per FFT, the fewest stages of 16 or 32 points per
+    // thread (knobs->p1_maxpts forces one).  Its plan is the four-step layout
+    // with N1 = M and N2 = 1 (the column pass's decode/window, bin k at
+    // accumulator entry k).  Not with the MFMA stage.
+    static const long long short_max = [] {
+        const char* e = getenv("SWSPEC_SHORT_MAX");
+        return e ? atoll(e) : 8192LL;
+    }();
+    if (!mfma64 && M <= short_max && !(knobs && knobs->single == 0)) {
+        SubPlan s16, s32;
+        int x16 = 0, x32 = 0;
+        const bool ok16 = subplan(M, 16, cxmask & 1, s16, x16) && subplan_threads(s16, 16) <= 256;
+        const bool ok32 = subplan(M, 32, cxmask & 1, s32, x32) && subplan_threads(s32, 32) <= 256;
+        int mp = 0;
+        if (ok16) mp = 16;
+        if (ok32 && (!ok16 || s32.nst < s16.nst)) mp = 32;
+        if (knobs && knobs->p1_maxpts == 16 && ok16) mp = 16;
+        if (knobs && knobs->p1_maxpts == 32 && ok32) mp = 32;
+        if (mp) {
+            p.single = 1;
+            p.N1 = (int)M;
+            p.N2 = 1;
+            p.B = 1;
+            p.p1_maxpts = mp;
+            p.sp1 = mp == 16 ? s16 : s32;
+            p.p1_cx = mp == 16 ? x16 : x32;
+            int T = subplan_threads(p.sp1, mp);
+            // whole waves: the extra threads idle in the FFT stages but share
+            // the decode/window fill and the bin pairs
+            if (T > 64 && T % 64) T = std::min(256, ((T + 63) / 64) * 64);
+            p.sp1.T = T;
+            p.p1_threads = T;
+            p.p1_fpw = 1;
+            p.sp2 = SubPlan();
+            p.p2_maxpts = 16;
+            p.p2_cx = 0;
+            p.p2_threads = 0;
+            p.npairs = (int)(M / 2 + 1);
+            p.twshift = 10;
+            return "";
+        }
+    }
+
     SubPlan c16, c32;
     int cx16 = 0, cx32 = 0;
     if (!subplan(p.N1, 16, cxmask & 1, c16, cx16, p1_order) || !subplan(p.N1, 32, cxmask & 1, c32, cx32, p1_order))
@@ -2549,10 +2927,6 @@ std::string plan_geometry(Plan& p, long long N, bool mfma64, const PlanKnobs* kn
     p.p1_fpw = std::max(1, std::min(256 / p.p1_threads,
                                     (int)((64 * 1024) / ((size_t)p.N1 * p.B * sizeof(float2)))));
     if (p.mfma64) p.p1_fpw = 1;
-    // accumulator slices: up to 1024, at most 16 MiB of slices per job
-    long long slice_mb = 16;
-    if (const char* e = getenv("SWSPEC_SLICE_MB")) slice_mb = std::max(1LL, atoll(e));
-    p.smax = (int)std::max<long long>(1, std::min<long long>(1024, (slice_mb << 20) / (8 * (p.M + 1))));
     return "";
 }
 

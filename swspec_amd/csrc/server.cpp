@@ -30,6 +30,7 @@ constexpr int kStreams = 2;
 constexpr size_t kYScratch = 512ull << 20;   // per stream: pass-1 output of one launch
 constexpr int kMaxG = 64;                    // FFTs per job per chunk with phase-cal (mask bits)
 constexpr int kMaxGPlain = 1024;             // FFTs per job per chunk without phase-cal
+constexpr int kMaxGShort = 16384;            // the same for single-pass plans (no intermediate)
 
 struct SJob {
     int64_t id = 0;
@@ -316,14 +317,17 @@ static int step_enqueue(swspec_hip_server* s, bool capture, std::vector<SJob*>* 
         Plan* p = g.second.first;
         std::vector<SJob*>& js = g.second.second;
         const double per = (double)p->M * sizeof(float2);
-        // jobs per launch: keep >= 8 FFTs per job per chunk when the scratch allows
-        int nj = (int)std::max(1.0, std::min((double)kMaxJobs, std::floor((double)kYScratch / (8.0 * per))));
+        // jobs per launch: keep >= 8 FFTs per job per chunk when the scratch
+        // allows (single-pass plans use none)
+        int nj = p->single ? kMaxJobs
+                           : (int)std::max(1.0, std::min((double)kMaxJobs, std::floor((double)kYScratch / (8.0 * per))));
         for (size_t b = 0; b < js.size(); b += nj) {
             const int n = (int)std::min<size_t>(nj, js.size() - b);
             bool anyp = false;
             for (int k = 0; k < n; k++) anyp |= js[b + k]->pcal;
-            const double gmax = anyp ? (double)kMaxG : (double)kMaxGPlain;
-            const int G = (int)std::max(1.0, std::min(gmax, std::floor((double)kYScratch / (n * per))));
+            const double gmax = anyp ? (double)kMaxG : (double)(p->single ? kMaxGShort : kMaxGPlain);
+            const int G = p->single ? (int)gmax
+                                    : (int)std::max(1.0, std::min(gmax, std::floor((double)kYScratch / (n * per))));
             const int si = launch++ % kStreams;
             hipStream_t st = s->st[si];
             used[si] = true;
@@ -356,7 +360,7 @@ static int step_enqueue(swspec_hip_server* s, bool capture, std::vector<SJob*>* 
                 J.nwrap = j->nwrap;
                 J.nf = nf;
                 J.up = j->up;
-                J.Y = s->yscr[si] + (size_t)k * G * (size_t)p->M;
+                J.Y = p->single ? nullptr : s->yscr[si] + (size_t)k * G * (size_t)p->M;
                 J.acc = j->acc;
                 if (j->pcal) {
                     J.pcal_part = j->pcal_part;
