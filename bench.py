@@ -44,31 +44,11 @@ def log(*a):
 
 
 def make_input(torch, seed=2024):
-    """Seeded 4-channel 2-bit payload on the device (uint8, SAMPLES bytes)."""
-    dev = torch.device("cuda:0")
-    g = torch.Generator(device=dev)
-    g.manual_seed(seed)
-    raw = torch.zeros(SAMPLES, dtype=torch.uint8, device=dev)
-    rng = np.random.default_rng(seed)
-    fs = int(FS)
-    tones = []
-    for ch in range(NCH):
-        tones.append([(0.141, 10_000 + k * 1_000_000, float(rng.uniform(0, 2 * np.pi))) for k in range(16)])
-    slab = 40_000_000
-    for ch in range(NCH):
-        for s0 in range(0, SAMPLES, slab):
-            n = min(slab, SAMPLES - s0)
-            x = torch.randn(n, generator=g, device=dev, dtype=torch.float32)
-            idx = torch.arange(s0, s0 + n, device=dev, dtype=torch.int64)
-            for (a, f, ph) in tones[ch]:
-                r = (idx * f) % fs          # exact phase numerator
-                x += a * torch.cos(r.to(torch.float32) * (2 * math.pi / fs) + ph)
-            t = 0.98
-            f2 = torch.where(x >= t, 2, torch.where(x >= 0, 0, torch.where(x >= -t, 1, 3))).to(torch.uint8)
-            raw[s0:s0 + n] |= f2 << (2 * ch)
-            del x, idx, f2
-    torch.cuda.synchronize()
-    return raw, tones
+    """Seeded 4-channel 2-bit payload on the device (uint8, SAMPLES bytes):
+    unit Gaussian noise plus a 16-tone comb per channel, quantised at
+    +-0.98 sigma (workloads.mk5b_payload)."""
+    from swspec_amd import workloads as W
+    return W.mk5b_payload(torch, seed, SAMPLES, fs=FS, nch=NCH, ntones=16)
 
 
 def settings(swspec_amd):
@@ -78,9 +58,16 @@ def settings(swspec_amd):
         MaxSourceBufferMB=1024)
 
 
+def path_flop_per_sample(n, pcal=True):
+    """Algorithmic FLOP per channel-sample of the whole path: 2.5 log2 N
+    (real FFT) + 1 (window) + 2 (|X|^2 + accumulate) + 4 (phase-cal): 61.0 at
+    3.2M points with phase-cal."""
+    return 2.5 * math.log2(n) + 3.0 + (4.0 if pcal else 0.0)
+
+
 def flops_per_sample(plan):
-    """Algorithmic FLOP per channel-sample split by kernel.  Whole path:
-    2.5 log2 N (real FFT) + 1 (window) + 2 (|X|^2 + accumulate) + 4 (phase-cal)."""
+    """The same split by kernel (pass 1: window + column FFTs; pass 2: row
+    FFTs + r2c split + detection; the phase-cal fold)."""
     l1 = math.log2(plan["N1"])
     l2 = math.log2(plan["N2"]) + 1.0    # + the real-to-complex split stage
     return {"pass1": 1.0 + 2.5 * l1, "pass2": 2.0 + 2.5 * l2, "pcal": 4.0, "finalize": 0.0}
@@ -108,9 +95,52 @@ def profiled_traffic(kernel, plan, jobs_per_launch):
     return None
 
 
-def cpu_baseline(raw_host, threads, ffts_per_thread=12):
+def cpu_info():
+    """CPU model, logical CPUs, CPUs this process may run on, physical cores."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": aff,
+            "physical_cores": len(cores) or None}
+
+
+def fft_gflops(fn, x, reps=2):
+    """GFLOP/s of one single-threaded real FFT of len(x) points (2.5 N log2 N)."""
+    fn(x)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn(x)
+    dt = (time.perf_counter() - t0) / reps
+    return 2.5 * len(x) * math.log2(len(x)) / dt / 1e9
+
+
+def cpu_baseline(raw_host, threads, ffts_per_thread=NFFT):
     """FP32 restatement of TaskCoreIPP::doMaths (oracle/) on `threads` threads,
-    each a core object working on its own 3.2M-point FFTs of one channel."""
+    each a core object integrating one channel's 3.2M-point FFTs (by default
+    the whole 100-FFT spectrum of the 10 s payload, channel t mod 4).
+    Also reports the CPU, and the single-thread GFLOP/s of the restatement's
+    FFT and of numpy's pocketfft at the same length for scale."""
     from oracle import oracle as O
     O.build()
     s = O.derive(fft_points=N, fft_integ_seconds=N * ffts_per_thread / FS, fft_overlap_factor=1,
@@ -140,10 +170,24 @@ def cpu_baseline(raw_host, threads, ffts_per_thread=12):
         t.join()
     dt = time.perf_counter() - t0
     samples = threads * per
-    return {"value": samples / dt / 1e9, "unit": "Gsamples/s", "cores": threads, "kind": "port",
-            "sample": "%d threads x %d FFTs of 3.2M points (%.1fM channel-samples), FP32 restatement of "
-                      "TaskCoreIPP::doMaths with phase-cal, %.2f s wall" % (threads, ffts_per_thread,
-                                                                           samples / 1e6, dt)}
+    info = cpu_info()
+    x = np.random.default_rng(1).standard_normal(N).astype(np.float32)
+    g_port = fft_gflops(O.rfft32, x)
+    g_np = fft_gflops(np.fft.rfft, x)
+    value = samples / dt / 1e9
+    out = {"value": round(value, 4), "unit": "Gsamples/s", "cores": threads, "kind": "port",
+           "sample": "%d threads x %d FFTs of 3.2M points (%.1fM channel-samples), FP32 restatement of "
+                     "TaskCoreIPP::doMaths with phase-cal, %.2f s wall" % (threads, ffts_per_thread,
+                                                                          samples / 1e6, dt),
+           "threads": threads, "threads_rule": "min(16, CPUs in this process's affinity set)",
+           "per_thread_gsps": round(value / threads, 5),
+           "path_fft_gflops": round(2.5 * N * math.log2(N) * threads * ffts_per_thread / dt / 1e9, 2),
+           "fft_gflops_1thread": round(g_port, 3), "numpy_pocketfft_gflops_1thread": round(g_np, 3)}
+    out.update(info)
+    if info["physical_cores"]:
+        # linear in cores: an upper bound for a memory-bound FFT
+        out["linear_estimate_all_physical_cores_gsps"] = round(value / threads * info["physical_cores"], 3)
+    return out
 
 
 def plain_cfg2(torch, swspec_amd, raw, s, reps=2):
@@ -358,7 +402,10 @@ def batch_cfg4(torch, replays=3, seed=4):
             "sizes": sizes}
 
 
-def serve_cfg5(torch, seconds, rate=200.0, seed=5, plain_seconds=None):
+def serve_cfg5(torch, seconds, rate=200.0, seed=5):
+    """One Poisson trace of `seconds` of arrivals at `rate` jobs/s, replayed in
+    real time twice: continuous batching (every admitted job advances in the
+    same steps) and one job (<= 4 channels) at a time, in arrival order."""
     from swspec_amd import workloads as W
     from swspec_amd.server import Server
     trace = W.serving_trace(seed=seed, rate=rate, duration=seconds)
@@ -370,23 +417,23 @@ def serve_cfg5(torch, seconds, rate=200.0, seed=5, plain_seconds=None):
                  17 + jb["fft_points"])
     torch.cuda.synchronize()
 
-    def run(tr, cap, max_wall):
+    def run(cap, max_wall):
         srv = Server(max_active=max(cap, 4))
-        r = W.run_serving(torch, srv, tr, pool, max_active_channels=cap, max_wall=max_wall)
+        r = W.run_serving(torch, srv, trace, pool, max_active_channels=cap, max_wall=max_wall)
+        st = srv.stats()
         srv.close()
         lat = np.array(r["latencies"]) if r["latencies"] else np.array([np.nan])
         return {"jobs": r["submitted"], "completed": r["completed"], "truncated": r["truncated"],
                 "gsps": round(r["samples"] / r["wall"] / 1e9, 3),
-                "p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
-                "wall_s": round(r["wall"], 2)}
+                "p50_s": round(float(np.percentile(lat, 50)), 4), "p90_s": round(float(np.percentile(lat, 90)), 4),
+                "p99_s": round(float(np.percentile(lat, 99)), 4), "max_s": round(float(np.max(lat)), 4),
+                "steps": st["steps"], "wall_s": round(r["wall"], 2)}
 
-    served = run(trace, 256, 3 * seconds + 30)
-    ptrace = [j for j in trace if j["t"] < (plain_seconds if plain_seconds else seconds)]
-    plain = run(ptrace, 4, 40)   # one job (<= 4 channels) at a time
+    served = run(256, 2 * seconds + 30)
+    plain = run(4, 2 * seconds + 30)
     offered = sum(j["nffts"] * j["fft_points"] * j["nch"] for j in trace) / seconds / 1e9
     return {"rate_jobs_per_s": rate, "arrival_window_s": seconds, "offered_gsps": round(offered, 2),
-            "continuous_batching": served, "plain_one_job_at_a_time": plain,
-            "plain_arrival_window_s": plain_seconds if plain_seconds else seconds}
+            "continuous_batching": served, "plain_one_job_at_a_time": plain}
 
 
 def pcal_extractor_leg(torch, nsamples=1 << 28, reps=5, cpu=True):
@@ -437,7 +484,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving replay")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain per-channel leg (profiling runs: only the measured launches)")
-    ap.add_argument("--serve-seconds", type=float, default=10.0, help="Poisson arrival window")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="Poisson arrival window at 200 jobs/s")
+    ap.add_argument("--serve-hi-seconds", type=float, default=5.0, help="arrival window near capacity")
     ap.add_argument("--serve-rate-hi", type=float, default=550.0,
                     help="arrival rate of the near-capacity serving point (jobs/s, 3 s window)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -450,10 +498,13 @@ def main():
     ap.add_argument("--no-host", action="store_true", help="skip the host-Buffer (PCIe-inclusive) leg")
     ap.add_argument("--only-host", action="store_true", help="run only the host-Buffer (PCIe-inclusive) leg")
     ap.add_argument("--only-batch", action="store_true", help="run only the 64-stream batch leg (with autotuning)")
+    ap.add_argument("--only-serve", action="store_true",
+                    help="run only the serving replays, one per rate of --serve-rates")
+    ap.add_argument("--serve-rates", default="200", help="comma-separated job rates for --only-serve")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: the workload is one GPU's share; "
-              "multi-GPU sharding is described in DESIGN.md, not launched", file=sys.stderr)
+        print("bench.py runs on one GPU only: the workload is one GPU's share, and "
+              "no multi-GPU run is launched", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -469,6 +520,11 @@ def main():
 
     if args.only_pcalx:
         print(json.dumps({"pcal_extractor": pcal_extractor_leg(torch, cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.only_serve:
+        for r in args.serve_rates.split(","):
+            log("serving replay at %s jobs/s ..." % r)
+            print(json.dumps({"serving": serve_cfg5(torch, args.serve_seconds, rate=float(r))}), flush=True)
         return
 
     import swspec_amd
@@ -530,8 +586,15 @@ def main():
     if "pcal" in per_kernel:
         per_kernel["pcal"]["overlapped_with_fft_passes"] = True
     dom = max((k for k in per_kernel if k != "pcal"), key=lambda k: per_kernel[k]["ms_per_step"])
-    ach = per_kernel[dom]["algorithmic_tflops"]
-    jobs_per_launch = int(round(NCH * NFFT / per_kernel[dom]["launches_per_step"]))
+    launches = per_kernel[dom]["launches_per_step"]
+    jobs_per_launch = int(round(NCH * NFFT / launches))
+    samples_per_launch = samples_per_step / launches
+    launch_s = per_kernel[dom]["avg_launch_us"] * 1e-6
+    # the path's algorithmic FLOP per channel-sample times the samples one
+    # launch of the dominant kernel processes, over its average launch time
+    fpath = path_flop_per_sample(N)
+    ach = fpath * samples_per_launch / launch_s / 1e12
+    step_tf = fpath * samples_per_step / (ms_per_step * 1e-3) / 1e12
     tr = profiled_traffic("k_" + dom, plan, jobs_per_launch)
     roofline = {"bound": "valu", "kernel": "k_" + dom, "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4),
@@ -540,8 +603,11 @@ def main():
                 "traffic_source": tr["source"] if tr else None,
                 "traffic_gbps_at_measured_launch": round(tr["bytes_per_launch"] / (per_kernel[dom]["avg_launch_us"] * 1e3), 1)
                 if tr else None,
-                "fft_jobs_per_launch": jobs_per_launch,
-                "flop_per_sample": round(fps[dom], 3)}
+                "flop_per_sample": round(fpath, 2), "samples_per_launch": int(samples_per_launch),
+                "fft_jobs_per_launch": jobs_per_launch, "avg_launch_us": round(per_kernel[dom]["avg_launch_us"], 2),
+                "whole_step": {"achieved": round(step_tf, 3), "frac": round(step_tf / PEAK_FP32_TFLOPS, 4)},
+                "kernel_own_flop_per_sample": round(fps[dom], 3),
+                "kernel_own_frac": round(per_kernel[dom]["algorithmic_tflops"] / PEAK_FP32_TFLOPS, 4)}
 
     # sanity on the output: spectra finite, tone comb visible in the phase-cal
     a0 = d_auto[0].cpu().numpy()
@@ -566,11 +632,11 @@ def main():
         batch = batch_cfg4(torch)
     if not args.no_serve:
         log("serving replay ...")
-        serve = serve_cfg5(torch, args.serve_seconds, plain_seconds=min(args.serve_seconds, 3.0))
+        serve = serve_cfg5(torch, args.serve_seconds)
         # the same job mix near the device's capacity: continuous batching
         # keeps up where one job at a time falls behind
         log("serving near capacity ...")
-        serve["near_capacity"] = serve_cfg5(torch, 3.0, rate=args.serve_rate_hi, seed=6, plain_seconds=3.0)
+        serve["near_capacity"] = serve_cfg5(torch, args.serve_hi_seconds, rate=args.serve_rate_hi, seed=6)
 
     xpol = None
     if not args.no_xpol:
@@ -584,9 +650,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = min(16, cpu_info()["affinity_cpus"])
         log("cpu baseline on %d threads ..." % threads)
-        raw_host = raw[: N * 12 * ((threads + NCH - 1) // NCH) + N].cpu().numpy()
+        raw_host = raw.cpu().numpy()
         cpu = cpu_baseline(raw_host, threads)
 
     out = {

@@ -188,9 +188,10 @@ def run_serving(torch, server, trace, pool, window="Cosine2", time_scale=1.0, ma
 
     Continuous batching: jobs that arrive while others run are admitted (all
     their channels at once) as soon as the active-channel cap allows, and then
-    advance in the same steps as the running ones.  Returns per-job latency
-    (s), total channel-samples and the wall time from the start to the last
-    completion."""
+    advance in the same steps as the running ones.  Returns the latencies (s)
+    in completion order, total channel-samples, the wall time from the start
+    to the last completion, and `jobs`: the trace's jobs, each completed one
+    carrying its "latency"."""
     pending = sorted((dict(j) for j in trace), key=lambda j: j["t"])   # per-run bookkeeping
     queue, live, lat = [], {}, []
     outs = {}
@@ -247,7 +248,8 @@ def run_serving(torch, server, trace, pool, window="Cosine2", time_scale=1.0, ma
             jb.setdefault("left", len(key))
             jb["left"] -= 1
             if jb["left"] == 0:
-                lat.append((time.perf_counter() - t0) / time_scale - jb["t"])
+                jb["latency"] = (time.perf_counter() - t0) / time_scale - jb["t"]
+                lat.append(jb["latency"])
                 ndone += 1
                 done_total += jb["nffts"] * jb["fft_points"] * jb["nch"]
                 del live[key]
@@ -266,11 +268,12 @@ def run_serving(torch, server, trace, pool, window="Cosine2", time_scale=1.0, ma
             jb.setdefault("left", len(key))
             jb["left"] -= 1
             if jb["left"] == 0:
-                lat.append((time.perf_counter() - t0) / time_scale - jb["t"])
+                jb["latency"] = (time.perf_counter() - t0) / time_scale - jb["t"]
+                lat.append(jb["latency"])
                 ndone += 1
                 done_total += jb["nffts"] * jb["fft_points"] * jb["nch"]
                 del live[key]
     wall = time.perf_counter() - t0
     # throughput counts completed jobs only (a truncated replay leaves some running)
     return dict(latencies=lat, samples=done_total, admitted_samples=total, wall=wall, completed=ndone,
-                submitted=len(pending), truncated=truncated)
+                submitted=len(pending), truncated=truncated, jobs=pending)
