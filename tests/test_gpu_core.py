@@ -720,3 +720,58 @@ def test_pcal_fold_bins_beyond_thread_count(hip, oracle, bw, off, nch, fmt):
     e64 = relerr(pc[0][0], p64[0][0])
     say("  pcal %.3e vs fp64 rotator %.3e" % (ep, e64))
     assert m == 1 and np.isfinite(pc[0][0]).all() and ep <= 1e-6 and e64 <= 1e-5
+
+
+# ---------------------------------------------------------------- phase-cal at 3.2M with overlap
+
+@pytest.mark.parametrize("fmt,nch,channels,framed", [("RawSigned", 1, [0], False), ("Mk5B", 4, [0, 1, 2, 3], False),
+                                                    ("Mk5B", 8, [1, 6], False), ("Mk5B", 4, [2, 3], True)])
+def test_pcal_3p2m_overlap_partial_vs_fold64(hip, oracle, fmt, nch, channels, framed):
+    """3.2M points with the 16-tone comb through every integer-count fold
+    instantiation (1-channel 2-bit, 4- and 8-channel Mk5B, framed Mk5B).
+    Each channel's folded vector against the FP64 fold with the FP32 rotator
+    (<= 1e-6), its tones, the oracle's doMaths phase-cal and spectrum; two
+    spectra (overlap 2, 3 FFTs each) and a partial third, phase-cal on every
+    second FFT of the run (TaskCoreIPP.cpp:633-693)."""
+    from swspec_amd._lib import FLAG_MK5B_FRAMED
+    n, fs = 3200000, 32e6
+    say("pcal 3.2M overlap", fmt, nch, channels, framed)
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * 2.1 / fs, FFToverlapFactor=2,
+                            WindowType="Cosine2", BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=nch,
+                            SourceFormat=fmt, ExtractPCal=True, MaxSourceBufferMB=4096)
+    K = s.core_overlapped_ffts
+    assert s.extract_pcal == 1 and s.pcal_tonebins == 32 and K == 3
+    nsamp = n // 2 * (2 * (K + 1) + 1) + n   # two spectra, then 2 FFTs of a third
+    rng = np.random.default_rng(90 + nch)
+    tones = [synth.pcal_comb(fs, fs / 2, amp=0.141, rng=rng) for _ in range(nch)]
+    if nch == 1:
+        raw = synth.raw_1ch(91, nsamp, tones=tones[0], fs=fs)
+    else:
+        raw = synth.raw_multi(91, nch, nsamp, tones_per_ch=tones, fs=fs)
+    data = synth.mk5b_frames(raw) if framed else raw
+    m, auto, _, pc, core = run_device(s, [data], channels=channels, flags=FLAG_MK5B_FRAMED if framed else 0)
+    for j, ch in enumerate(channels):
+        os_ = oracle_settings(oracle, hip.derive_settings(**dict(s.keys, UseFile1Channel=ch + 1)))
+        m64, a64, _, p64 = oracle.domaths(os_, [raw], 64)
+        assert m == m64 == 3
+        x = oracle.unpack(os_, raw, nsamp, ch).astype(np.float64)
+        for sp in range(m):
+            nff = K if sp < 2 else 2
+            acc = np.zeros(64)
+            for k in range(nff):
+                # the reference folds FFTs whose running count (across
+                # spectra) is a multiple of the overlap factor
+                if (sp * K + k) % 2:
+                    continue
+                f0 = (sp * (K + 1) + k) * n // 2
+                acc = oracle.pcal_fold64(x[f0:f0 + n], fs, 10e3, 32, 3200, rot32=True, acc=acc)
+            pref = acc[0::2] + 1j * acc[1::2]
+            ep = relerr(pc[j][sp], pref)
+            ep64 = relerr(pc[j][sp], p64[0][sp])
+            tg = core.pcal_tones(j, sp)
+            et = relerr(tg, oracle.pcal_tones64(acc, len(tg), 1e6, fs))
+            e = relerr(auto[j][sp], a64[0][sp])
+            say("  ch%d spectrum %d: auto %.3e pcal %.3e (vs fp64 rotator %.3e) tones %.3e" % (ch, sp, e, ep, ep64,
+                                                                                              et))
+            assert e <= TOL and ep <= 1e-6 and ep64 <= 1e-5 and et <= 1e-6
+    core.finalize()
