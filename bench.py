@@ -481,6 +481,51 @@ def pcal_extractor_leg(torch, nsamples=1 << 28, reps=5, cpu=True):
     return out
 
 
+def trace_busy(trace_dir, top=6):
+    """GPU busy fraction from a rocprofv3 kernel trace (CSV under trace_dir):
+    dispatches are cut into activity windows at idle stretches over 2 ms, and
+    per window the union of kernel intervals over its span is the busy share
+    (kernels overlapping on several streams count once).  No GPU needed."""
+    import collections
+    import csv
+    import glob
+    paths = glob.glob(os.path.join(trace_dir, "**", "*kernel_trace.csv"), recursive=True)
+    if not paths:
+        raise SystemExit("no *kernel_trace.csv under %s" % trace_dir)
+    rows = []
+    with open(paths[0]) as fh:
+        for d in csv.DictReader(fh):
+            rows.append((int(d["Start_Timestamp"]), int(d["End_Timestamp"]), d["Kernel_Name"].split("(")[0]))
+    rows.sort()
+    wins, cur, end = [], [rows[0]], rows[0][1]
+    for x in rows[1:]:
+        if x[0] - end > 2_000_000:
+            wins.append(cur)
+            cur, end = [x], x[1]
+        else:
+            cur.append(x)
+            end = max(end, x[1])
+    wins.append(cur)
+    out = []
+    for w in sorted(wins, key=lambda w: -(max(e for _, e, _ in w) - w[0][0]))[:top]:
+        t0, t1 = w[0][0], max(e for _, e, _ in w)
+        busy, cover, gaps = 0, t0, []
+        ks = collections.Counter()
+        for s0, e0, k in w:
+            if s0 > cover:
+                gaps.append(s0 - cover)
+            busy += max(0, e0 - max(s0, cover))
+            cover = max(cover, e0)
+            ks[k] += e0 - s0
+        big = [g for g in gaps if g > 10_000]
+        out.append({"span_ms": round((t1 - t0) / 1e6, 2), "dispatches": len(w),
+                    "busy_pct": round(100.0 * busy / (t1 - t0), 2), "gaps_over_10us": len(big),
+                    "gap_ms_over_10us": round(sum(big) / 1e6, 2),
+                    "largest_gaps_us": [round(g / 1e3, 1) for g in sorted(gaps, reverse=True)[:5]],
+                    "top_kernels_ms": [(k, round(v / 1e6, 2)) for k, v in ks.most_common(5)]})
+    return {"dispatches": len(rows), "windows": len(wins), "longest": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -509,7 +554,12 @@ def main():
     ap.add_argument("--only-serve", action="store_true",
                     help="run only the serving replays, one per rate of --serve-rates")
     ap.add_argument("--serve-rates", default="200", help="comma-separated job rates for --only-serve")
+    ap.add_argument("--trace-busy", metavar="DIR",
+                    help="print the GPU busy share of a rocprofv3 kernel trace under DIR and exit (no GPU)")
     args = ap.parse_args()
+    if args.trace_busy:
+        print(json.dumps(trace_busy(args.trace_busy)), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the workload is one GPU's share, and "
               "no multi-GPU run is launched", file=sys.stderr)

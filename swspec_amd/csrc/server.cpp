@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -27,7 +28,19 @@ using namespace swspec;
 namespace {
 
 constexpr int kStreams = 2;
-constexpr size_t kYScratch = 512ull << 20;   // per stream: pass-1 output of one launch
+// Per stream: the pass-1 output of one launch, which bounds the FFTs a
+// two-pass job advances per step (4-channel 3.2M-point jobs: 40 at 2 GiB,
+// 10 at 512 MiB; the 64-stream batch ran 3-4% faster at 2 GiB, graph and
+// plain).  SWSPEC_SERVER_SCRATCH_MB overrides (64..16384).
+static size_t y_scratch_bytes()
+{
+    static const size_t v = [] {
+        const char* e = getenv("SWSPEC_SERVER_SCRATCH_MB");
+        const long long mb = e ? atoll(e) : 2048;
+        return (size_t)std::min(16384LL, std::max(64LL, mb)) << 20;
+    }();
+    return v;
+}
 constexpr int kMaxG = 64;                    // FFTs per job per chunk with phase-cal (mask bits)
 constexpr int kMaxGPlain = 1024;             // FFTs per job per chunk without phase-cal
 constexpr int kMaxGShort = 16384;            // the same for single-pass plans (no intermediate)
@@ -142,7 +155,7 @@ extern "C" int swspec_hip_server_create(int device, int64_t budget, int max_acti
             swspec_hip_server_destroy(s);
             return SWSPEC_EDEVICE;
         }
-        if (hipMalloc(&s->yscr[i], kYScratch) != hipSuccess) {
+        if (hipMalloc(&s->yscr[i], y_scratch_bytes()) != hipSuccess) {
             swspec_hip_server_destroy(s);
             return SWSPEC_ENOMEM;
         }
@@ -320,14 +333,14 @@ static int step_enqueue(swspec_hip_server* s, bool capture, std::vector<SJob*>* 
         // jobs per launch: keep >= 8 FFTs per job per chunk when the scratch
         // allows (single-pass plans use none)
         int nj = p->single ? kMaxJobs
-                           : (int)std::max(1.0, std::min((double)kMaxJobs, std::floor((double)kYScratch / (8.0 * per))));
+                           : (int)std::max(1.0, std::min((double)kMaxJobs, std::floor((double)y_scratch_bytes() / (8.0 * per))));
         for (size_t b = 0; b < js.size(); b += nj) {
             const int n = (int)std::min<size_t>(nj, js.size() - b);
             bool anyp = false;
             for (int k = 0; k < n; k++) anyp |= js[b + k]->pcal;
             const double gmax = anyp ? (double)kMaxG : (double)(p->single ? kMaxGShort : kMaxGPlain);
             const int G = p->single ? (int)gmax
-                                    : (int)std::max(1.0, std::min(gmax, std::floor((double)kYScratch / (n * per))));
+                                    : (int)std::max(1.0, std::min(gmax, std::floor((double)y_scratch_bytes() / (n * per))));
             const int si = launch++ % kStreams;
             hipStream_t st = s->st[si];
             used[si] = true;
