@@ -639,10 +639,11 @@ def main():
             tf = fps[k] * samples_per_step * args.steps / (ms * 1e-3) / 1e12 if fps[k] else 0.0
             per_kernel[k] = {"ms_per_step": ms / args.steps, "launches_per_step": n / args.steps,
                              "avg_launch_us": ms / n * 1e3, "algorithmic_tflops": tf}
-    # the phase-cal fold runs on its own stream beside the FFT passes: its
-    # event time is wall time shared with them, not the kernel's own duration
+    # the phase-cal fold runs between the passes on the core's stream unless
+    # SWSPEC_PCAL_SIDE=1 gives it a stream of its own; then its event time is
+    # wall time shared with the passes, not the kernel's own duration
     if "pcal" in per_kernel:
-        per_kernel["pcal"]["overlapped_with_fft_passes"] = True
+        per_kernel["pcal"]["overlapped_with_fft_passes"] = os.environ.get("SWSPEC_PCAL_SIDE", "0") not in ("", "0")
     dom = max((k for k in per_kernel if k != "pcal"), key=lambda k: per_kernel[k]["ms_per_step"])
     launches = per_kernel[dom]["launches_per_step"]
     jobs_per_launch = int(round(NCH * NFFT / launches))

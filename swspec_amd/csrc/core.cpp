@@ -476,7 +476,16 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
     HIPCHK(c, hipEventCreate(&c->ev1));
     // VLBA phase-cal reads the per-chunk blank-zone table the main stream
     // rewrites, so it stays on the main stream
-    if (cfg->extract_pcal && !c->vlba) {
+    // The phase-cal fold runs on the main stream between the passes: with the
+    // grouped fold (each raw word read once for a source's channels) that
+    // measured as fast as a second stream beside them (4.43 ms per 3.2M
+    // 4-channel step either way), and the kernel keeps its own duration in a
+    // trace.  SWSPEC_PCAL_SIDE=1 puts it on a stream of its own.
+    static const bool pcal_side = [] {
+        const char* e = getenv("SWSPEC_PCAL_SIDE");
+        return e && atoi(e) != 0;
+    }();
+    if (cfg->extract_pcal && !c->vlba && pcal_side) {
         HIPCHK(c, hipStreamCreateWithFlags(&c->pstream, hipStreamNonBlocking));   // default priority
         for (hipEvent_t* e : {&c->ev_in, &c->ev_pc, &c->ev_em})
             HIPCHK(c, hipEventCreateWithFlags(e, hipEventDisableTiming));

@@ -199,6 +199,9 @@ struct XpolDev {
     float* acc = nullptr;          // (re, im, comp_re, comp_im) per bin, permuted layout
 };
 
+// Level-1 blocks per job of the phase-cal partial reduce (k_pcal_reduce1).
+constexpr int kPcalRedBlocks = 32;
+
 struct PcalGeom {
     int T = 0;        // tone bins
     int L = 0;        // rotator length
@@ -210,6 +213,10 @@ struct PcalGeom {
     int threads_fast = 0;
     int ppb = 0;      // rotator periods per block (fast kernel)
     int gpt = 0;      // groups of 4 positions per thread (fast kernel)
+    // grouped fast kernel (k_pcal_group): blocks of threads_grp threads, one
+    // group of 4 positions each; npos_grp blocks cover the L/4 groups, times
+    // the period chunks of ppb_grp periods (nblk_grp blocks per FFT)
+    int threads_grp = 0, npos_grp = 0, ppb_grp = 0, nblk_grp = 0;
     int nblk_max = 0; // partial-sum slots per FFT to allocate
     const float2* d_rot = nullptr;  // FP32 rotator (FP64-computed), L entries
 };

@@ -1855,39 +1855,212 @@ __global__ void __launch_bounds__(1024) k_pcal_fast(PcArgs a)
     }
 }
 
+// Grouped fold: the NCH channels of one multi-channel source in one thread.
+// Jobs z*NCH .. z*NCH+NCH-1 read the same bytes with the same layout and
+// phase-cal mask and differ only in their bit shift (the launcher checks), so
+// each raw word is loaded once for all of them instead of once per channel.
+// One word holds the channel byte of 4 time samples (with 2 bytes per time
+// sample, STEP 2, the 4 bytes are first gathered from an 8-byte group by
+// v_perm_b32), and all channels of that byte are counted together in SIMD-
+// within-a-register lanes.  One group of 4 rotator positions per thread, so
+// the integer counts stay in registers until the FP64 rotator products at the
+// end.
+template <int STEP, bool FR, int NCH>
+__global__ void __launch_bounds__(1024) k_pcal_group(PcArgs a)
+{
+    static_assert(STEP == 1 || STEP == 2, "multi-channel layouts only");
+    extern __shared__ double2 red4[];   // [nth][4]
+    const int z = blockIdx.z;
+    const JobDev J = a.jobs[z * NCH];
+    const int f = blockIdx.y, blk = blockIdx.x;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int T = a.g.T, L = a.g.L;
+    const int pchunk = blk / a.g.npos_grp;
+    const int g = (blk - pchunk * a.g.npos_grp) * nth + tid;
+    const bool on = ((J.pcal_mask >> f) & 1ull) && g < (L >> 2);
+    int A[NCH][4], Bc[NCH][4];
+#pragma unroll
+    for (int c = 0; c < NCH; c++)
+#pragma unroll
+        for (int u = 0; u < 4; u++) A[c][u] = Bc[c][u] = 0;
+    if (on) {
+        const uint8_t* __restrict__ raw = job_fft_ptr(J, f);
+        // framed Mk5B as in k_pcal_fast: an aligned word (STEP 1) or 8-byte
+        // group (STEP 2) never straddles a 10000-byte payload
+        const unsigned pay = FR ? (unsigned)(J.pay0 + (long long)f * J.fft_stride) : 0u;
+        auto at = [&](long long P) -> const uint8_t* {
+            if constexpr (!FR) {
+                return raw + P;
+            } else {
+                const unsigned Q = pay + (unsigned)P;
+                return J.raw + (Q + 16u * (Q / 10000u + 1u));
+            }
+        };
+        const int nper = (int)(a.N / L);
+        const int p0 = pchunk * a.g.ppb_grp;
+        const int p1 = min(p0 + a.g.ppb_grp, nper);
+        const int cbyte = J.up.cbyte;
+        int sh[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; c++) sh[c] = a.jobs[z * NCH + c].up.shift;
+        // STEP 2: bytes cbyte and cbyte + 2 of each dword of the 8-byte group
+        const unsigned sel = (unsigned)cbyte | ((unsigned)(cbyte + 2) << 8) | ((unsigned)(cbyte + 4) << 16) |
+                             ((unsigned)(cbyte + 6) << 24);
+        // The channel byte of each of the 4 time samples of a word holds up to
+        // 4 two-bit fields at shifts 0, 2, 4, 6 (bit 0 sign, bit 1 magnitude).
+        // Magnitude, sign and magnitude-and-sign bits of all of them are
+        // counted at once: shifts 0 and 4 (E) and 2 and 6 (O) in separate
+        // registers as nibble lanes (8 per register: 4 samples x 2 shifts),
+        // moved into byte lanes every 15 periods (nibble capacity) and read
+        // out per channel every 255 periods (byte capacity):
+        //   A = (np - #big) - 2 (#neg - #big&neg),  B = #big - 2 #big&neg.
+        constexpr unsigned K5 = 0x55555555u, K1 = 0x11111111u, K0F = 0x0F0F0F0Fu;
+        int slot[NCH];   // byte-lane accumulator of the channel's shift: sh / 2
+#pragma unroll
+        for (int c = 0; c < NCH; c++) slot[c] = sh[c] >> 1;
+        auto pick = [](const unsigned (&v)[4], int k) { return k == 0 ? v[0] : k == 1 ? v[1] : k == 2 ? v[2] : v[3]; };
+        for (int pc = p0; pc < p1; pc += 255) {
+            const int pe = min(pc + 255, p1);
+            unsigned bm[4] = {0u, 0u, 0u, 0u}, bs[4] = {0u, 0u, 0u, 0u}, bx[4] = {0u, 0u, 0u, 0u};
+            for (int pq = pc; pq < pe; pq += 15) {
+                unsigned mE = 0u, mO = 0u, sE = 0u, sO = 0u, xE = 0u, xO = 0u;
+                // 15 periods, all loads issued together; periods past pe load a
+                // valid word (clamped) and count it as 0 (no bits set)
+#pragma unroll
+                for (int k = 0; k < 15; k++) {
+                    const int p = min(pq + k, pe - 1);
+                    unsigned wd;
+                    if constexpr (STEP == 1) {
+                        wd = *(const uint32_t*)at((long long)p * L + 4 * g + cbyte);
+                    } else {
+                        const uint2 v = *(const uint2*)at(((long long)p * L + 4 * g) * 2);
+                        wd = __builtin_amdgcn_perm(v.y, v.x, sel);
+                    }
+                    wd = pq + k < pe ? wd : 0u;
+                    const unsigned S = wd & K5, Mg = (wd >> 1) & K5, X = S & Mg;
+                    mE += Mg & K1;
+                    mO += (Mg >> 2) & K1;
+                    sE += S & K1;
+                    sO += (S >> 2) & K1;
+                    xE += X & K1;
+                    xO += (X >> 2) & K1;
+                }
+                // byte-lane slots: 0 = shift 0 (E low nibble), 1 = shift 2 (O low),
+                // 2 = shift 4 (E high), 3 = shift 6 (O high)
+                bm[0] += mE & K0F; bm[2] += (mE >> 4) & K0F; bm[1] += mO & K0F; bm[3] += (mO >> 4) & K0F;
+                bs[0] += sE & K0F; bs[2] += (sE >> 4) & K0F; bs[1] += sO & K0F; bs[3] += (sO >> 4) & K0F;
+                bx[0] += xE & K0F; bx[2] += (xE >> 4) & K0F; bx[1] += xO & K0F; bx[3] += (xO >> 4) & K0F;
+            }
+            const int np = pe - pc;
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+                const unsigned m = pick(bm, slot[c]), sg = pick(bs, slot[c]), x = pick(bx, slot[c]);
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int nb = (int)((m >> (8 * u)) & 0xFFu);
+                    const int nn = (int)((sg >> (8 * u)) & 0xFFu);
+                    const int nbn = (int)((x >> (8 * u)) & 0xFFu);
+                    A[c][u] += (np - nb) - 2 * (nn - nbn);
+                    Bc[c][u] += nb - 2 * nbn;
+                }
+            }
+        }
+    }
+    float2 r[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) r[u] = on ? a.g.d_rot[4 * g + u] : make_float2(0.f, 0.f);
+    const int q = T >> 2;
+    // Lanes h and h' of a wave fold into the same bins when h = h' mod q.  With
+    // q dividing 64 and whole waves (the launcher's shape), a butterfly over
+    // the lane bits >= log2 q sums them in registers (every lane of a class
+    // ends with the same, fixed-order sum), lanes 0 .. q-1 keep one entry per
+    // wave in LDS, and bin t sums the waves' entries in order.
+    const bool wred = (64 % q) == 0 && (nth & 63) == 0;   // uniform
+    const int lane = tid & 63, wave = tid >> 6, nw = nth >> 6;
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+        if (c > 0) __syncthreads();   // the previous channel's reads of red4 are done
+        double vr[4], vi[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double v = (double)A[c][u] + (double)3.3359f * (double)Bc[c][u];
+            vr[u] = v * (double)r[u].x;
+            vi[u] = v * (double)r[u].y;
+        }
+        double* __restrict__ part_base = a.jobs[z * NCH + c].pcal_part;
+        if (wred) {
+            for (int m = q; m < 64; m <<= 1) {
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    vr[u] += __shfl_xor(vr[u], m);
+                    vi[u] += __shfl_xor(vi[u], m);
+                }
+            }
+            if (lane < q) {
+#pragma unroll
+                for (int u = 0; u < 4; u++) red4[4 * (wave * q + lane) + u] = make_double2(vr[u], vi[u]);
+            }
+            __syncthreads();
+            for (int t = tid; t < T; t += nth) {
+                const int u = t & 3, k = t >> 2;
+                double sr = 0.0, si = 0.0;
+                for (int w = 0; w < nw; w++) { sr += red4[4 * (w * q + k) + u].x; si += red4[4 * (w * q + k) + u].y; }
+                *(double2*)(part_base + 2 * (((long long)f * a.g.nblk_max + blk) * T + t)) = make_double2(sr, si);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; u++) red4[4 * tid + u] = make_double2(vr[u], vi[u]);
+            __syncthreads();
+            // bins as in k_pcal_fast (thread count a multiple of T/4)
+            for (int t = tid; t < T; t += nth) {
+                const int u = t & 3;
+                double sr = 0.0, si = 0.0;
+                for (int h = t >> 2; h < nth; h += q) { sr += red4[4 * h + u].x; si += red4[4 * h + u].y; }
+                *(double2*)(part_base + 2 * (((long long)f * a.g.nblk_max + blk) * T + t)) = make_double2(sr, si);
+            }
+        }
+    }
+}
+
 struct PcRedArgs {
     int T, nblk, stride, nfft;
+    int R;      // level-1 blocks per job; their sums go to FFT 0's slots stride - R .. stride - 1
     JobDev jobs[kMaxJobs];
 };
 
-// Sum the per-block partials of a chunk into the job's accumulator.  blockDim
-// = lanes * T: lane l of bin t sums entries l, l+lanes, ... of the (fft, block)
-// list, then lane 0 adds the lanes in order (fixed order, deterministic).
-// Entries are read four at a time (independent loads in flight), each added
-// in list order.
-__global__ void __launch_bounds__(1024) k_pcal_reduce(PcRedArgs a)
+// Two-level sum of a chunk's per-block partials into the job's accumulator,
+// in a fixed order (deterministic).  The (FFT, block) entries of a job,
+// e = f * nblk + b, are cut into R contiguous ranges.  Level 1, block (job,
+// r): lane l of bin t sums entries l, l + lanes, ... of range r (all loads of
+// a lane issued together), lanes added in order, result to FFT 0's spare slot
+// stride - R + r (no fold writes there: stride = nblk_max includes the R spare
+// slots).  Level 2: bin t adds the R range sums in order to the accumulator.
+__global__ void __launch_bounds__(1024) k_pcal_reduce1(PcRedArgs a)
 {
     __shared__ double2 red[1024];
     const JobDev J = a.jobs[blockIdx.x];
+    const int r = blockIdx.y;
     const int T = a.T;
     const int tid = threadIdx.x, lanes = blockDim.x / T;
     const int t = tid % T, l = tid / T;
-    double sr = 0.0, si = 0.0;
     const int nent = a.nfft * a.nblk;
-    auto entry = [&](int e, double2& v) {
-        const int f = e / a.nblk, b = e - f * a.nblk;
-        const bool on = e < nent && ((J.pcal_mask >> f) & 1ull);
-        const int fe = on ? f : 0, be = on ? b : 0;
-        v = *(const double2*)(J.pcal_part + 2 * (((long long)fe * a.stride + be) * T + t));
-        return on;
-    };
-    for (int e = l; e < nent; e += 4 * lanes) {
-        double2 v[4];
-        bool on[4];
+    const int per = (nent + a.R - 1) / a.R;
+    const int e0 = r * per, e1 = min(e0 + per, nent);
+    double sr = 0.0, si = 0.0;
+    constexpr int U = 8;
+    for (int e = e0 + l; e < e1; e += U * lanes) {
+        double2 v[U];
+        bool on[U];
 #pragma unroll
-        for (int u = 0; u < 4; u++) on[u] = entry(e + u * lanes, v[u]);
+        for (int u = 0; u < U; u++) {
+            const int ee = e + u * lanes;
+            const int f = ee / a.nblk, b = ee - f * a.nblk;
+            on[u] = ee < e1 && ((J.pcal_mask >> f) & 1ull);
+            const int fe = on[u] ? f : 0, be = on[u] ? b : 0;
+            v[u] = *(const double2*)(J.pcal_part + 2 * (((long long)fe * a.stride + be) * T + t));
+        }
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int u = 0; u < U; u++)
             if (on[u]) {
                 sr += v[u].x;
                 si += v[u].y;
@@ -1897,6 +2070,28 @@ __global__ void __launch_bounds__(1024) k_pcal_reduce(PcRedArgs a)
     __syncthreads();
     if (l == 0) {
         for (int k = 1; k < lanes; k++) { sr += red[k * T + t].x; si += red[k * T + t].y; }
+        *(double2*)(J.pcal_part + 2 * ((long long)(a.stride - a.R + r) * T + t)) = make_double2(sr, si);
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_pcal_reduce2(PcRedArgs a)
+{
+    const JobDev J = a.jobs[blockIdx.x];
+    const int T = a.T;
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        double sr = 0.0, si = 0.0;
+        for (int r0 = 0; r0 < a.R; r0 += 8) {   // 8 loads in flight, added in order
+            double2 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                v[u] = *(const double2*)(J.pcal_part + 2 * ((long long)(a.stride - a.R + min(r0 + u, a.R - 1)) * T + t));
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (r0 + u < a.R) {
+                    sr += v[u].x;
+                    si += v[u].y;
+                }
+        }
         J.pcal_acc[2 * t] += sr;
         J.pcal_acc[2 * t + 1] += si;
     }
@@ -2305,6 +2500,7 @@ void pcal_geometry(PcalGeom& g, long long N)
     g.chunk = (int)chunk;
     g.nblk = (int)((N + chunk - 1) / chunk);
     g.fast = 0;
+    g.nblk_grp = 0;
     // fast kernel: groups of 4 positions, 4*nth % T == 0, <= 4 groups per thread
     if ((L % 4) == 0 && (T % 4) == 0 && (N % L) == 0) {
         const int ngroups = L / 4, q = T / 4;
@@ -2322,8 +2518,42 @@ void pcal_geometry(PcalGeom& g, long long N)
                 g.nblk_fast = (int)((nper + g.ppb - 1) / g.ppb);
             }
         }
+        // grouped fold (a source's channels per thread): blocks of <= 256
+        // threads, so that their LDS (64 B per thread) fits beside the FFT
+        // passes' workgroups, a multiple of T/4 threads (every block then folds
+        // into the same bins)
+        int npos = (ngroups + 255) / 256;
+        int thg = (ngroups + npos - 1) / npos;
+        thg = ((thg + q - 1) / q) * q;
+        if (64 % q == 0) {
+            // whole waves (the kernel's in-register wave sums): the block size
+            // of 64 .. 256 threads that pads the L/4 groups least
+            int best = 0;
+            for (int t : {256, 192, 128, 64}) {
+                const int np = (ngroups + t - 1) / t;
+                if (best == 0 || np * t < npos * best) { best = t; npos = np; }
+            }
+            thg = best;
+            npos = (ngroups + thg - 1) / thg;
+        }
+        if (thg <= 1024) {
+            static const int nbp_env = [] {
+                const char* e = getenv("SWSPEC_PCAL_GRP_NB");
+                return e ? atoi(e) : 0;
+            }();
+            const long long nper = N / L;
+            // 10 period chunks per FFT (3.2M points, L = 3200: 105 periods per
+            // thread) measured best of 5 to 50 on the 4-channel Mk5B workload
+            const int nbp = (int)std::min<long long>(nper, nbp_env > 0 ? nbp_env : 10);
+            g.threads_grp = thg;
+            g.npos_grp = npos;
+            g.ppb_grp = (int)((nper + nbp - 1) / nbp);
+            g.ppb_grp = std::min<long long>(((g.ppb_grp + 14) / 15) * 15, std::max<long long>(nper, 1));   // whole load batches
+            g.nblk_grp = npos * (int)((nper + g.ppb_grp - 1) / g.ppb_grp);
+        }
     }
-    g.nblk_max = std::max(g.nblk, g.nblk_fast);
+    // + the level-1 reduce's spare slots (k_pcal_reduce1)
+    g.nblk_max = std::max(g.nblk, std::max(g.nblk_fast, g.nblk_grp)) + kPcalRedBlocks;
 }
 
 hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int njobs, int nfft, hipStream_t s)
@@ -2352,11 +2582,48 @@ hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int n
     for (int j = 0; j < njobs; j++) nfr += jobs[j].up.framed != 0;
     if (nfr != 0 && nfr != njobs) step = -2;   // framed and payload jobs mixed: generic fold
     const bool fast = g.fast && aligned && (step == 0 || step == 1 || step == 2 || step == 4);
-    if (fast) {
-        // one job (channel) per blockIdx.z: folding the 4 channels of one source
-        // per thread (each raw word read once) measured 1.5x slower on the
-        // 4-channel 3.2M workload (a quarter of the workgroups, 4x the integer
-        // work per thread; the re-reads hit in cache)
+    // channels of one multi-channel source fold together (k_pcal_group) when
+    // consecutive jobs read the same bytes and differ only in their bit shift
+    int grp = 1;
+    static const int group_env = [] {
+        const char* e = getenv("SWSPEC_PCAL_GROUP");
+        return e ? atoi(e) : -1;
+    }();
+    if (fast && group_env != 0 && g.nblk_grp > 0 && (step == 1 || step == 2)) {
+        for (int G : {4, 2}) {
+            if (njobs % G) continue;
+            bool ok = true;
+            for (int j = 0; ok && j < njobs; j++)   // fields within the byte (byte-lane counting)
+                ok = jobs[j].up.shift >= 0 && jobs[j].up.shift <= 6 && (jobs[j].up.shift & 1) == 0;
+            for (int k = 0; ok && k < njobs; k += G)
+                for (int c = 1; ok && c < G; c++) {
+                    const JobDev& x = jobs[k];
+                    const JobDev& y = jobs[k + c];
+                    ok = y.raw == x.raw && y.fft_stride == x.fft_stride && y.fft0 == x.fft0 && y.nwrap == x.nwrap &&
+                         y.pcal_mask == x.pcal_mask && y.pay0 == x.pay0 && y.up.kind == x.up.kind &&
+                         y.up.step == x.up.step && y.up.cbyte == x.up.cbyte && y.up.framed == x.up.framed;
+                }
+            if (ok) { grp = G; break; }
+        }
+    }
+    if (fast && grp > 1) {
+        const dim3 grid(g.nblk_grp, nfft, njobs / grp), block(g.threads_grp);
+        const size_t lds = (size_t)g.threads_grp * 4 * sizeof(double2);
+#define SWSPEC_PCAL_GRP(ST, FRM)                                                                        \
+    {                                                                                                 \
+        if (grp == 4) hipLaunchKernelGGL((k_pcal_group<ST, FRM, 4>), grid, block, lds, s, a);           \
+        else hipLaunchKernelGGL((k_pcal_group<ST, FRM, 2>), grid, block, lds, s, a);                    \
+    }
+        if (step == 1) {
+            if (framed) SWSPEC_PCAL_GRP(1, true)
+            else SWSPEC_PCAL_GRP(1, false)
+        } else {
+            if (framed) SWSPEC_PCAL_GRP(2, true)
+            else SWSPEC_PCAL_GRP(2, false)
+        }
+#undef SWSPEC_PCAL_GRP
+    } else if (fast) {
+        // one job (channel) per blockIdx.z
         const dim3 grid(g.nblk_fast, nfft, njobs), block(g.threads_fast);
         const size_t lds = (size_t)g.threads_fast * 4 * sizeof(double2);
 #define SWSPEC_PCAL_FAST(ST, FRM)                                                                       \
@@ -2383,10 +2650,14 @@ hipError_t launch_pcal(const PcalGeom& g, long long N, const JobDev* jobs, int n
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     PcRedArgs r;
-    r.T = g.T; r.nblk = fast ? g.nblk_fast : g.nblk; r.stride = g.nblk_max; r.nfft = nfft;
+    r.T = g.T; r.nblk = !fast ? g.nblk : (grp > 1 ? g.nblk_grp : g.nblk_fast); r.stride = g.nblk_max; r.nfft = nfft;
+    r.R = kPcalRedBlocks;
     for (int j = 0; j < njobs; j++) r.jobs[j] = jobs[j];
-    const int th = g.T <= 1024 ? (1024 / g.T) * g.T : g.T;   // lanes * T, <= 1024
-    hipLaunchKernelGGL(k_pcal_reduce, dim3(njobs), dim3(th), 0, s, r);
+    const int th = g.T <= 256 ? (256 / g.T) * g.T : g.T;   // lanes * T
+    hipLaunchKernelGGL(k_pcal_reduce1, dim3(njobs, r.R), dim3(th), 0, s, r);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pcal_reduce2, dim3(njobs), dim3(std::min(g.T, 1024)), 0, s, r);
     return hipGetLastError();
 }
 

@@ -725,11 +725,15 @@ def test_pcal_fold_bins_beyond_thread_count(hip, oracle, bw, off, nch, fmt):
 # ---------------------------------------------------------------- phase-cal at 3.2M with overlap
 
 @pytest.mark.parametrize("fmt,nch,channels,framed", [("RawSigned", 1, [0], False), ("Mk5B", 4, [0, 1, 2, 3], False),
-                                                    ("Mk5B", 8, [1, 6], False), ("Mk5B", 4, [2, 3], True)])
+                                                    ("Mk5B", 8, [1, 6], False), ("Mk5B", 4, [2, 3], True),
+                                                    ("Mk5B", 4, [1], False), ("Mk5B", 8, [4, 5, 6, 7], False)])
 def test_pcal_3p2m_overlap_partial_vs_fold64(hip, oracle, fmt, nch, channels, framed):
     """3.2M points with the 16-tone comb through every integer-count fold
-    instantiation (1-channel 2-bit, 4- and 8-channel Mk5B, framed Mk5B).
-    Each channel's folded vector against the FP64 fold with the FP32 rotator
+    instantiation: the per-channel fold (1-channel 2-bit; one channel of a
+    4-channel Mk5B byte; 8-channel Mk5B channels 1 and 6, which sit in
+    different bytes) and the grouped fold that reads a byte once for all its
+    channels (4 channels of one byte, 4-channel and 8-channel Mk5B; 2 channels,
+    framed Mk5B).  Each channel's folded vector against the FP64 fold with the FP32 rotator
     (<= 1e-6), its tones, the oracle's doMaths phase-cal and spectrum; two
     spectra (overlap 2, 3 FFTs each) and a partial third, phase-cal on every
     second FFT of the run (TaskCoreIPP.cpp:633-693)."""
