@@ -149,6 +149,7 @@ struct Plan {
     float2* d_rA = nullptr;     // W_N^{k1}, k1 < N1
     float2* d_rB = nullptr;     // W_N^{N1 k2}, k2 < N2
     float2* d_winphi = nullptr; // N1 window row phases (cos, sin)
+    float4* d_wcol = nullptr;   // window column constants (P, Q, R, S) per (column c < N2, sample r): [2c + r]
     uint4* d_mft = nullptr;     // radix-64 MFMA stage: DFT-64 A operands [8][4][hi, lo][64 lanes]
     WindowGen win;
 };

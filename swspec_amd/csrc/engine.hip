@@ -654,6 +654,8 @@ struct P1Args {
     const uint4* mft; // MF: DFT-64 operand table
     WindowGen win;
     int zfast;        // jobs share raw bytes: job index fastest in the workgroup order
+    int rowfill;      // 1: row fill (fill_rows, k_pass1<.., RF = B>), 0: column fill
+    const float4* wcol;  // window column constants (row fill)
     JobDev jobs[kMaxJobs];
 };
 
@@ -825,9 +827,95 @@ __device__ __forceinline__ void fill_points(float2* __restrict__ lp, const uint8
     else fill_w(I2());
 }
 
+// Row fill: decode + window of one FFT's points with each thread taking whole
+// rows of the workgroup's BC columns (BC = 4 or 8), for layouts with one byte
+// per time sample (2-bit multi-channel, payload or framed Mk5B).  Row e's BC
+// points z[N2 e + c0 + b] are 2 BC consecutive raw bytes: one 8- or 16-byte
+// load (the launcher checks the alignment; framed: an aligned group never
+// straddles a 10000-byte payload).  The window of sample r of column c0 + b
+// in row e is
+//   w = a0 + cos(phi_e) P + sin(phi_e) Q + cos(2 phi_e) R + sin(2 phi_e) S
+// with (cos, sin)(phi_e) the row's table entry and (P, Q, R, S) the column's
+// constants (the column fill's per-thread constants, tabulated per column on
+// the host in FP64): the same for every thread of the workgroup, so they sit
+// in scalar registers.  lp[e*BC + b] is written as one 32- or 64-byte row
+// segment.  (A 16-entry LDS table of level pairs indexed by the two fields
+// measured no faster than map2bit's integer build.)
+template <int BC, int WM, bool FR>
+__device__ __forceinline__ void fill_rows(float2* __restrict__ lp, const uint8_t* __restrict__ raw,
+                                          const Unpack& up, long long pay,
+                                          const WindowGen& wg, const float4* __restrict__ wcol, int tid, int nth,
+                                          int N1, int N2, int c0)
+{
+    static_assert(BC == 4 || BC == 8, "row fill: 4 or 8 columns");
+    using Raw = typename std::conditional<BC == 4, uint2, uint4>::type;
+    constexpr int NW = BC / 2;   // raw dwords per row segment
+    float4 K[BC][2];
+#pragma unroll
+    for (int b = 0; b < BC; b++)
+#pragma unroll
+        for (int r = 0; r < 2; r++) K[b][r] = WM ? wcol[2 * (c0 + b) + r] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float a0 = wg.a0;
+    const unsigned sh = (unsigned)up.shift;
+    constexpr int U = 4;   // rows in flight per thread
+    for (int e0 = tid; e0 < N1; e0 += U * nth) {
+        Raw q[U];
+        float2 z[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int e = min(e0 + u * nth, N1 - 1);   // clamped: values unused past N1
+            const unsigned off = 2u * (__umul24((unsigned)N2, (unsigned)e) + (unsigned)c0);
+            const uint8_t* src;
+            if constexpr (FR) src = raw + mk5b_frame_addr(pay + (long long)off);
+            else src = raw + off;
+            q[u] = *(const Raw*)src;
+            z[u] = WM ? wg.phi[e] : make_float2(1.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int e = e0 + u * nth;
+            if (e >= N1) break;
+            float c2 = 0.f, s2 = 0.f;
+            if constexpr (WM == 2) {
+                c2 = z[u].x * z[u].x - z[u].y * z[u].y;
+                s2 = 2.f * z[u].x * z[u].y;
+            }
+            const unsigned* qw = (const unsigned*)&q[u];
+            float2 pt[BC];
+#pragma unroll
+            for (int d = 0; d < NW; d++) {
+                // byte lanes: fields of samples 2b, 2b+1 (b = 2d, 2d+1) of this channel
+                const unsigned t = (qw[d] >> sh) & 0x03030303u;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int b = 2 * d + h;
+                    const unsigned f0 = (t >> (16 * h)) & 3u, f1 = (t >> (16 * h + 8)) & 3u;
+                    const float2 lv = make_float2(map2bit(f0), map2bit(f1));
+                    float w0 = 1.f, w1 = 1.f;
+                    if constexpr (WM != 0) {
+                        w0 = a0 + z[u].x * K[b][0].x + z[u].y * K[b][0].y;
+                        w1 = a0 + z[u].x * K[b][1].x + z[u].y * K[b][1].y;
+                        if constexpr (WM == 2) {
+                            w0 += c2 * K[b][0].z + s2 * K[b][0].w;
+                            w1 += c2 * K[b][1].z + s2 * K[b][1].w;
+                        }
+                    }
+                    pt[b] = make_float2(lv.x * w0, lv.y * w1);
+                }
+            }
+            float4* dst = (float4*)(lp + (size_t)e * BC);
+#pragma unroll
+            for (int b = 0; b < BC; b += 2) dst[b / 2] = make_float4(pt[b].x, pt[b].y, pt[b + 1].x, pt[b + 1].y);
+        }
+    }
+}
+
 // MF: the first column stage is radix 64 on the matrix cores (stage_mfma64;
 // one pipeline per workgroup, whole waves)
-template <int MAXPTS, bool CX, bool MF = false>
+// RF: row fill of RF = 4 or 8 columns (fill_rows; launcher: a.rowfill), 0:
+// column fill.  Separate instantiations, so that each is register-allocated
+// for its own fill.
+template <int MAXPTS, bool CX, bool MF = false, int RF = 0>
 __global__ void __launch_bounds__(512) k_pass1(P1Args a)
 {
     extern __shared__ float2 lds[];
@@ -853,10 +941,26 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     const int cb = c0 + bcol;
     const int total = N1 * B;
     const int estep = nth / B;
-    const WinCoef wc = win_coef(a.win, cb);
-    // MODE 3 (VLBA): samples decoded through the FFT's row of blank zones
-    const int2* __restrict__ brow = is_m5a(up) ? up.vblank + (size_t)(active ? f : 0) * up.vnseg : nullptr;
-    if (active) fill_points<MF>(lp, raw, up, pay, brow, a.win, wc, tid, nth, B, N1, N2, cb, et);
+    if constexpr (RF != 0) {
+        if (active) {
+            auto rows = [&](auto bc, auto fr) __attribute__((always_inline)) {
+                constexpr int BC = decltype(bc)::value;
+                constexpr bool FR = decltype(fr)::value;
+                const int wm = a.win.kind == 0 ? 0 : (a.win.a3 == 0.f ? 1 : 2);
+                if (wm == 0) fill_rows<BC, 0, FR>(lp, raw, up, pay, a.win, a.wcol, tid, nth, N1, N2, c0);
+                else if (wm == 1) fill_rows<BC, 1, FR>(lp, raw, up, pay, a.win, a.wcol, tid, nth, N1, N2, c0);
+                else fill_rows<BC, 2, FR>(lp, raw, up, pay, a.win, a.wcol, tid, nth, N1, N2, c0);
+            };
+            using IR = std::integral_constant<int, RF>;
+            if (up.framed) rows(IR(), std::true_type());
+            else rows(IR(), std::false_type());
+        }
+    } else {
+        const WinCoef wc = win_coef(a.win, cb);
+        // MODE 3 (VLBA): samples decoded through the FFT's row of blank zones
+        const int2* __restrict__ brow = is_m5a(up) ? up.vblank + (size_t)(active ? f : 0) * up.vnseg : nullptr;
+        if (active) fill_points<MF>(lp, raw, up, pay, brow, a.win, wc, tid, nth, B, N1, N2, cb, et);
+    }
     __syncthreads();
 
     // A plain last radix (2, 3, 4, 5, 8) after at least one LDS stage runs
@@ -2300,19 +2404,49 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
     for (int j = 0; j < njobs; j++) a.jobs[j] = jobs[j];
     dim3 grid(p.N2 / p.B, (nfft + a.fpw - 1) / a.fpw, njobs);
     const size_t lds = p1_lds(p) * a.fpw;
+    // Row fill: 4 or 8 columns, one pipeline, every job a one-byte-per-time-
+    // sample 2-bit layout whose row segments are aligned (payload: raw and FFT
+    // stride multiples of 16; framed: payload offsets multiples of 16, a
+    // segment never straddling a 10000-byte payload).  SWSPEC_P1_ROWFILL=0
+    // keeps the column fill.
+    a.rowfill = 0;
+    a.wcol = p.d_wcol;
+    static const int rowfill_env = [] {
+        const char* e = getenv("SWSPEC_P1_ROWFILL");
+        return e ? atoi(e) : -1;
+    }();
+    if (rowfill_env != 0 && !p.mfma64 && a.fpw == 1 && (p.B == 4 || p.B == 8) && p.N2 % 8 == 0 && p.d_wcol) {
+        bool ok = true;
+        for (int j = 0; ok && j < njobs; j++) {
+            const Unpack& u = jobs[j].up;
+            ok = u.kind == kUnpackMultiChannel && u.step == 1 && u.cbyte == 0 && u.shift >= 0 && u.shift <= 6 &&
+                 jobs[j].fft_stride % 16 == 0;
+            if (ok && u.framed) ok = jobs[j].pay0 % 16 == 0 && (uintptr_t)jobs[j].raw % 16 == 0;
+            else if (ok) ok = (uintptr_t)jobs[j].raw % 16 == 0;
+        }
+        a.rowfill = ok ? 1 : 0;
+    }
     const dim3 block(a.fpw * a.pth);
     if (p.mfma64) {
         if (p.p1_cx) hipLaunchKernelGGL((k_pass1<16, true, true>), grid, block, lds, s, a);
         else hipLaunchKernelGGL((k_pass1<16, false, true>), grid, block, lds, s, a);
         return hipGetLastError();
     }
+    const int rf = a.rowfill ? p.B : 0;
+#define SWSPEC_P1(MP, CXV)                                                                              \
+    {                                                                                                 \
+        if (rf == 4) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4>), grid, block, lds, s, a);         \
+        else if (rf == 8) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 8>), grid, block, lds, s, a);    \
+        else hipLaunchKernelGGL((k_pass1<MP, CXV, false, 0>), grid, block, lds, s, a);                 \
+    }
     switch (p.p1_maxpts * 2 + (p.p1_cx ? 1 : 0)) {
-    case 32: hipLaunchKernelGGL((k_pass1<16, false>), grid, block, lds, s, a); break;
-    case 33: hipLaunchKernelGGL((k_pass1<16, true>), grid, block, lds, s, a); break;
-    case 64: hipLaunchKernelGGL((k_pass1<32, false>), grid, block, lds, s, a); break;
-    case 65: hipLaunchKernelGGL((k_pass1<32, true>), grid, block, lds, s, a); break;
+    case 32: SWSPEC_P1(16, false) break;
+    case 33: SWSPEC_P1(16, true) break;
+    case 64: SWSPEC_P1(32, false) break;
+    case 65: SWSPEC_P1(32, true) break;
     default: return hipErrorInvalidValue;
     }
+#undef SWSPEC_P1
     return hipGetLastError();
 }
 
@@ -3331,6 +3465,22 @@ std::string plan_create(Plan& p, long long N, int window, hipStream_t stream, bo
         }
         if (upload(&p.d_winphi, ph, stream) != hipSuccess) return "device allocation failed (window)";
         p.win.phi = p.d_winphi;
+        // per-column constants of the row fill, the column fill's win_coef in
+        // FP64: sample r of column c has psi = 2 pi (2c + r + delta) / D
+        if (p.win.kind != 0 && !p.single) {
+            std::vector<float4> wc((size_t)2 * p.N2);
+            for (int c = 0; c < p.N2; c++)
+                for (int r = 0; r < 2; r++) {
+                    const double x = ((double)(2 * c + r) + p.win.delta) / p.win.D;
+                    const double a = 2.0 * M_PI * x;
+                    const double co = cos(a), si = sin(a);
+                    const double c2 = co * co - si * si, s2 = 2.0 * si * co;
+                    wc[(size_t)2 * c + r] = make_float4((float)((double)p.win.a1 * co + (double)p.win.a2 * si),
+                                                        (float)((double)p.win.a2 * co - (double)p.win.a1 * si),
+                                                        (float)((double)p.win.a3 * c2), (float)(-(double)p.win.a3 * s2));
+                }
+            if (upload(&p.d_wcol, wc, stream) != hipSuccess) return "device allocation failed (window columns)";
+        }
     }
     return "";
 }
@@ -3338,7 +3488,7 @@ std::string plan_create(Plan& p, long long N, int window, hipStream_t stream, bo
 void plan_destroy(Plan& p)
 {
     for (void* q : {(void*)p.d_stw1, (void*)p.d_stw2, (void*)p.d_twA, (void*)p.d_twB, (void*)p.d_rA,
-                    (void*)p.d_rB, (void*)p.d_winphi, (void*)p.d_mft})
+                    (void*)p.d_rB, (void*)p.d_winphi, (void*)p.d_mft, (void*)p.d_wcol})
         if (q) (void)hipFree(q);
     p.d_stw1 = p.d_stw2 = p.d_twA = p.d_twB = p.d_rA = p.d_rB = nullptr;
     p.d_winphi = nullptr;

@@ -106,6 +106,38 @@ def test_multichannel_mk5b_and_pcal(hip, oracle):
     core.finalize()
 
 
+@pytest.mark.parametrize("n,win,framed", [(320000, "Blackman", False), (1000000, "None", False),
+                                           (1 << 20, "Cosine", False), (3200000, "Hamming", False),
+                                           (3200000, "Blackman", True), (320000, "Hann", True)])
+def test_row_fill_windows_vs_oracle(hip, oracle, n, win, framed):
+    """The row fill of pass 1 (a thread decodes and windows whole rows of the
+    workgroup's 4 or 8 columns from one wide load; one byte per time sample)
+    for every window form: none, one cosine term (Cosine, Hann, Hamming) and
+    the cos(2 theta) term (Blackman), 4- and 8-column plans (3.2M: 4; 320000,
+    10^6, 2^20: 8), payload and framed Mk5B.  All four channels of a
+    4-channel Mk5B stream against the FP64 oracle."""
+    from swspec_amd import plan_info
+    from swspec_amd._lib import FLAG_MK5B_FRAMED
+    nf, fs = 2, 32e6
+    assert plan_info(n)["B"] in (4, 8)
+    say("row fill", n, win, "framed" if framed else "payload")
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=1, WindowType=win,
+                            BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=4, SourceFormat="Mk5B",
+                            MaxSourceBufferMB=4096)
+    payload = synth.raw_multi(11, 4, n * nf, tones_per_ch=[[(0.2, (3 + c) * 1e6, 0.1 * c)] for c in range(4)],
+                              fs=fs)
+    data = synth.mk5b_frames(payload) if framed else payload
+    m, auto, _, _, core = run_device(s, [data], channels=[0, 1, 2, 3], flags=FLAG_MK5B_FRAMED if framed else 0)
+    core.finalize()
+    assert m == 1
+    for ch in range(4):
+        os_ = oracle_settings(oracle, hip.derive_settings(**dict(s.keys, UseFile1Channel=ch + 1)))
+        _, a64, _, _ = oracle.domaths(os_, [payload], 64)
+        e = relerr(auto[ch], a64[0])
+        say("  ch%d spectrum %.3e" % (ch, e))
+        assert e <= TOL
+
+
 @pytest.mark.parametrize("nch,channels,framed", [(8, list(range(8)), False), (4, [1, 3], False),
                                                  (4, [0, 1, 2, 3], True), (8, [4, 6], True)])
 def test_channel_subsets_vs_oracle(hip, oracle, nch, channels, framed):
