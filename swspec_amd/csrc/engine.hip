@@ -829,7 +829,8 @@ __device__ __forceinline__ void fill_points(float2* __restrict__ lp, const uint8
 
 // Row fill: decode + window of one FFT's points with each thread taking whole
 // rows of the workgroup's BC columns (BC = 4 or 8), for layouts with one byte
-// per time sample (2-bit multi-channel, payload or framed Mk5B).  Row e's BC
+// per time sample (2-bit multi-channel, payload or framed Mk5B) and the
+// single-channel 2-bit layout.  Row e's BC
 // points z[N2 e + c0 + b] are 2 BC consecutive raw bytes: one 8- or 16-byte
 // load (the launcher checks the alignment; framed: an aligned group never
 // straddles a 10000-byte payload).  The window of sample r of column c0 + b
@@ -841,15 +842,19 @@ __device__ __forceinline__ void fill_points(float2* __restrict__ lp, const uint8
 // in scalar registers.  lp[e*BC + b] is written as one 32- or 64-byte row
 // segment.  (A 16-entry LDS table of level pairs indexed by the two fields
 // measured no faster than map2bit's integer build.)
-template <int BC, int WM, bool FR>
+// SC: single-channel 2-bit layout (4 samples per byte, oldest in the MSB):
+// the row's 2 BC samples are BC / 2 bytes, one 16- or 32-bit load.
+template <int BC, int WM, bool FR, bool SC = false>
 __device__ __forceinline__ void fill_rows(float2* __restrict__ lp, const uint8_t* __restrict__ raw,
                                           const Unpack& up, long long pay,
                                           const WindowGen& wg, const float4* __restrict__ wcol, int tid, int nth,
                                           int N1, int N2, int c0)
 {
     static_assert(BC == 4 || BC == 8, "row fill: 4 or 8 columns");
-    using Raw = typename std::conditional<BC == 4, uint2, uint4>::type;
-    constexpr int NW = BC / 2;   // raw dwords per row segment
+    static_assert(!(SC && FR), "framed input is multi-channel");
+    using Raw = typename std::conditional<SC, typename std::conditional<BC == 4, uint16_t, uint32_t>::type,
+                                          typename std::conditional<BC == 4, uint2, uint4>::type>::type;
+    constexpr int NW = SC ? 1 : BC / 2;   // raw dwords per row segment
     float4 K[BC][2];
 #pragma unroll
     for (int b = 0; b < BC; b++)
@@ -864,10 +869,11 @@ __device__ __forceinline__ void fill_rows(float2* __restrict__ lp, const uint8_t
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int e = min(e0 + u * nth, N1 - 1);   // clamped: values unused past N1
-            const unsigned off = 2u * (__umul24((unsigned)N2, (unsigned)e) + (unsigned)c0);
+            const unsigned pt0 = __umul24((unsigned)N2, (unsigned)e) + (unsigned)c0;   // first point of the row segment
             const uint8_t* src;
-            if constexpr (FR) src = raw + mk5b_frame_addr(pay + (long long)off);
-            else src = raw + off;
+            if constexpr (SC) src = raw + (pt0 >> 1);
+            else if constexpr (FR) src = raw + mk5b_frame_addr(pay + 2LL * pt0);
+            else src = raw + 2u * pt0;
             q[u] = *(const Raw*)src;
             z[u] = WM ? wg.phi[e] : make_float2(1.f, 0.f);
         }
@@ -880,16 +886,27 @@ __device__ __forceinline__ void fill_rows(float2* __restrict__ lp, const uint8_t
                 c2 = z[u].x * z[u].x - z[u].y * z[u].y;
                 s2 = 2.f * z[u].x * z[u].y;
             }
-            const unsigned* qw = (const unsigned*)&q[u];
             float2 pt[BC];
 #pragma unroll
             for (int d = 0; d < NW; d++) {
-                // byte lanes: fields of samples 2b, 2b+1 (b = 2d, 2d+1) of this channel
-                const unsigned t = (qw[d] >> sh) & 0x03030303u;
+                // multi-channel: byte lanes of dword d hold the fields of samples
+                // 2b, 2b+1 (b = 2d, 2d+1) of this channel; single channel: byte
+                // b/2 holds points b and b+1, oldest sample in the MSB
+                unsigned t;
+                if constexpr (SC) t = (unsigned)q[u];
+                else t = (((const unsigned*)&q[u])[d] >> sh) & 0x03030303u;
 #pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int b = 2 * d + h;
-                    const unsigned f0 = (t >> (16 * h)) & 3u, f1 = (t >> (16 * h + 8)) & 3u;
+                for (int h = 0; h < (SC ? BC : 2); h++) {
+                    const int b = SC ? h : 2 * d + h;
+                    unsigned f0, f1;
+                    if constexpr (SC) {
+                        const int s0 = 8 * (b >> 1) + 6 - 4 * (b & 1);
+                        f0 = (t >> s0) & 3u;
+                        f1 = (t >> (s0 - 2)) & 3u;
+                    } else {
+                        f0 = (t >> (16 * h)) & 3u;
+                        f1 = (t >> (16 * h + 8)) & 3u;
+                    }
                     const float2 lv = make_float2(map2bit(f0), map2bit(f1));
                     float w0 = 1.f, w1 = 1.f;
                     if constexpr (WM != 0) {
@@ -943,17 +960,17 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     const int estep = nth / B;
     if constexpr (RF != 0) {
         if (active) {
-            auto rows = [&](auto bc, auto fr) __attribute__((always_inline)) {
-                constexpr int BC = decltype(bc)::value;
+            auto rows = [&](auto fr, auto sc) __attribute__((always_inline)) {
                 constexpr bool FR = decltype(fr)::value;
+                constexpr bool SC = decltype(sc)::value;
                 const int wm = a.win.kind == 0 ? 0 : (a.win.a3 == 0.f ? 1 : 2);
-                if (wm == 0) fill_rows<BC, 0, FR>(lp, raw, up, pay, a.win, a.wcol, tid, nth, N1, N2, c0);
-                else if (wm == 1) fill_rows<BC, 1, FR>(lp, raw, up, pay, a.win, a.wcol, tid, nth, N1, N2, c0);
-                else fill_rows<BC, 2, FR>(lp, raw, up, pay, a.win, a.wcol, tid, nth, N1, N2, c0);
+                if (wm == 0) fill_rows<RF, 0, FR, SC>(lp, raw, up, pay, a.win, a.wcol, tid, nth, N1, N2, c0);
+                else if (wm == 1) fill_rows<RF, 1, FR, SC>(lp, raw, up, pay, a.win, a.wcol, tid, nth, N1, N2, c0);
+                else fill_rows<RF, 2, FR, SC>(lp, raw, up, pay, a.win, a.wcol, tid, nth, N1, N2, c0);
             };
-            using IR = std::integral_constant<int, RF>;
-            if (up.framed) rows(IR(), std::true_type());
-            else rows(IR(), std::false_type());
+            if (up.kind == kUnpackSingleChannel) rows(std::false_type(), std::true_type());
+            else if (up.framed) rows(std::true_type(), std::false_type());
+            else rows(std::false_type(), std::false_type());
         }
     } else {
         const WinCoef wc = win_coef(a.win, cb);
@@ -2419,6 +2436,10 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
         bool ok = true;
         for (int j = 0; ok && j < njobs; j++) {
             const Unpack& u = jobs[j].up;
+            if (u.kind == kUnpackSingleChannel) {   // BC / 2 bytes per row segment, aligned
+                ok = !u.framed && (uintptr_t)jobs[j].raw % 16 == 0 && jobs[j].fft_stride % 16 == 0;
+                continue;
+            }
             ok = u.kind == kUnpackMultiChannel && u.step == 1 && u.cbyte == 0 && u.shift >= 0 && u.shift <= 6 &&
                  jobs[j].fft_stride % 16 == 0;
             if (ok && u.framed) ok = jobs[j].pay0 % 16 == 0 && (uintptr_t)jobs[j].raw % 16 == 0;
