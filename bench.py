@@ -549,6 +549,7 @@ def main():
     ap.add_argument("--no-xpol", action="store_true", help="skip the 2^20-point cross-pol leg (FP32 vs MFMA stage)")
     ap.add_argument("--only-xpol", action="store_true", help="run only the 2^20-point cross-pol leg")
     ap.add_argument("--no-host", action="store_true", help="skip the host-Buffer (PCIe-inclusive) leg")
+    ap.add_argument("--no-framed", action="store_true", help="skip the on-disk Mk5B frame leg")
     ap.add_argument("--only-host", action="store_true", help="run only the host-Buffer (PCIe-inclusive) leg")
     ap.add_argument("--only-batch", action="store_true", help="run only the 64-stream batch leg (with autotuning)")
     ap.add_argument("--only-serve", action="store_true",
@@ -673,8 +674,10 @@ def main():
     assert np.isfinite(a0).all()
     core.finalize()
 
-    log("Mk5B frames as on disk (device-side header skip) ...")
-    framed = framed_cfg2_leg(torch, swspec_amd, raw, s)
+    framed = None
+    if not args.no_framed:
+        log("Mk5B frames as on disk (device-side header skip) ...")
+        framed = framed_cfg2_leg(torch, swspec_amd, raw, s)
 
     host = None
     if not args.no_host:
