@@ -511,13 +511,17 @@ static int core_setup(swspec_hip_core* c, const swspec_hip_cfg* cfg, int njobs, 
     }
     c->G = std::max(1, std::min(c->G, cfg->core_overlapped_ffts));
     {
-        // Row pass on its own stream for one-job cores whose spectra span
-        // several chunks: the column pass of the next chunk then fills the row
-        // pass's launch tail (single-channel 3.2M cores with phase-cal: +8%).
-        // Multi-job cores (4-channel 3.2M: -2%) and one-chunk spectra (no
-        // next chunk, only the stream handoffs) keep both passes on one
-        // stream.  SWSPEC_ROW_STREAM=1/0 forces it on/off.
-        bool on = c->njobs == 1 && cfg->core_overlapped_ffts > c->G;
+        // Row pass on its own stream (two Y buffers) when a spectrum spans
+        // several chunks, for one-job cores and for cores with phase-cal:
+        // the column pass and the phase-cal fold of the next chunk then run
+        // beside the row pass.  Single-channel 3.2M cores with phase-cal:
+        // +8%; the 4-channel 3.2M phase-cal step: 4.51 -> 4.32 ms (the fold
+        // hidden: 4.33 with phase-cal, 4.32 without on the same streams).
+        // Multi-job cores without phase-cal (4-channel 3.2M: 4.23 -> 4.32 ms)
+        // and one-chunk spectra (no next chunk, only the stream handoffs)
+        // keep both passes on one stream.  SWSPEC_ROW_STREAM=1/0 forces it
+        // on/off.
+        bool on = (c->njobs == 1 || cfg->extract_pcal) && cfg->core_overlapped_ffts > c->G;
         if (const char* e = getenv("SWSPEC_ROW_STREAM")) on = atoi(e) != 0;
         if (single) on = false;   // one pass: nothing to overlap
         if (on) {

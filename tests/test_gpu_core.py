@@ -106,6 +106,42 @@ def test_multichannel_mk5b_and_pcal(hip, oracle):
     core.finalize()
 
 
+def test_row_stream_with_pcal_matches_one_stream(hip, oracle):
+    """Cores with phase-cal whose spectra span several chunks run the row pass
+    on a second stream over two Y buffers, beside the next chunk's column pass
+    and phase-cal fold.  4-channel Mk5B with phase-cal, 2 spectra of 6 FFTs in
+    chunks of 2: spectra and phase-cal bit for bit those of the one-stream
+    run (SWSPEC_ROW_STREAM=0), and every channel against the oracle."""
+    import os
+    n, K, fs = 320000, 6, 32e6
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * K / fs, FFToverlapFactor=1,
+                            WindowType="Cosine2", BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=4,
+                            SourceFormat="Mk5B", ExtractPCal=True, MaxSourceBufferMB=4096)
+    assert s.core_overlapped_ffts == K and s.extract_pcal == 1
+    rng = np.random.default_rng(61)
+    tones = [synth.pcal_comb(fs, fs / 2, amp=0.15, rng=rng) for _ in range(4)]
+    raw = synth.raw_multi(62, 4, n * K * 2, tones_per_ch=tones, fs=fs)
+    res = {}
+    for mode in ("0", "1"):
+        os.environ["SWSPEC_ROW_STREAM"] = mode
+        try:
+            m, auto, _, pc, core = run_device(s, [raw], channels=[0, 1, 2, 3], ffts_per_chunk=2)
+        finally:
+            del os.environ["SWSPEC_ROW_STREAM"]
+        core.finalize()
+        assert m == 2
+        res[mode] = (auto, pc)
+    for ch in range(4):
+        assert np.array_equal(res["0"][0][ch], res["1"][0][ch]), ch
+        assert np.array_equal(res["0"][1][ch], res["1"][1][ch]), ch
+        os_ = oracle_settings(oracle, hip.derive_settings(**dict(s.keys, UseFile1Channel=ch + 1)))
+        m64, a64, _, p64 = oracle.domaths(os_, [raw], 64)
+        assert m64 == 2
+        e, ep = relerr(res["1"][0][ch], a64[0]), relerr(res["1"][1][ch], p64[0])
+        say("  ch%d spectra %.3e pcal %.3e" % (ch, e, ep))
+        assert e <= TOL and ep <= 1e-5
+
+
 @pytest.mark.parametrize("n,win,framed", [(320000, "Blackman", False), (1000000, "None", False),
                                            (1 << 20, "Cosine", False), (3200000, "Hamming", False),
                                            (3200000, "Blackman", True), (320000, "Hann", True)])
