@@ -669,6 +669,43 @@ def main():
                 "kernel_own_flop_per_sample": round(fps[dom], 3),
                 "kernel_own_frac": round(per_kernel[dom]["algorithmic_tflops"] / PEAK_FP32_TFLOPS, 4)}
 
+    # The core runs the row pass on a second stream beside the next chunk's
+    # column pass and phase-cal fold, so the events above time each launch
+    # with that overlap.  The same kernels on one stream (a second core with
+    # SWSPEC_ROW_STREAM=0, a few profiled steps) give their own durations.
+    os.environ["SWSPEC_ROW_STREAM"] = "0"
+    try:
+        c1 = TaskCoreHIP(0)
+        c1.prepare(s, channels=list(range(NCH)), ffts_per_chunk=args.chunk)
+    finally:
+        del os.environ["SWSPEC_ROW_STREAM"]
+
+    def step1():
+        c1.run_device([raw], [SAMPLES], d_auto, None, d_pcal)
+        assert c1.join() == 1
+
+    for _ in range(2):
+        step1()
+    nst1 = max(1, min(args.steps, 5))
+    c1.set_profiling(True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(nst1):
+        step1()
+    torch.cuda.synchronize()
+    dt1 = (time.perf_counter() - t1) / nst1
+    k1 = c1.kernel_stats()
+    c1.set_profiling(False)
+    c1.finalize()
+    ms1, n1 = k1[dom]
+    own_us = ms1 / n1 * 1e3
+    own_ach = fpath * samples_per_launch / (own_us * 1e-6) / 1e12
+    roofline["concurrency"] = ("row pass and phase-cal fold on a second stream beside the column pass: "
+                               "avg_launch_us is each launch's wall time with that overlap")
+    roofline["one_stream"] = {"avg_launch_us": round(own_us, 2), "achieved": round(own_ach, 3),
+                              "frac": round(own_ach / PEAK_FP32_TFLOPS, 4), "ms_per_step": round(dt1 * 1e3, 3),
+                              "kernels_ms_per_step": {k: round(v[0] / nst1, 3) for k, v in k1.items() if v[1]}}
+
     # sanity on the output: spectra finite, tone comb visible in the phase-cal
     a0 = d_auto[0].cpu().numpy()
     assert np.isfinite(a0).all()
