@@ -350,13 +350,14 @@ int swspec_hip_unpack_device_noalloc(const swspec_hip_cfg* cfg, const void* d_ra
                                      float* d_out, void* stream);
 
 /* Plan geometry for an FFT length (no device work) under the plan-shaping
- * flags (SWSPEC_FLAG_MFMA64).  info[0..14] = N1, N2, pass-1 columns per
+ * flags (SWSPEC_FLAG_MFMA64).  info[0..15] = N1, N2, pass-1 columns per
  * workgroup, pass-1 threads, pass-2 threads, row pairs, pass-1 / pass-2
  * points per thread, pass-1 / pass-2 composite-radix flags, pass-1 / pass-2
  * stages, MFMA radix-64 stage on, first pass-1 radix, single pass (short
  * FFTs: the whole M = N/2-point FFT in one kernel; then N1 = M, N2 = 1 and
- * the pass-1 fields describe that FFT).  ninfo may be smaller: the first
- * ninfo entries are written. */
+ * the pass-1 fields describe that FFT), LDS padding of the penultimate row
+ * stage's output blocks (0: none).  ninfo may be smaller: the first ninfo
+ * entries are written. */
 int swspec_hip_plan_info(int64_t fft_points, uint32_t flags, int32_t* info, int ninfo);
 
 /* Autotuner (no counterpart in the reference, which has one FFT plan from

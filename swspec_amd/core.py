@@ -307,10 +307,10 @@ def unpack_device(settings, d_raw, channel, count, d_out, device=0, flags=0):
 
 
 def plan_info(fft_points, flags=0):
-    info = (C.c_int32 * 15)()
-    check(load().swspec_hip_plan_info(fft_points, flags, info, 15), None, "swspec_hip_plan_info")
+    info = (C.c_int32 * 16)()
+    check(load().swspec_hip_plan_info(fft_points, flags, info, 16), None, "swspec_hip_plan_info")
     keys = ["N1", "N2", "B", "p1_threads", "p2_threads", "npairs", "p1_maxpts", "p2_maxpts",
-            "p1_cx", "p2_cx", "p1_stages", "p2_stages", "mfma64", "p1_radix0", "single"]
+            "p1_cx", "p2_cx", "p1_stages", "p2_stages", "mfma64", "p1_radix0", "single", "p2_pad"]
     return dict(zip(keys, list(info)))
 
 

@@ -1187,10 +1187,10 @@ extern "C" int swspec_hip_plan_info(int64_t fft_points, uint32_t flags, int32_t*
     Plan p;
     std::string e = plan_geometry(p, fft_points, (flags & SWSPEC_FLAG_MFMA64) != 0);
     if (!e.empty()) return fail(nullptr, SWSPEC_EUNSUPPORTED, e);
-    int32_t v[15] = {p.N1,        p.N2,        p.B,      p.p1_threads, p.sp2.T * 2, p.npairs,    p.p1_maxpts,
+    int32_t v[16] = {p.N1,        p.N2,        p.B,      p.p1_threads, p.sp2.T * 2, p.npairs,    p.p1_maxpts,
                      p.p2_maxpts, p.p1_cx,     p.p2_cx,  p.sp1.nst,    p.sp2.nst,   p.mfma64,    p.sp1.radix[0],
-                     p.single};
-    for (int i = 0; i < ninfo && i < 15; i++) info[i] = v[i];
+                     p.single,    p.sp2.padst >= 0 ? p.sp2.pad : 0};
+    for (int i = 0; i < ninfo && i < 16; i++) info[i] = v[i];
     return SWSPEC_OK;
 }
 

@@ -106,6 +106,13 @@ struct SubPlan {
     int twoff[kMaxStages];   // offset (in float2) of the stage's twiddles
     int T = 0;               // threads per FFT
     int twlen = 0;           // total twiddles
+    // Row pass only: stage padst writes its output blocks (radix x ns
+    // elements) pad elements apart beyond their length, and the last stage,
+    // whose read stride is that block length, reads them with stride
+    // block + pad.  Breaks up the LDS bank pattern of a stage whose ns < 16
+    // and whose blocks are a multiple of 16 elements.  padst -1: none.
+    int padst = -1;
+    int pad = 0;
 };
 
 // Window generated in the column pass: for real sample j = 2(N2 e + c) + r,

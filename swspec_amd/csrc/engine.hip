@@ -248,13 +248,17 @@ __device__ __forceinline__ int mod_magic(int q, int ns, unsigned mag)
 
 // TW: apply the stage twiddles (every stage after the first; a first stage,
 // ns = 1, has k = 0 and may also run twiddled with its table's ones).
-template <int R, int NB, bool TW>
+// PAD (row pass): the stage's output blocks of R*ns elements are laid out
+// padw elements apart beyond their length, and its inputs are read with the
+// stride nb + padr (the previous stage's padded blocks, nb = its block
+// length); 0 leaves the plain layout.
+template <int R, int NB, bool TW, bool PAD = false>
 __device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int ns, unsigned mag,
                                           const float2* __restrict__ tw, int ebase, int es,
-                                          int q0, int T)
+                                          int q0, int T, int padw = 0, int padr = 0)
 {
     const int nb = L / R;
-    const int rstr = (int)__umul24((unsigned)nb, (unsigned)es);    // uniform
+    const int rstr = (int)__umul24((unsigned)nb, (unsigned)es) + (PAD ? padr : 0);    // uniform
     constexpr int NT = radix_ntw(R);
     // Up to 8 twiddles per thread are loaded before the barrier, so their
     // latency overlaps the LDS reads and the barrier wait (k = q % ns < ns
@@ -284,7 +288,12 @@ __device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int n
         const int q = q0 + i * T;
         if (q < nb) {
             const int k = (TW && ns > 1) ? mod_magic(q, ns, mag) : 0;
-            const int o = (int)__umul24((unsigned)(q - k), (unsigned)R) + k;
+            int o = (int)__umul24((unsigned)(q - k), (unsigned)R) + k;
+            if constexpr (PAD) {
+                // block q / ns of the output starts padw elements later per block
+                const int d = ns > 1 ? (int)__umulhi((unsigned)q, mag) : q;
+                o += (int)__umul24((unsigned)d, (unsigned)padw);
+            }
             float2* __restrict__ dst = lds + ebase + (int)__umul24((unsigned)o, (unsigned)es);
             DftStore<R>::template run<TW>(v[i], PRE ? wr[PRE ? i : 0] : tw + (int)__umul24((unsigned)k, (unsigned)NT),
                                           [&](int r, float2 x) { dst[r * dstr] = x; });
@@ -296,54 +305,56 @@ __device__ __forceinline__ void stage_lds(float2* __restrict__ lds, int L, int n
 // All radices a kernel instantiation supports are compiled into one switch and
 // the register allocation covers the largest, so the composite radices (CX)
 // live in their own instantiations: a plan uses them only where they pay.
-template <int MAXPTS, bool CX, bool TW>
+template <int MAXPTS, bool CX, bool TW, bool PAD = false>
 __device__ __forceinline__ void stage_any(float2* lds, const SubPlan& sp, const float2* __restrict__ stw, int s,
                                           int ebase, int es, int q0, int T)
 {
     const float2* tw = stw + sp.twoff[s];
     const int ns = sp.ns[s];
     const unsigned mag = sp.mag[s];
+    const int padw = PAD && s == sp.padst ? sp.pad : 0;        // uniform
+    const int padr = PAD && s == sp.padst + 1 ? sp.pad : 0;
     // Opaque per-stage copies: otherwise every case's per-thread index math
     // is hoisted out of the stage loop and held live across all stages.
     int q0s = q0, ebs = ebase;
     asm volatile("" : "+v"(q0s), "+v"(ebs));
     switch (sp.radix[s]) {
-    case 2: stage_lds<2, MAXPTS / 2, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T); break;
-    case 3: stage_lds<3, MAXPTS / 3, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T); break;
-    case 4: stage_lds<4, MAXPTS / 4, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T); break;
-    case 5: stage_lds<5, MAXPTS / 5, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T); break;
-    case 8: stage_lds<8, MAXPTS / 8, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T); break;
+    case 2: stage_lds<2, MAXPTS / 2, TW, PAD>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T, padw, padr); break;
+    case 3: stage_lds<3, MAXPTS / 3, TW, PAD>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T, padw, padr); break;
+    case 4: stage_lds<4, MAXPTS / 4, TW, PAD>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T, padw, padr); break;
+    case 5: stage_lds<5, MAXPTS / 5, TW, PAD>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T, padw, padr); break;
+    case 8: stage_lds<8, MAXPTS / 8, TW, PAD>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T, padw, padr); break;
     case 10:
-        if constexpr (CX) stage_lds<10, MAXPTS / 10, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T);
+        if constexpr (CX) stage_lds<10, MAXPTS / 10, TW, PAD>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T, padw, padr);
         break;
     case 16:
-        if constexpr (CX) stage_lds<16, MAXPTS / 16, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T);
+        if constexpr (CX) stage_lds<16, MAXPTS / 16, TW, PAD>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T, padw, padr);
         break;
     case 20:
-        if constexpr (CX && MAXPTS >= 20) stage_lds<20, MAXPTS / 20, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T);
+        if constexpr (CX && MAXPTS >= 20) stage_lds<20, MAXPTS / 20, TW, PAD>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T, padw, padr);
         break;
     case 25:
-        if constexpr (CX && MAXPTS >= 25) stage_lds<25, MAXPTS / 25, TW>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T);
+        if constexpr (CX && MAXPTS >= 25) stage_lds<25, MAXPTS / 25, TW, PAD>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T, padw, padr);
         break;
     case 32:
         // radix 32 always takes the twiddled form (its table holds ones for a
         // first stage): without the multiplies the compiler schedules its
         // 32-point butterfly into ~20 more registers, costing occupancy
-        if constexpr (CX && MAXPTS >= 32) stage_lds<32, MAXPTS / 32, true>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T);
+        if constexpr (CX && MAXPTS >= 32) stage_lds<32, MAXPTS / 32, true, PAD>(lds, sp.L, ns, mag, tw, ebs, es, q0s, T, padw, padr);
         break;
     default: break;
     }
 }
 
-template <int MAXPTS, bool CX>
+template <int MAXPTS, bool CX, bool PAD = false>
 __device__ __forceinline__ void fft_lds(float2* lds, const SubPlan& sp, const float2* __restrict__ stw,
                                         int ebase, int es, int q0, int T, int nst, int s0 = 0)
 {
     // stages [s0, nst); one loop (not a peeled first stage): the loop keeps
     // the compiler from interleaving the first stage with the code before it
     for (int s = s0; s < nst; s++) {
-        if (s == 0) stage_any<MAXPTS, CX, false>(lds, sp, stw, s, ebase, es, q0, T);
-        else stage_any<MAXPTS, CX, true>(lds, sp, stw, s, ebase, es, q0, T);
+        if (s == 0) stage_any<MAXPTS, CX, false, PAD>(lds, sp, stw, s, ebase, es, q0, T);
+        else stage_any<MAXPTS, CX, true, PAD>(lds, sp, stw, s, ebase, es, q0, T);
     }
 }
 
@@ -1094,6 +1105,7 @@ struct P2Args {
     int fpw;          // slice pipelines per workgroup
     int pth;          // threads per pipeline (2 * NSRC * T)
     int pipe_lds;     // float2 units of LDS per pipeline
+    int rowst;        // float2 units of LDS per row (N2, or more with a padded stage)
     long long accstride;   // floats between accumulator slices of a job (2*(M+1))
     long long xaccstride;  // floats between cross-power slices (4*(M+1))
     SubPlan sp;
@@ -1111,6 +1123,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
     extern __shared__ float2 lds_all[];
     const int p = blockIdx.x, grp = blockIdx.y;
     const int N1 = a.N1, N2 = a.N2;
+    const int RS = a.rowst;   // LDS row stride (>= N2)
     const int rowA = p;
     const int rowB = (p == 0) ? 0 : N1 - p;
     const bool self = (rowA == rowB);
@@ -1198,7 +1211,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
         for (int sl = 0; sl < NSL; sl++)
 #pragma unroll
             for (int u = 0; u < U; u++)
-                if (tid + u * nth < N2) lds[sl * N2 + tid + u * nth] = t[sl * U + u];
+                if (tid + u * nth < N2) lds[sl * RS + tid + u * nth] = t[sl * U + u];
     };
     // A plain first radix F1 (16-point plans: 3.2M rows 5*16*16) runs from
     // registers: the prefetch loads each thread's first-stage butterfly inputs
@@ -1232,7 +1245,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
         with_r1([&](auto rc) __attribute__((always_inline)) {
             constexpr int R = decltype(rc)::value;
             const int nb = N2 / R;
-            float2* __restrict__ row = lds + slot * N2;
+            float2* __restrict__ row = lds + slot * RS;
             int qb = q0;
             asm volatile("" : "+v"(qb));   // as in load_bfly
 #pragma unroll
@@ -1275,11 +1288,11 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
                 for (int sl = 0; sl < NSL; sl++)
 #pragma unroll
                     for (int u = 0; u < U; u++)
-                        if (base + u * nth < N2) lds[sl * N2 + base + u * nth] = tb[sl][u];
+                        if (base + u * nth < N2) lds[sl * RS + base + u * nth] = tb[sl][u];
             }
         }
         __syncthreads();
-        fft_lds<MAXPTS, CX>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * N2, 1, q0, T, a.sp.nst, fuse1 ? 1 : 0);
+        fft_lds<MAXPTS, CX, true>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * RS, 1, q0, T, a.sp.nst, fuse1 ? 1 : 0);
         if (PREF) {
             if (sact && f + 1 < nfj && it + 1 < per) {
                 if (fuse1) load_bfly(f + 1);
@@ -1305,8 +1318,8 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
             float2 X1[NSRC], X2[NSRC];
 #pragma unroll
             for (int s = 0; s < NSRC; s++) {
-                const float2 Z = lds[(2 * s) * N2 + pi];
-                const float2 Zm = lds[(2 * s + ms) * N2 + mc];
+                const float2 Z = lds[(2 * s) * RS + pi];
+                const float2 Zm = lds[(2 * s + ms) * RS + mc];
                 const float2 E = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
                 const float2 O = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
                 const float2 wO = cmul(w, O);
@@ -2391,9 +2404,14 @@ void vlba_modulation_bits(uint32_t* words)
 
 static size_t p1_lds(const Plan& p) { return (size_t)p.N1 * p.B * sizeof(float2); }
 // FFT rows only (the chunk sums live in registers)
+static int p2_rowst(const Plan& p)
+{
+    const SubPlan& sp = p.sp2;
+    return p.N2 + (sp.padst >= 0 ? sp.pad * sp.radix[sp.nst - 1] : 0);
+}
 static size_t p2_lds(const Plan& p, int nsrc)
 {
-    return (size_t)2 * nsrc * p.N2 * sizeof(float2);
+    return (size_t)2 * nsrc * p2_rowst(p) * sizeof(float2);
 }
 
 hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, hipStream_t s)
@@ -2569,6 +2587,7 @@ hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const Xpol
     if ((long long)2 * p.N2 > (long long)(p.p2_maxpts / nsrc) * a.pth) return hipErrorInvalidValue;
     const size_t lds1 = p2_lds(p, nsrc);
     a.pipe_lds = (int)(lds1 / sizeof(float2));
+    a.rowst = p2_rowst(p);
     // pack short rows: up to 256 threads (SWSPEC_P2_WG_THREADS) and 64 KB of LDS per workgroup
     static const int wg_threads = [] {
         const char* e = getenv("SWSPEC_P2_WG_THREADS");
@@ -3011,6 +3030,8 @@ static void subplan_from(int L, const int* best, int bn, SubPlan& sp)
     int ns = 1, twl = 0;
     sp.L = L;
     sp.nst = bn;
+    sp.padst = -1;
+    sp.pad = 0;
     for (int s = 0; s < bn; s++) {
         const int R = best[s];
         sp.radix[s] = R;
@@ -3347,6 +3368,26 @@ std::string plan_geometry(Plan& p, long long N, bool mfma64, const PlanKnobs* kn
         T2 = ((2 * T2 + 63) / 64) * 32;
     p.sp2.T = T2;
     p.p2_threads = 2 * T2;
+    // Padded penultimate row stage.  Its butterfly q = ns*j + i writes element
+    // R*ns*j + i + r*ns: with ns < 16 and R*ns a multiple of 16, the 64 lanes
+    // of a write land on only ns distinct bank pairs of the 32 four-byte banks
+    // (3.2M, rows 5*16*16: 5 of 16, 13 lanes each).  Blocks R*ns + ns apart
+    // put lane q at q mod 16, conflict-free; the last stage reads them with
+    // stride R*ns + ns (it reads blocks whole, q < R*ns) and writes the plain
+    // layout in place.  Not for a register first stage (it writes without
+    // the stage loop).  SWSPEC_P2_PAD=0 disables it.
+    {
+        static const bool pad_env = [] {
+            const char* e = getenv("SWSPEC_P2_PAD");
+            return !(e && atoi(e) == 0);
+        }();
+        SubPlan& sp = p.sp2;
+        const int pen = sp.nst - 2;
+        if (pad_env && pen >= 1 && sp.ns[pen] < 16 && (sp.radix[pen] * sp.ns[pen]) % 16 == 0) {
+            sp.padst = pen;
+            sp.pad = sp.ns[pen];
+        }
+    }
     p.npairs = p.N1 / 2 + 1;
     p.twshift = 10;
     // short columns: several FFTs per pass-1 workgroup (<= 256 threads, <= 64 KB LDS)
