@@ -409,10 +409,11 @@ def serve_cfg5(torch, seconds, rate=200.0, seed=5):
     """One Poisson trace of `seconds` of arrivals at `rate` jobs/s, replayed in
     real time twice: continuous batching (jobs admitted in arrival order while
     at most SERVE_CAP_CHANNELS channels are active, all advancing in the same
-    steps) and one job (<= 4 channels) at a time, in arrival order.  With no
-    cap every arrival shares the device at once and the largest jobs' latency
-    grows with the load (550 jobs/s, 10 s: p99 97 ms uncapped, 56 ms at 16
-    channels, 70 ms one job at a time)."""
+    steps) and one job (<= 4 channels) at a time, in arrival order.  Latency
+    is arrival -> completion, wait is arrival -> admission; both are also
+    given per third of the jobs by channel-samples (small / medium / large),
+    where the two schedules differ: continuous batching removes the wait of
+    short jobs behind long ones, and long jobs share their steps."""
     from swspec_amd import workloads as W
     from swspec_amd.server import Server
     trace = W.serving_trace(seed=seed, rate=rate, duration=seconds)
@@ -424,22 +425,42 @@ def serve_cfg5(torch, seconds, rate=200.0, seed=5):
                  17 + jb["fft_points"])
     torch.cuda.synchronize()
 
-    def run(cap, max_wall):
+    # size classes: thirds of the trace's jobs by channel-samples (the same
+    # cut points for both schedules)
+    work = np.array([j["nffts"] * j["fft_points"] * j["nch"] for j in trace], dtype=float)
+    cuts = np.percentile(work, [100.0 / 3, 200.0 / 3])
+
+    def pct(x, q):
+        return round(float(np.percentile(x, q)), 4) if len(x) else None
+
+    def run(cap, max_wall, max_jobs=None):
         srv = Server(max_active=max(cap, 4))
-        r = W.run_serving(torch, srv, trace, pool, max_active_channels=cap, max_wall=max_wall)
+        r = W.run_serving(torch, srv, trace, pool, max_active_channels=cap, max_wall=max_wall,
+                          max_active_jobs=max_jobs)
         st = srv.stats()
         srv.close()
         lat = np.array(r["latencies"]) if r["latencies"] else np.array([np.nan])
+        done = [j for j in r["jobs"] if "latency" in j]
+        classes = {}
+        for name, lo, hi in (("small", -1.0, cuts[0]), ("medium", cuts[0], cuts[1]), ("large", cuts[1], np.inf)):
+            js = [j for j in done if lo < j["nffts"] * j["fft_points"] * j["nch"] <= hi]
+            lt = np.array([j["latency"] for j in js])
+            wt = np.array([j["wait"] for j in js])
+            classes[name] = {"jobs": len(js), "p50_s": pct(lt, 50), "p99_s": pct(lt, 99),
+                             "wait_p50_s": pct(wt, 50), "wait_p99_s": pct(wt, 99)}
+        wt = np.array([j["wait"] for j in done])
         return {"jobs": r["submitted"], "completed": r["completed"], "truncated": r["truncated"],
                 "gsps": round(r["samples"] / r["wall"] / 1e9, 3),
                 "p50_s": round(float(np.percentile(lat, 50)), 4), "p90_s": round(float(np.percentile(lat, 90)), 4),
                 "p99_s": round(float(np.percentile(lat, 99)), 4), "max_s": round(float(np.max(lat)), 4),
+                "wait_p50_s": pct(wt, 50), "wait_p99_s": pct(wt, 99), "by_size": classes,
                 "steps": st["steps"], "wall_s": round(r["wall"], 2)}
 
     served = run(SERVE_CAP_CHANNELS, 2 * seconds + 30)
-    plain = run(4, 2 * seconds + 30)
+    plain = run(4, 2 * seconds + 30, max_jobs=1)
     offered = sum(j["nffts"] * j["fft_points"] * j["nch"] for j in trace) / seconds / 1e9
     return {"rate_jobs_per_s": rate, "arrival_window_s": seconds, "offered_gsps": round(offered, 2),
+            "size_class_cuts_samples": [float(c) for c in cuts],
             "continuous_batching": dict(served, max_active_channels=SERVE_CAP_CHANNELS),
             "plain_one_job_at_a_time": plain}
 

@@ -183,7 +183,7 @@ def serving_trace(seed=5, rate=200.0, duration=60.0):
 
 
 def run_serving(torch, server, trace, pool, window="Cosine2", time_scale=1.0, max_wall=None,
-                max_active_channels=256):
+                max_active_channels=256, max_active_jobs=None):
     """Replay a Poisson trace against the server in real time (scaled).
 
     Continuous batching: jobs that arrive while others run are admitted (all
@@ -191,7 +191,8 @@ def run_serving(torch, server, trace, pool, window="Cosine2", time_scale=1.0, ma
     advance in the same steps as the running ones.  Returns the latencies (s)
     in completion order, total channel-samples, the wall time from the start
     to the last completion, and `jobs`: the trace's jobs, each completed one
-    carrying its "latency"."""
+    carrying its "latency" (arrival -> completion) and "wait" (arrival ->
+    admission)."""
     pending = sorted((dict(j) for j in trace), key=lambda j: j["t"])   # per-run bookkeeping
     queue, live, lat = [], {}, []
     outs = {}
@@ -216,8 +217,10 @@ def run_serving(torch, server, trace, pool, window="Cosine2", time_scale=1.0, ma
             queue.append(pending[i])
             i += 1
         # admit whole jobs (all channels) in arrival order
-        while queue and active_ch + queue[0]["nch"] <= max_active_channels:
+        while (queue and active_ch + queue[0]["nch"] <= max_active_channels
+               and (max_active_jobs is None or len(live) < max_active_jobs)):
             jb = queue.pop(0)
+            jb["wait"] = (time.perf_counter() - t0) / time_scale - jb["t"]   # arrival -> admission
             fmt = "raw1" if jb["nch"] == 1 else "mk5b4"
             tones = [[(0.1, 4.8e6 * jb["fs"] / 32e6, 0.0)]] * 4
             ring = pool.get(fmt, jb["fft_points"], jb["fs"], 0, tones, 17 + jb["fft_points"])
@@ -257,7 +260,10 @@ def run_serving(torch, server, trace, pool, window="Cosine2", time_scale=1.0, ma
             if i < len(pending):
                 dt = pending[i]["t"] - (time.perf_counter() - t0) / time_scale
                 if dt > 0:
-                    time.sleep(min(dt * time_scale, 0.05))
+                    # with jobs still on the device, wake often enough to see
+                    # them complete (a sleep to the next arrival would add up
+                    # to the inter-arrival gap to their latency)
+                    time.sleep(min(dt * time_scale, 2e-4 if live else 0.05))
             else:
                 server.sync()
     server.sync()
