@@ -3,7 +3,8 @@
 Streams are generated on the device from a seed (Gaussian noise + tones,
 quantised at +-0.98 sigma with the reference's 2-bit field convention,
 DataUnpackers.cpp:212) and stored as rings of a few FFTs that jobs read
-cyclically (the engine takes FFT k from slot k mod ring length).
+cyclically (the engine takes FFT k from slot k mod ring length), or, for
+throughput runs, as whole per-stream inputs of random bytes (full_inputs).
 
   batch_workload : 64 streams, FFT sizes drawn from {4096, 20000, 65536,
                    320000, 2^20, 3.2M}, 10-5000 FFTs each, raw 1-channel or
@@ -146,14 +147,40 @@ class RingPool:
         return self.rings[key]
 
 
-def submit_batch(torch, server, streams, pool, fs=32e6, window="Cosine2"):
-    """Admit every stream of a batch workload; returns (outputs, total samples)."""
-    outs, total = [], 0
+def full_inputs(torch, streams, seed=11, dev="cuda"):
+    """Every stream's whole input (nffts FFTs, no ring) in one device buffer,
+    each at a 256-byte aligned offset, filled with uniform random bytes (every
+    2-bit field equally likely; the kernels' work does not depend on the
+    values).  Returns (buffer, [(offset, bytes)] per stream)."""
+    offs, o = [], 0
     for s in streams:
-        tones = [[(0.1, s["tone"], 0.0)]] * 4
-        ring = pool.get(s["fmt"], s["fft_points"], fs, round(s["tone"]), tones, s["seed"])
+        b = s["nffts"] * raw_bytes_per_fft(s["fmt"], s["fft_points"])
+        offs.append((o, b))
+        o += (b + 255) // 256 * 256
+    buf = torch.empty(o, dtype=torch.uint8, device=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    slab = 1 << 32
+    for a in range(0, o, slab):
+        buf[a:a + slab].random_(0, 256, generator=g)
+    torch.cuda.synchronize()
+    return buf, offs
+
+
+def submit_batch(torch, server, streams, pool, fs=32e6, window="Cosine2", full=None):
+    """Admit every stream of a batch workload; returns (outputs, total samples).
+    full: (buffer, offsets) of full_inputs, read instead of the pool's rings."""
+    outs, total = [], 0
+    for i, s in enumerate(streams):
+        if full is not None:
+            ring = full[0].data_ptr() + full[1][i][0]
+            nbytes = full[1][i][1]
+        else:
+            tones = [[(0.1, s["tone"], 0.0)]] * 4
+            ring = pool.get(s["fmt"], s["fft_points"], fs, round(s["tone"]), tones, s["seed"])
+            nbytes = ring.numel()
         out = torch.empty(s["fft_points"] // 2 + 1, dtype=torch.float32, device=pool.dev)
-        jid = server.submit(s["fft_points"], s["nffts"], ring, ring.numel(), out, window=window,
+        jid = server.submit(s["fft_points"], s["nffts"], ring, nbytes, out, window=window,
                             source_format="RawSigned" if s["fmt"] == "raw1" else "Mk5B",
                             source_channels=1 if s["fmt"] == "raw1" else 4,
                             channel=0 if s["fmt"] == "raw1" else s["channel"], samplingfreq=fs)
