@@ -49,6 +49,12 @@ def test_derive_errors_and_plan_info():
         swspec_amd.derive_settings(ExtractPCal=True, PCalOffsetHz=0.0)
     info = swspec_amd.plan_info(3200000)
     assert info["N1"] * info["N2"] == 1600000
+    # rows 5*16*16: the radix-16 stage after the radix-5 one (ns = 5, blocks of
+    # 80 points) writes padded blocks; 2^20 rows 4*16*16 likewise (ns = 4);
+    # 32000 has two row stages and no padding
+    assert info["p2_pad"] == 5
+    assert swspec_amd.plan_info(1 << 20)["p2_pad"] == 4
+    assert swspec_amd.plan_info(32000)["p2_pad"] == 0
     with pytest.raises(RuntimeError):
         swspec_amd.plan_info(2 * 7 * 1000)   # prime factor 7: no plan
     assert info["mfma64"] == 0
