@@ -938,12 +938,91 @@ __device__ __forceinline__ void fill_rows(float2* __restrict__ lp, const uint8_t
     }
 }
 
+// Row fill with the first column stage fused (F1: its radix; a plain Stockham
+// first stage, ns = 1).  A thread takes butterfly q of a pair of adjacent
+// columns cA, cA+1 of the workgroup's BC: the F1 rows q + r*(N1/F1) of those
+// two columns are F1 four-byte loads (2 points = 4 samples of one byte each;
+// framed: a 4-byte group never straddles a 10000-byte payload), decoded and
+// windowed as fill_rows does, and both columns' F1-point DFTs run in
+// registers; output r of butterfly q is element q*F1 + r of its column,
+// written for both columns as one 16-byte row piece.  This saves the fill's
+// LDS write and the first stage's LDS read of every point and one barrier.
+// SC: single-channel 2-bit layout, the two points one byte (oldest sample in
+// the MSB).
+template <int BC, int F1, int WM, bool FR, bool SC = false>
+__device__ __forceinline__ void fill_stage1(float2* __restrict__ lp, const uint8_t* __restrict__ raw,
+                                            const Unpack& up, long long pay, const WindowGen& wg,
+                                            const float4* __restrict__ wcol, int tid, int nth, int N1, int N2,
+                                            int c0)
+{
+    static_assert(BC == 4 || BC == 8, "row fill: 4 or 8 columns");
+    constexpr int NP = BC / 2;   // column pairs
+    const int nb = N1 / F1;
+    const unsigned sh = (unsigned)up.shift;
+    const float a0 = wg.a0;
+    for (int it = tid; it < nb * NP; it += nth) {
+        const int q = it / NP;
+        const int cA = 2 * (it - q * NP);
+        float4 K[2][2];   // window constants of columns cA, cA+1, samples 0, 1
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 2; r++)
+                K[b][r] = WM ? wcol[2 * (c0 + cA + b) + r] : make_float4(0.f, 0.f, 0.f, 0.f);
+        unsigned w[F1];
+        float2 z[F1];
+#pragma unroll
+        for (int r = 0; r < F1; r++) {
+            const int e = q + r * nb;
+            const unsigned pt = __umul24((unsigned)N2, (unsigned)e) + (unsigned)(c0 + cA);
+            if constexpr (SC) w[r] = raw[pt >> 1];
+            else if constexpr (FR) w[r] = *(const unsigned*)(raw + mk5b_frame_addr(pay + 2LL * pt));
+            else w[r] = *(const unsigned*)(raw + 2u * pt);
+            z[r] = WM ? wg.phi[e] : make_float2(1.f, 0.f);
+        }
+        float2 v[2][F1];
+#pragma unroll
+        for (int r = 0; r < F1; r++) {
+            const unsigned t = SC ? w[r] : (w[r] >> sh) & 0x03030303u;
+            float c2 = 0.f, s2 = 0.f;
+            if constexpr (WM == 2) {
+                c2 = z[r].x * z[r].x - z[r].y * z[r].y;
+                s2 = 2.f * z[r].x * z[r].y;
+            }
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                const unsigned f0 = SC ? (t >> (6 - 4 * b)) & 3u : (t >> (16 * b)) & 3u;
+                const unsigned f1 = SC ? (t >> (4 - 4 * b)) & 3u : (t >> (16 * b + 8)) & 3u;
+                float w0 = 1.f, w1 = 1.f;
+                if constexpr (WM != 0) {
+                    w0 = a0 + z[r].x * K[b][0].x + z[r].y * K[b][0].y;
+                    w1 = a0 + z[r].x * K[b][1].x + z[r].y * K[b][1].y;
+                    if constexpr (WM == 2) {
+                        w0 += c2 * K[b][0].z + s2 * K[b][0].w;
+                        w1 += c2 * K[b][1].z + s2 * K[b][1].w;
+                    }
+                }
+                v[b][r] = make_float2(map2bit(f0) * w0, map2bit(f1) * w1);
+            }
+        }
+        float2 o[2][F1];
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+            DftStore<F1>::template run<false>(v[b], nullptr, [&](int r, float2 x) { o[b][r] = x; });
+        float4* dst = (float4*)(lp + (size_t)q * F1 * BC + cA);
+#pragma unroll
+        for (int r = 0; r < F1; r++) dst[r * (BC / 2)] = make_float4(o[0][r].x, o[0][r].y, o[1][r].x, o[1][r].y);
+    }
+}
+
 // MF: the first column stage is radix 64 on the matrix cores (stage_mfma64;
 // one pipeline per workgroup, whole waves)
 // RF: row fill of RF = 4 or 8 columns (fill_rows; launcher: a.rowfill), 0:
 // column fill.  Separate instantiations, so that each is register-allocated
 // for its own fill.
-template <int MAXPTS, bool CX, bool MF = false, int RF = 0>
+// F1 (with RF): the first column stage, of radix F1, fused into the row fill
+// (fill_stage1; multi-channel layouts), so the LDS stages start at stage 1.
+template <int MAXPTS, bool CX, bool MF = false, int RF = 0, int F1 = 0>
 __global__ void __launch_bounds__(512) k_pass1(P1Args a)
 {
     extern __shared__ float2 lds[];
@@ -969,7 +1048,21 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     const int cb = c0 + bcol;
     const int total = N1 * B;
     const int estep = nth / B;
-    if constexpr (RF != 0) {
+    if constexpr (RF != 0 && F1 != 0) {
+        if (active) {
+            auto rows1 = [&](auto fr, auto sc) __attribute__((always_inline)) {
+                constexpr bool FR = decltype(fr)::value;
+                constexpr bool SC = decltype(sc)::value;
+                const int wm = a.win.kind == 0 ? 0 : (a.win.a3 == 0.f ? 1 : 2);
+                if (wm == 0) fill_stage1<RF, F1, 0, FR, SC>(lp, raw, up, pay, a.win, a.wcol, tid, nth, N1, N2, c0);
+                else if (wm == 1) fill_stage1<RF, F1, 1, FR, SC>(lp, raw, up, pay, a.win, a.wcol, tid, nth, N1, N2, c0);
+                else fill_stage1<RF, F1, 2, FR, SC>(lp, raw, up, pay, a.win, a.wcol, tid, nth, N1, N2, c0);
+            };
+            if (up.kind == kUnpackSingleChannel) rows1(std::false_type(), std::true_type());
+            else if (up.framed) rows1(std::true_type(), std::false_type());
+            else rows1(std::false_type(), std::false_type());
+        }
+    } else if constexpr (RF != 0) {
         if (active) {
             auto rows = [&](auto fr, auto sc) __attribute__((always_inline)) {
                 constexpr bool FR = decltype(fr)::value;
@@ -1000,7 +1093,7 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     const bool fuse = sp.nst >= 2 && (rlast == 2 || rlast == 3 || rlast == 4 || rlast == 5 || rlast == 8);
     const int ebase = pipe * (N1 * B) + bcol;
     if constexpr (MF) stage_mfma64(lds, a.mft, N1, B);
-    fft_lds<MAXPTS, CX>(lds, sp, a.stw, ebase, B, et, estep, fuse ? sp.nst - 1 : sp.nst, MF ? 1 : 0);
+    fft_lds<MAXPTS, CX>(lds, sp, a.stw, ebase, B, et, estep, fuse ? sp.nst - 1 : sp.nst, (MF || F1) ? 1 : 0);
 
     float2* __restrict__ Y = J.Y + (long long)(active ? f : 0) * a.M;
     const unsigned mask = (1u << a.twshift) - 1u;
@@ -2472,9 +2565,20 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
         return hipGetLastError();
     }
     const int rf = a.rowfill ? p.B : 0;
+    // First column stage fused into the row fill: a radix-10 first stage of a
+    // plan with a later stage, 4 columns, jobs the row fill takes (its checks
+    // give 16-byte aligned rows; fill_stage1 reads 4-byte groups of them, or
+    // single bytes of the single-channel layout).  SWSPEC_P1_FUSE1=0 keeps
+    // the separate first stage.
+    static const bool f1_env = [] {
+        const char* e = getenv("SWSPEC_P1_FUSE1");
+        return !(e && atoi(e) == 0);
+    }();
+    const bool f1 = f1_env && rf == 4 && p.sp1.radix[0] == 10 && p.sp1.nst >= 2;
 #define SWSPEC_P1(MP, CXV)                                                                              \
     {                                                                                                 \
-        if (rf == 4) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4>), grid, block, lds, s, a);         \
+        if (CXV && f1) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4, CXV ? 10 : 0>), grid, block, lds, s, a); \
+        else if (rf == 4) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4>), grid, block, lds, s, a);    \
         else if (rf == 8) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 8>), grid, block, lds, s, a);    \
         else hipLaunchKernelGGL((k_pass1<MP, CXV, false, 0>), grid, block, lds, s, a);                 \
     }

@@ -45,6 +45,8 @@ def test_unpack_bitexact(hip, oracle, fmt, nch, incr):
 SIZES = [
     (4096, "Hann", 5), (20000, "Cosine2", 3), (32000, "Hann", 4), (65536, "Hamming", 3),
     (320000, "Blackman", 2), (1 << 20, "Cosine", 2), (1000000, "None", 2), (3200000, "Cosine2", 2),
+    # 3.2M with the first column stage fused into the fill: the other window forms
+    (3200000, "Blackman", 2), (3200000, "None", 2),
     # odd row lengths: 2700 = 2 * 30 x 45 (even column count: a self-mirrored
     # middle row with a self-paired bin), 4374 = 2 * 27 x 81, 156250 = 2 * 125 x 625
     (2700, "Hann", 5), (4374, "Cosine2", 4), (156250, "Hamming", 2),
@@ -144,14 +146,16 @@ def test_row_stream_with_pcal_matches_one_stream(hip, oracle):
 
 @pytest.mark.parametrize("n,win,framed", [(320000, "Blackman", False), (1000000, "None", False),
                                            (1 << 20, "Cosine", False), (3200000, "Hamming", False),
-                                           (3200000, "Blackman", True), (320000, "Hann", True)])
+                                           (3200000, "Blackman", True), (320000, "Hann", True),
+                                           (3200000, "None", False), (3200000, "Cosine", True)])
 def test_row_fill_windows_vs_oracle(hip, oracle, n, win, framed):
     """The row fill of pass 1 (a thread decodes and windows whole rows of the
     workgroup's 4 or 8 columns from one wide load; one byte per time sample)
     for every window form: none, one cosine term (Cosine, Hann, Hamming) and
-    the cos(2 theta) term (Blackman), 4- and 8-column plans (3.2M: 4; 320000,
-    10^6, 2^20: 8), payload and framed Mk5B.  All four channels of a
-    4-channel Mk5B stream against the FP64 oracle."""
+    the cos(2 theta) term (Blackman), 4- and 8-column plans (3.2M: 4, with
+    the radix-10 first column stage fused into the fill; 320000, 10^6, 2^20:
+    8), payload and framed Mk5B.  All four channels of a 4-channel Mk5B
+    stream against the FP64 oracle."""
     from swspec_amd import plan_info
     from swspec_amd._lib import FLAG_MK5B_FRAMED
     nf, fs = 2, 32e6
