@@ -147,14 +147,15 @@ def test_row_stream_with_pcal_matches_one_stream(hip, oracle):
 @pytest.mark.parametrize("n,win,framed", [(320000, "Blackman", False), (1000000, "None", False),
                                            (1 << 20, "Cosine", False), (3200000, "Hamming", False),
                                            (3200000, "Blackman", True), (320000, "Hann", True),
-                                           (3200000, "None", False), (3200000, "Cosine", True)])
+                                           (3200000, "None", False), (3200000, "Cosine", True),
+                                           (1 << 20, "Hann", True)])
 def test_row_fill_windows_vs_oracle(hip, oracle, n, win, framed):
     """The row fill of pass 1 (a thread decodes and windows whole rows of the
     workgroup's 4 or 8 columns from one wide load; one byte per time sample)
     for every window form: none, one cosine term (Cosine, Hann, Hamming) and
     the cos(2 theta) term (Blackman), 4- and 8-column plans (3.2M: 4, with
     the radix-10 first column stage fused into the fill; 320000, 10^6, 2^20:
-    8), payload and framed Mk5B.  All four channels of a 4-channel Mk5B
+    8, 2^20 with its radix-16 first stage fused), payload and framed Mk5B.  All four channels of a 4-channel Mk5B
     stream against the FP64 oracle."""
     from swspec_amd import plan_info
     from swspec_amd._lib import FLAG_MK5B_FRAMED
@@ -164,7 +165,10 @@ def test_row_fill_windows_vs_oracle(hip, oracle, n, win, framed):
     s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=1, WindowType=win,
                             BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=4, SourceFormat="Mk5B",
                             MaxSourceBufferMB=4096)
-    payload = synth.raw_multi(11, 4, n * nf, tones_per_ch=[[(0.2, (3 + c) * 1e6, 0.1 * c)] for c in range(4)],
+    # framed input is whole 10000-byte payloads: the bytes past the last FFT
+    # (less than one FFT) are read by neither the core nor the oracle
+    nbytes = -(-n * nf // 10000) * 10000 if framed else n * nf
+    payload = synth.raw_multi(11, 4, nbytes, tones_per_ch=[[(0.2, (3 + c) * 1e6, 0.1 * c)] for c in range(4)],
                               fs=fs)
     data = synth.mk5b_frames(payload) if framed else payload
     m, auto, _, _, core = run_device(s, [data], channels=[0, 1, 2, 3], flags=FLAG_MK5B_FRAMED if framed else 0)
