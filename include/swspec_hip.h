@@ -254,8 +254,12 @@ double swspec_hip_last_run_seconds(const swspec_hip_core* core);
  * Per-kernel timing with HIP events recorded on the core's stream around each
  * launch (no TaskCore counterpart; the reference times whole doMaths calls,
  * TaskCoreIPP.cpp:390-391,516-517).  kernel: SWSPEC_K_* below.  Totals
- * accumulate over runs while profiling is on; set_profiling(core, 1) clears.
+ * accumulate over runs while profiling is on; set_profiling clears them.
+ * on: 0 off, 1 every kernel, or SWSPEC_PROF_MASK | (1 << SWSPEC_K_...) | ...
+ * for only the kernels named (each event pair costs the stream a little: the
+ * 3.2M 4-channel step takes about 2% longer with every kernel timed).
  */
+#define SWSPEC_PROF_MASK 0x100
 #define SWSPEC_K_PASS1 0
 #define SWSPEC_K_PASS2 1
 #define SWSPEC_K_PCAL 2

@@ -273,8 +273,16 @@ class TaskCoreHIP:
               "swspec_hip_pcal_tones")
         return out[0: 2 * nt: 2] + 1j * out[1: 2 * nt: 2]
 
-    def set_profiling(self, on=True):
-        check(load().swspec_hip_set_profiling(self._core, 1 if on else 0), self._core, "swspec_hip_set_profiling")
+    def set_profiling(self, on=True, kernels=None):
+        """HIP events around each launch; kernels: names of kernel_stats to
+        time (default every kernel)."""
+        v = 1 if on else 0
+        if on and kernels is not None:
+            names = ["pass1", "pass2", "pcal", "finalize"]
+            v = 0x100
+            for k in kernels:
+                v |= 1 << names.index(k)
+        check(load().swspec_hip_set_profiling(self._core, v), self._core, "swspec_hip_set_profiling")
 
     def kernel_stats(self):
         """{name: (total_ms, launches)} from HIP events around each launch."""

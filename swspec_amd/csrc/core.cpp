@@ -99,6 +99,7 @@ struct swspec_hip_core {
     std::string err;
     // per-kernel profiling (HIP events around launches on `stream`)
     bool profiling = false;
+    unsigned prof_mask = 0xF;        // bit k: time SWSPEC_K_ k
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<int> ev_kind;        // kernel id of event pair i
@@ -114,7 +115,7 @@ struct ProfScope {
     ProfScope(swspec_hip_core* core, int kind, hipStream_t stream = nullptr)
         : c(core), s(stream ? stream : core->stream)
     {
-        if (!c->profiling) return;
+        if (!c->profiling || !(c->prof_mask & (1u << kind))) return;
         const size_t need = 2 * (c->ev_used + 1);
         while (c->ev_pool.size() < need) {
             hipEvent_t e;
@@ -1103,7 +1104,9 @@ extern "C" int swspec_hip_set_profiling(swspec_hip_core* c, int on)
 {
     if (!c) return fail(nullptr, SWSPEC_EINVAL, "null core");
     if (c->pending) return fail(c, SWSPEC_ESTATE, "set_profiling while a run is pending");
+    if (on != 0 && on != 1 && !(on & SWSPEC_PROF_MASK)) return fail(c, SWSPEC_EINVAL, "set_profiling: on");
     c->profiling = on != 0;
+    c->prof_mask = (on & SWSPEC_PROF_MASK) ? (unsigned)on & 0xFu : 0xFu;
     c->ev_used = 0;
     for (int k = 0; k < 4; k++) { c->k_ms[k] = 0.0; c->k_n[k] = 0; }
     return SWSPEC_OK;
