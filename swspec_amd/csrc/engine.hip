@@ -140,6 +140,34 @@ __host__ __device__ constexpr int radix_ntw(int R)
     return radix_factored_tw(R) ? radix_a(R) + R / radix_a(R) - 2 : R - 1;
 }
 
+// A three-stage sub-plan as subplan_from builds it (the stage twiddle table
+// layout included), at compile time: the row pass of the 3.2M plan has its
+// own kernel instantiation in which every stride and bound is a constant.
+__host__ __device__ constexpr SubPlan fixed_subplan(int L, int r0, int r1, int r2, int T, int padst, int pad)
+{
+    SubPlan sp{};
+    const int rr[3] = {r0, r1, r2};
+    int ns = 1, twl = 0;
+    sp.L = L;
+    sp.nst = 3;
+    for (int st = 0; st < 3; st++) {
+        sp.radix[st] = rr[st];
+        sp.ns[st] = ns;
+        sp.mag[st] = ns > 1 ? (unsigned)(((1ULL << 32) + (unsigned long long)ns - 1) / (unsigned long long)ns) : 0u;
+        sp.twoff[st] = twl;
+        twl += ns * radix_ntw(rr[st]);
+        ns *= rr[st];
+    }
+    sp.twlen = twl;
+    sp.T = T;
+    sp.padst = padst;
+    sp.pad = pad;
+    return sp;
+}
+// 3.2M rows: 1280 = 5*16*16, 96 threads per row, penultimate stage padded by 5
+constexpr int kFxN2 = 1280, kFxT = 96;
+__host__ __device__ constexpr SubPlan fixed_rows_3p2m() { return fixed_subplan(kFxN2, 5, 16, 16, kFxT, 1, 5); }
+
 // Roots of unity at compile time: Taylor series on the angle reduced to (-pi, pi].
 constexpr double kPiD = 3.14159265358979323846264338327950288;
 constexpr double cx_sin(double x)
@@ -1275,26 +1303,39 @@ struct P2Args {
 };
 
 // F1: plain first row radix run from registers (0: none; see fuse1 below)
-template <int MAXPTS, int NSRC, bool CX, int F1 = 0>
+template <bool A, class T>
+__device__ __forceinline__ const T& pick_ref(const T& a, const T& b)
+{
+    if constexpr (A) return a;
+    else return b;
+}
+
+// FX: the 3.2M row plan (fixed_rows_3p2m) with its strides and bounds as
+// constants, one pipeline per workgroup.
+template <int MAXPTS, int NSRC, bool CX, int F1 = 0, bool FX = false>
 __global__ void __launch_bounds__(512) k_pass2(P2Args a)
 {
     extern __shared__ float2 lds_all[];
+    constexpr SubPlan kFx = fixed_rows_3p2m();
+    const SubPlan& SP = pick_ref<FX>(kFx, a.sp);
     const int p = blockIdx.x, grp = blockIdx.y;
-    const int N1 = a.N1, N2 = a.N2;
-    const int RS = a.rowst;   // LDS row stride (>= N2)
+    const int N1 = a.N1, N2 = FX ? kFxN2 : a.N2;
+    const int RS = FX ? kFxN2 + 5 * 16 : a.rowst;   // LDS row stride (>= N2)
     const int rowA = p;
     const int rowB = (p == 0) ? 0 : N1 - p;
     const bool self = (rowA == rowB);
     // pipeline (slice) of this thread: blockDim == fpw * pth
-    const int pth = a.pth;
+    const int pth = FX ? 2 * NSRC * kFxT : a.pth;
+    const int fpw = FX ? 1 : a.fpw;
     int pipe = 0;
-    while (pipe + 1 < a.fpw && threadIdx.x >= (unsigned)((pipe + 1) * pth)) pipe++;
+    while (pipe + 1 < fpw && threadIdx.x >= (unsigned)((pipe + 1) * pth)) pipe++;
     const int tid = threadIdx.x - pipe * pth, nth = pth;
-    const int sl_i = blockIdx.z * a.fpw + pipe;          // slice index
+    const int sl_i = blockIdx.z * fpw + pipe;          // slice index
     const bool sact = sl_i < a.S;
-    float2* __restrict__ lds = lds_all + pipe * a.pipe_lds;
+    const int pipe_lds = FX ? 2 * NSRC * RS : a.pipe_lds;
+    float2* __restrict__ lds = lds_all + pipe * pipe_lds;
     constexpr int NSL = 2 * NSRC;
-    const int T = a.sp.T;                   // pth == NSL * T
+    const int T = FX ? kFxT : a.sp.T;       // pth == NSL * T
     int slot = 0;
 #pragma unroll
     for (int s = 1; s < NSL; s++) slot += (tid >= s * T) ? 1 : 0;
@@ -1450,7 +1491,9 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
             }
         }
         __syncthreads();
-        fft_lds<MAXPTS, CX, true>(lds_all, a.sp, a.stw, pipe * a.pipe_lds + slot * RS, 1, q0, T, a.sp.nst, fuse1 ? 1 : 0);
+        // padded rows (plan_geometry pads only plans that take a register first stage)
+        fft_lds<MAXPTS, CX, (F1 != 0)>(lds_all, SP, a.stw, pipe * pipe_lds + slot * RS, 1, q0, T, SP.nst,
+                                       fuse1 ? 1 : 0);
         if (PREF) {
             if (sact && f + 1 < nfj && it + 1 < per) {
                 if (fuse1) load_bfly(f + 1);
@@ -2795,8 +2838,25 @@ hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const Xpol
         return !(e && atoi(e) == 0);
     }();
     const int r1 = p.sp2.radix[0];
-    if (f1_env && p.p2_maxpts == 16 && (r1 == 4 || r1 == 5 || r1 == 8))
+    // a padded plan needs the register-first-stage kernel (the only one built
+    // with the padded layout)
+    if ((f1_env || p.sp2.padst >= 0) && p.p2_maxpts == 16 && (r1 == 4 || r1 == 5 || r1 == 8))
         k = table_f1[r1 == 4 ? 0 : (r1 == 5 ? 1 : 2)][nsrc - 1][p.p2_cx ? 1 : 0];
+    // the 3.2M row plan with constant strides (SWSPEC_P2_FIXED=0: the generic kernel)
+    static const bool fx_env = [] {
+        const char* e = getenv("SWSPEC_P2_FIXED");
+        return !(e && atoi(e) == 0);
+    }();
+    if (fx_env && k == table_f1[1][nsrc - 1][1] && a.fpw == 1) {
+        constexpr SubPlan f = fixed_rows_3p2m();
+        const SubPlan& q = p.sp2;
+        bool same = p.N2 == kFxN2 && q.L == f.L && q.nst == f.nst && q.T == f.T && q.padst == f.padst &&
+                    q.pad == f.pad && a.rowst == p2_rowst(p) && p2_rowst(p) == kFxN2 + f.pad * f.radix[2];
+        for (int st = 0; same && st < f.nst; st++)
+            same = q.radix[st] == f.radix[st] && q.ns[st] == f.ns[st] && q.mag[st] == f.mag[st] &&
+                   q.twoff[st] == f.twoff[st];
+        if (same) k = nsrc == 1 ? k_pass2<16, 1, true, 5, true> : k_pass2<16, 2, true, 5, true>;
+    }
     hipLaunchKernelGGL(k, grid, block, lds, s, a);
     return hipGetLastError();
 }
@@ -3560,7 +3620,10 @@ std::string plan_geometry(Plan& p, long long N, bool mfma64, const PlanKnobs* kn
         }();
         SubPlan& sp = p.sp2;
         const int pen = sp.nst - 2;
-        if (pad_env && pen >= 1 && sp.ns[pen] < 16 && (sp.radix[pen] * sp.ns[pen]) % 16 == 0) {
+        // only for the row kernels with a register first stage (16 points per
+        // thread, first radix 4, 5 or 8): the others are built without it
+        const bool f1k = p.p2_maxpts == 16 && (sp.radix[0] == 4 || sp.radix[0] == 5 || sp.radix[0] == 8);
+        if (pad_env && f1k && pen >= 1 && sp.ns[pen] < 16 && (sp.radix[pen] * sp.ns[pen]) % 16 == 0) {
             sp.padst = pen;
             sp.pad = sp.ns[pen];
         }
