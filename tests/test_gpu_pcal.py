@@ -134,7 +134,9 @@ def test_delay_vs_oracle(tau_samples):
 
 
 def test_fold_throughput():
-    """HBM-bound streaming fold: 4 bytes read per sample (reported, not asserted)."""
+    """HBM-bound streaming fold: 4 bytes read per sample.  Asserted against a
+    floor of a quarter of the 8 TB/s HBM peak (it measures 5.5-6 TB/s), and the
+    folded sums stay finite over the six calls."""
     import torch
     from swspec_amd.pcal import PCal
     x = torch.randn(1 << 28, device="cuda")    # 1 GiB of float32 samples
@@ -149,4 +151,8 @@ def test_fold_throughput():
     e1.record()
     torch.cuda.synchronize()
     s = e0.elapsed_time(e1) / 1e3 / reps
-    say("  ImplicitShift fold: %.1f Gsamples/s, %.0f GB/s" % (x.numel() / s / 1e9, x.numel() * 4 / s / 1e9))
+    gbps = x.numel() * 4 / s / 1e9
+    say("  ImplicitShift fold: %.1f Gsamples/s, %.0f GB/s" % (x.numel() / s / 1e9, gbps))
+    _, tones = g.getFinalPCal64()
+    assert np.all(np.isfinite(tones))
+    assert gbps >= 2000.0
