@@ -167,6 +167,16 @@ __host__ __device__ constexpr SubPlan fixed_subplan(int L, int r0, int r1, int r
 // 3.2M rows: 1280 = 5*16*16, 96 threads per row, penultimate stage padded by 5
 constexpr int kFxN2 = 1280, kFxT = 96;
 __host__ __device__ constexpr SubPlan fixed_rows_3p2m() { return fixed_subplan(kFxN2, 5, 16, 16, kFxT, 1, 5); }
+// 3.2M columns: 1250 = 10*25*5, 4 columns of 64 threads
+constexpr int kFxN1 = 1250, kFxP1Threads = 256;
+__host__ __device__ constexpr SubPlan fixed_cols_3p2m() { return fixed_subplan(kFxN1, 10, 25, 5, kFxP1Threads / 4, -1, 0); }
+
+template <bool A, class T>
+__device__ __forceinline__ const T& pick_ref(const T& a, const T& b)
+{
+    if constexpr (A) return a;
+    else return b;
+}
 
 // Roots of unity at compile time: Taylor series on the angle reduced to (-pi, pi].
 constexpr double kPiD = 3.14159265358979323846264338327950288;
@@ -1115,7 +1125,9 @@ __device__ __forceinline__ void fill_stage1_col(float2* __restrict__ lp, const u
 // for its own fill.
 // F1 (with RF): the first column stage, of radix F1, fused into the row fill
 // (fill_stage1; multi-channel layouts), so the LDS stages start at stage 1.
-template <int MAXPTS, bool CX, bool MF = false, int RF = 0, int F1 = 0>
+// FX: the 3.2M column plan (fixed_cols_3p2m: 1250 = 10*25*5, 4 columns, 256
+// threads, one pipeline) with its strides and bounds as constants.
+template <int MAXPTS, bool CX, bool MF = false, int RF = 0, int F1 = 0, bool FX = false>
 __global__ void __launch_bounds__(512) k_pass1(P1Args a)
 {
     extern __shared__ float2 lds[];
@@ -1123,13 +1135,13 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     xcd_remap(bx, by, bz, a.zfast);
     const JobDev J = a.jobs[bz];
     // a workgroup runs `fpw` independent pipelines, one FFT each (short columns)
-    const int fpw = a.fpw, pth = a.pth;
+    const int fpw = FX ? 1 : a.fpw, pth = FX ? kFxP1Threads : a.pth;
     if ((int)by * fpw >= J.nf) return;   // no FFT of this job here (whole workgroup)
     int pipe = 0;
     while (pipe + 1 < fpw && threadIdx.x >= (unsigned)((pipe + 1) * pth)) pipe++;
     const int f = (int)by * fpw + pipe;
     const bool active = f < J.nf;
-    const int B = a.B, N1 = a.N1, N2 = a.N2;
+    const int B = FX ? 4 : a.B, N1 = FX ? kFxN1 : a.N1, N2 = FX ? kFxN2 : a.N2;
     const int c0 = (int)bx * B;
     const int tid = threadIdx.x - pipe * pth, nth = pth;
     float2* __restrict__ lp = lds + pipe * (N1 * B);
@@ -1181,7 +1193,8 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     // fused with the store below: its butterflies read LDS and write their
     // outputs straight to Y, saving an LDS write, a barrier and a read per
     // point.
-    const SubPlan& sp = a.sp;
+    constexpr SubPlan kFx1 = fixed_cols_3p2m();
+    const SubPlan& sp = pick_ref<FX>(kFx1, a.sp);
     const int rlast = sp.radix[sp.nst - 1];
     const bool fuse = sp.nst >= 2 && (rlast == 2 || rlast == 3 || rlast == 4 || rlast == 5 || rlast == 8);
     const int ebase = pipe * (N1 * B) + bcol;
@@ -1303,13 +1316,6 @@ struct P2Args {
 };
 
 // F1: plain first row radix run from registers (0: none; see fuse1 below)
-template <bool A, class T>
-__device__ __forceinline__ const T& pick_ref(const T& a, const T& b)
-{
-    if constexpr (A) return a;
-    else return b;
-}
-
 // FX: the 3.2M row plan (fixed_rows_3p2m) with its strides and bounds as
 // constants, one pipeline per workgroup.
 template <int MAXPTS, int NSRC, bool CX, int F1 = 0, bool FX = false>
@@ -2690,9 +2696,26 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
     const int f1 = !(f1_env && p.sp1.nst >= 2 && p.p1_cx) ? 0
                    : (rf == 4 && r0 == 10) ? 10
                    : (rf == 8 && r0 == 16) ? 16 : 0;
+    // the 3.2M column plan with constant strides (SWSPEC_P1_FIXED=0: the generic kernel)
+    static const bool fx_env = [] {
+        const char* e = getenv("SWSPEC_P1_FIXED");
+        return !(e && atoi(e) == 0);
+    }();
+    bool fx = fx_env && f1 == 10 && p.N1 == kFxN1 && p.N2 == kFxN2 && p.B == 4 && p.p1_threads == kFxP1Threads &&
+              a.fpw == 1 && p.p1_maxpts == 32;
+    if (fx) {
+        constexpr SubPlan fc = fixed_cols_3p2m();
+        const SubPlan& q = p.sp1;
+        fx = q.L == fc.L && q.nst == fc.nst && q.T == fc.T;
+        for (int st = 0; fx && st < fc.nst; st++)
+            fx = q.radix[st] == fc.radix[st] && q.ns[st] == fc.ns[st] && q.mag[st] == fc.mag[st] &&
+                 q.twoff[st] == fc.twoff[st];
+    }
 #define SWSPEC_P1(MP, CXV)                                                                              \
     {                                                                                                 \
-        if (CXV && f1 == 10) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4, CXV ? 10 : 0>), grid, block, lds, s, a); \
+        if (CXV && MP == 32 && fx)                                                                    \
+            hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4, CXV ? 10 : 0, CXV && MP == 32>), grid, block, lds, s, a); \
+        else if (CXV && f1 == 10) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4, CXV ? 10 : 0>), grid, block, lds, s, a); \
         else if (CXV && f1 == 16) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 8, CXV ? 16 : 0>), grid, block, lds, s, a); \
         else if (rf == 4) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4>), grid, block, lds, s, a);    \
         else if (rf == 8) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 8>), grid, block, lds, s, a);    \
