@@ -140,17 +140,19 @@ __host__ __device__ constexpr int radix_ntw(int R)
     return radix_factored_tw(R) ? radix_a(R) + R / radix_a(R) - 2 : R - 1;
 }
 
-// A three-stage sub-plan as subplan_from builds it (the stage twiddle table
-// layout included), at compile time: the row pass of the 3.2M plan has its
-// own kernel instantiation in which every stride and bound is a constant.
-__host__ __device__ constexpr SubPlan fixed_subplan(int L, int r0, int r1, int r2, int T, int padst, int pad)
+// A sub-plan of up to three stages as subplan_from builds it (the stage
+// twiddle table layout included), at compile time: the plans of the common
+// lengths have their own kernel instantiations in which every stride and
+// bound is a constant.
+__host__ __device__ constexpr SubPlan fixed_subplan(int L, int T, int padst, int pad, int r0, int r1, int r2 = 0)
 {
     SubPlan sp{};
     const int rr[3] = {r0, r1, r2};
+    const int n = r2 ? 3 : 2;
     int ns = 1, twl = 0;
     sp.L = L;
-    sp.nst = 3;
-    for (int st = 0; st < 3; st++) {
+    sp.nst = n;
+    for (int st = 0; st < n; st++) {
         sp.radix[st] = rr[st];
         sp.ns[st] = ns;
         sp.mag[st] = ns > 1 ? (unsigned)(((1ULL << 32) + (unsigned long long)ns - 1) / (unsigned long long)ns) : 0u;
@@ -164,12 +166,35 @@ __host__ __device__ constexpr SubPlan fixed_subplan(int L, int r0, int r1, int r
     sp.pad = pad;
     return sp;
 }
-// 3.2M rows: 1280 = 5*16*16, 96 threads per row, penultimate stage padded by 5
-constexpr int kFxN2 = 1280, kFxT = 96;
-__host__ __device__ constexpr SubPlan fixed_rows_3p2m() { return fixed_subplan(kFxN2, 5, 16, 16, kFxT, 1, 5); }
-// 3.2M columns: 1250 = 10*25*5, 4 columns of 64 threads
-constexpr int kFxN1 = 1250, kFxP1Threads = 256;
-__host__ __device__ constexpr SubPlan fixed_cols_3p2m() { return fixed_subplan(kFxN1, 10, 25, 5, kFxP1Threads / 4, -1, 0); }
+// Fixed plans.  Id 1, 3.2M points: columns 1250 = 10*25*5, 4 columns of 64
+// threads; rows 1280 = 5*16*16, 96 threads per row, penultimate stage padded
+// by 5.  (The 2^20 plan as id 2, columns 16*32 and rows 4*16*16, measured no
+// faster than its generic kernels and is not built.)
+constexpr int kFixedPlans = 1;
+struct FixedPlan {
+    int N1, N2, B, p1_threads;
+    SubPlan cols, rows;
+    int rowst;   // LDS row stride of the row pass
+};
+__host__ __device__ constexpr FixedPlan fixed_plan(int id)
+{
+    FixedPlan f{};
+    if (id == 1) {
+        f.N1 = 1250; f.N2 = 1280; f.B = 4; f.p1_threads = 256;
+        f.cols = fixed_subplan(1250, 64, -1, 0, 10, 25, 5);
+        f.rows = fixed_subplan(1280, 96, 1, 5, 5, 16, 16);
+    }
+    f.rowst = f.N2 + (f.rows.padst >= 0 ? f.rows.pad * f.rows.radix[f.rows.nst - 1] : 0);
+    return f;
+}
+__host__ __device__ constexpr bool same_subplan(const SubPlan& a, const SubPlan& b)
+{
+    if (a.L != b.L || a.nst != b.nst || a.T != b.T || a.padst != b.padst || a.pad != b.pad) return false;
+    for (int st = 0; st < a.nst; st++)
+        if (a.radix[st] != b.radix[st] || a.ns[st] != b.ns[st] || a.mag[st] != b.mag[st] || a.twoff[st] != b.twoff[st])
+            return false;
+    return true;
+}
 
 template <bool A, class T>
 __device__ __forceinline__ const T& pick_ref(const T& a, const T& b)
@@ -1125,9 +1150,9 @@ __device__ __forceinline__ void fill_stage1_col(float2* __restrict__ lp, const u
 // for its own fill.
 // F1 (with RF): the first column stage, of radix F1, fused into the row fill
 // (fill_stage1; multi-channel layouts), so the LDS stages start at stage 1.
-// FX: the 3.2M column plan (fixed_cols_3p2m: 1250 = 10*25*5, 4 columns, 256
-// threads, one pipeline) with its strides and bounds as constants.
-template <int MAXPTS, bool CX, bool MF = false, int RF = 0, int F1 = 0, bool FX = false>
+// FX: a fixed plan (fixed_plan(FX): the 3.2M or the 2^20 column plan, one
+// pipeline) with its strides and bounds as constants; 0: the plan's values.
+template <int MAXPTS, bool CX, bool MF = false, int RF = 0, int F1 = 0, int FX = 0>
 __global__ void __launch_bounds__(512) k_pass1(P1Args a)
 {
     extern __shared__ float2 lds[];
@@ -1135,13 +1160,14 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     xcd_remap(bx, by, bz, a.zfast);
     const JobDev J = a.jobs[bz];
     // a workgroup runs `fpw` independent pipelines, one FFT each (short columns)
-    const int fpw = FX ? 1 : a.fpw, pth = FX ? kFxP1Threads : a.pth;
+    constexpr FixedPlan kFp = fixed_plan(FX);
+    const int fpw = FX ? 1 : a.fpw, pth = FX ? kFp.p1_threads : a.pth;
     if ((int)by * fpw >= J.nf) return;   // no FFT of this job here (whole workgroup)
     int pipe = 0;
     while (pipe + 1 < fpw && threadIdx.x >= (unsigned)((pipe + 1) * pth)) pipe++;
     const int f = (int)by * fpw + pipe;
     const bool active = f < J.nf;
-    const int B = FX ? 4 : a.B, N1 = FX ? kFxN1 : a.N1, N2 = FX ? kFxN2 : a.N2;
+    const int B = FX ? kFp.B : a.B, N1 = FX ? kFp.N1 : a.N1, N2 = FX ? kFp.N2 : a.N2;
     const int c0 = (int)bx * B;
     const int tid = threadIdx.x - pipe * pth, nth = pth;
     float2* __restrict__ lp = lds + pipe * (N1 * B);
@@ -1193,8 +1219,8 @@ __global__ void __launch_bounds__(512) k_pass1(P1Args a)
     // fused with the store below: its butterflies read LDS and write their
     // outputs straight to Y, saving an LDS write, a barrier and a read per
     // point.
-    constexpr SubPlan kFx1 = fixed_cols_3p2m();
-    const SubPlan& sp = pick_ref<FX>(kFx1, a.sp);
+    constexpr SubPlan kFx1 = kFp.cols;
+    const SubPlan& sp = pick_ref<FX != 0>(kFx1, a.sp);
     const int rlast = sp.radix[sp.nst - 1];
     const bool fuse = sp.nst >= 2 && (rlast == 2 || rlast == 3 || rlast == 4 || rlast == 5 || rlast == 8);
     const int ebase = pipe * (N1 * B) + bcol;
@@ -1316,22 +1342,23 @@ struct P2Args {
 };
 
 // F1: plain first row radix run from registers (0: none; see fuse1 below)
-// FX: the 3.2M row plan (fixed_rows_3p2m) with its strides and bounds as
-// constants, one pipeline per workgroup.
-template <int MAXPTS, int NSRC, bool CX, int F1 = 0, bool FX = false>
+// FX: a fixed row plan (fixed_plan(FX): 3.2M or 2^20) with its strides and
+// bounds as constants, one pipeline per workgroup; 0: the plan's values.
+template <int MAXPTS, int NSRC, bool CX, int F1 = 0, int FX = 0>
 __global__ void __launch_bounds__(512) k_pass2(P2Args a)
 {
     extern __shared__ float2 lds_all[];
-    constexpr SubPlan kFx = fixed_rows_3p2m();
-    const SubPlan& SP = pick_ref<FX>(kFx, a.sp);
+    constexpr FixedPlan kFp = fixed_plan(FX);
+    constexpr SubPlan kFx = kFp.rows;
+    const SubPlan& SP = pick_ref<FX != 0>(kFx, a.sp);
     const int p = blockIdx.x, grp = blockIdx.y;
-    const int N1 = a.N1, N2 = FX ? kFxN2 : a.N2;
-    const int RS = FX ? kFxN2 + 5 * 16 : a.rowst;   // LDS row stride (>= N2)
+    const int N1 = a.N1, N2 = FX ? kFp.N2 : a.N2;
+    const int RS = FX ? kFp.rowst : a.rowst;   // LDS row stride (>= N2)
     const int rowA = p;
     const int rowB = (p == 0) ? 0 : N1 - p;
     const bool self = (rowA == rowB);
     // pipeline (slice) of this thread: blockDim == fpw * pth
-    const int pth = FX ? 2 * NSRC * kFxT : a.pth;
+    const int pth = FX ? 2 * NSRC * kFx.T : a.pth;
     const int fpw = FX ? 1 : a.fpw;
     int pipe = 0;
     while (pipe + 1 < fpw && threadIdx.x >= (unsigned)((pipe + 1) * pth)) pipe++;
@@ -1341,7 +1368,7 @@ __global__ void __launch_bounds__(512) k_pass2(P2Args a)
     const int pipe_lds = FX ? 2 * NSRC * RS : a.pipe_lds;
     float2* __restrict__ lds = lds_all + pipe * pipe_lds;
     constexpr int NSL = 2 * NSRC;
-    const int T = FX ? kFxT : a.sp.T;       // pth == NSL * T
+    const int T = FX ? kFx.T : a.sp.T;       // pth == NSL * T
     int slot = 0;
 #pragma unroll
     for (int s = 1; s < NSL; s++) slot += (tid >= s * T) ? 1 : 0;
@@ -2696,25 +2723,22 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
     const int f1 = !(f1_env && p.sp1.nst >= 2 && p.p1_cx) ? 0
                    : (rf == 4 && r0 == 10) ? 10
                    : (rf == 8 && r0 == 16) ? 16 : 0;
-    // the 3.2M column plan with constant strides (SWSPEC_P1_FIXED=0: the generic kernel)
+    // a fixed column plan with constant strides (SWSPEC_P1_FIXED=0: the generic kernel)
     static const bool fx_env = [] {
         const char* e = getenv("SWSPEC_P1_FIXED");
         return !(e && atoi(e) == 0);
     }();
-    bool fx = fx_env && f1 == 10 && p.N1 == kFxN1 && p.N2 == kFxN2 && p.B == 4 && p.p1_threads == kFxP1Threads &&
-              a.fpw == 1 && p.p1_maxpts == 32;
-    if (fx) {
-        constexpr SubPlan fc = fixed_cols_3p2m();
-        const SubPlan& q = p.sp1;
-        fx = q.L == fc.L && q.nst == fc.nst && q.T == fc.T;
-        for (int st = 0; fx && st < fc.nst; st++)
-            fx = q.radix[st] == fc.radix[st] && q.ns[st] == fc.ns[st] && q.mag[st] == fc.mag[st] &&
-                 q.twoff[st] == fc.twoff[st];
+    int fx = 0;
+    for (int id = 1; fx_env && f1 && !fx && id <= kFixedPlans; id++) {
+        const FixedPlan fp = fixed_plan(id);
+        if (p.N1 == fp.N1 && p.N2 == fp.N2 && p.B == fp.B && p.p1_threads == fp.p1_threads && a.fpw == 1 &&
+            p.p1_maxpts == 32 && f1 == fp.cols.radix[0] && same_subplan(p.sp1, fp.cols))
+            fx = id;
     }
 #define SWSPEC_P1(MP, CXV)                                                                              \
     {                                                                                                 \
-        if (CXV && MP == 32 && fx)                                                                    \
-            hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4, CXV ? 10 : 0, CXV && MP == 32>), grid, block, lds, s, a); \
+        if (CXV && MP == 32 && fx == 1)                                                               \
+            hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4, CXV ? 10 : 0, (CXV && MP == 32) ? 1 : 0>), grid, block, lds, s, a); \
         else if (CXV && f1 == 10) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4, CXV ? 10 : 0>), grid, block, lds, s, a); \
         else if (CXV && f1 == 16) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 8, CXV ? 16 : 0>), grid, block, lds, s, a); \
         else if (rf == 4) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4>), grid, block, lds, s, a);    \
@@ -2865,20 +2889,20 @@ hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const Xpol
     // with the padded layout)
     if ((f1_env || p.sp2.padst >= 0) && p.p2_maxpts == 16 && (r1 == 4 || r1 == 5 || r1 == 8))
         k = table_f1[r1 == 4 ? 0 : (r1 == 5 ? 1 : 2)][nsrc - 1][p.p2_cx ? 1 : 0];
-    // the 3.2M row plan with constant strides (SWSPEC_P2_FIXED=0: the generic kernel)
+    // a fixed row plan with constant strides (SWSPEC_P2_FIXED=0: the generic kernel)
     static const bool fx_env = [] {
         const char* e = getenv("SWSPEC_P2_FIXED");
         return !(e && atoi(e) == 0);
     }();
-    if (fx_env && k == table_f1[1][nsrc - 1][1] && a.fpw == 1) {
-        constexpr SubPlan f = fixed_rows_3p2m();
-        const SubPlan& q = p.sp2;
-        bool same = p.N2 == kFxN2 && q.L == f.L && q.nst == f.nst && q.T == f.T && q.padst == f.padst &&
-                    q.pad == f.pad && a.rowst == p2_rowst(p) && p2_rowst(p) == kFxN2 + f.pad * f.radix[2];
-        for (int st = 0; same && st < f.nst; st++)
-            same = q.radix[st] == f.radix[st] && q.ns[st] == f.ns[st] && q.mag[st] == f.mag[st] &&
-                   q.twoff[st] == f.twoff[st];
-        if (same) k = nsrc == 1 ? k_pass2<16, 1, true, 5, true> : k_pass2<16, 2, true, 5, true>;
+    if (fx_env && a.fpw == 1 && p.p2_cx && p.p2_maxpts == 16) {
+        for (int id = 1; id <= kFixedPlans; id++) {
+            const FixedPlan fp = fixed_plan(id);
+            const int r0 = fp.rows.radix[0];
+            if (k != table_f1[r0 == 4 ? 0 : (r0 == 5 ? 1 : 2)][nsrc - 1][1]) continue;
+            if (p.N2 != fp.N2 || a.rowst != fp.rowst || !same_subplan(p.sp2, fp.rows)) continue;
+            k = nsrc == 1 ? k_pass2<16, 1, true, 5, 1> : k_pass2<16, 2, true, 5, 1>;
+            break;
+        }
     }
     hipLaunchKernelGGL(k, grid, block, lds, s, a);
     return hipGetLastError();
