@@ -148,7 +148,10 @@ def test_row_stream_with_pcal_matches_one_stream(hip, oracle):
                                            (1 << 20, "Cosine", False), (3200000, "Hamming", False),
                                            (3200000, "Blackman", True), (320000, "Hann", True),
                                            (3200000, "None", False), (3200000, "Cosine", True),
-                                           (1 << 20, "Hann", True)])
+                                           (1 << 20, "Hann", True),
+                                           # the fused fill in the kernels with the plan's own strides
+                                           # (1250 x 1600, 32 points per thread; 1000 x 1000, 16)
+                                           (4000000, "Cosine2", False), (2000000, "Hann", True)])
 def test_row_fill_windows_vs_oracle(hip, oracle, n, win, framed):
     """The row fill of pass 1 (a thread decodes and windows whole rows of the
     workgroup's 4 or 8 columns from one wide load; one byte per time sample)
