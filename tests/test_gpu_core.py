@@ -1,6 +1,6 @@
 """GPU parity tests: TaskCoreHIP (HIP kernels via the C ABI) against the oracle.
 
-Tolerances (floating point, stated per the north star):
+Tolerances (floating point):
   spectra / cross-spectra : normwise relative error <= 1e-5 vs the FP64 oracle
                             and vs the FP32 restatement of the reference core
   decoded samples         : bit-exact
