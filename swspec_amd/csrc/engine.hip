@@ -3658,8 +3658,8 @@ std::string plan_geometry(Plan& p, long long N, bool mfma64, const PlanKnobs* kn
     // (3.2M, rows 5*16*16: 5 of 16, 13 lanes each).  Blocks R*ns + ns apart
     // put lane q at q mod 16, conflict-free; the last stage reads them with
     // stride R*ns + ns (it reads blocks whole, q < R*ns) and writes the plain
-    // layout in place.  Not for a register first stage (it writes without
-    // the stage loop).  SWSPEC_P2_PAD=0 disables it.
+    // layout in place.  The padded stage is never stage 0 (a register first
+    // stage writes outside the stage loop).  SWSPEC_P2_PAD=0 disables it.
     {
         static const bool pad_env = [] {
             const char* e = getenv("SWSPEC_P2_PAD");
