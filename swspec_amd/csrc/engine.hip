@@ -1011,25 +1011,14 @@ __device__ __forceinline__ void fill_rows(float2* __restrict__ lp, const uint8_t
 // written for both columns as one 16-byte row piece.  This saves the fill's
 // LDS write and the first stage's LDS read of every point and one barrier.
 // SC: single-channel 2-bit layout, the two points one byte (oldest sample in
-// the MSB).  With 8 columns (radix 16) a thread takes butterfly q of one
-// column instead (fill_stage1_col).
-template <int BC, int F1, int WM, bool FR, bool SC>
-__device__ __forceinline__ void fill_stage1_col(float2* __restrict__ lp, const uint8_t* __restrict__ raw,
-                                                const Unpack& up, long long pay, const WindowGen& wg,
-                                                const float4* __restrict__ wcol, int tid, int nth, int N1, int N2,
-                                                int c0);
-
+// the MSB).
 template <int BC, int F1, int WM, bool FR, bool SC = false>
 __device__ __forceinline__ void fill_stage1(float2* __restrict__ lp, const uint8_t* __restrict__ raw,
                                             const Unpack& up, long long pay, const WindowGen& wg,
                                             const float4* __restrict__ wcol, int tid, int nth, int N1, int N2,
                                             int c0)
 {
-    static_assert(BC == 4 || BC == 8, "row fill: 4 or 8 columns");
-    if constexpr (BC == 8) {
-        fill_stage1_col<BC, F1, WM, FR, SC>(lp, raw, up, pay, wg, wcol, tid, nth, N1, N2, c0);
-        return;
-    }
+    static_assert(BC == 4, "fused first stage: 4 columns");
     constexpr int NP = BC / 2;   // column pairs
     const int nb = N1 / F1;
     const unsigned sh = (unsigned)up.shift;
@@ -1086,60 +1075,6 @@ __device__ __forceinline__ void fill_stage1(float2* __restrict__ lp, const uint8
         float4* dst = (float4*)(lp + (size_t)q * F1 * BC + cA);
 #pragma unroll
         for (int r = 0; r < F1; r++) dst[r * (BC / 2)] = make_float4(o[0][r].x, o[0][r].y, o[1][r].x, o[1][r].y);
-    }
-}
-
-// One column per thread: butterfly q of column c (item q*BC + c, columns
-// fastest, so a wave's LDS writes of one output are contiguous).  Row e of the
-// column is one point: 2 bytes (one byte per time sample; framed: a 2-byte
-// group never straddles a payload) or half a byte (single channel).  Outputs
-// go to LDS as the DFT produces them.
-template <int BC, int F1, int WM, bool FR, bool SC>
-__device__ __forceinline__ void fill_stage1_col(float2* __restrict__ lp, const uint8_t* __restrict__ raw,
-                                                const Unpack& up, long long pay, const WindowGen& wg,
-                                                const float4* __restrict__ wcol, int tid, int nth, int N1, int N2,
-                                                int c0)
-{
-    const int nb = N1 / F1;
-    const unsigned sh = (unsigned)up.shift;
-    const float a0 = wg.a0;
-    for (int it = tid; it < nb * BC; it += nth) {
-        const int q = it / BC;
-        const int c = it - q * BC;
-        float4 K[2];
-#pragma unroll
-        for (int r = 0; r < 2; r++) K[r] = WM ? wcol[2 * (c0 + c) + r] : make_float4(0.f, 0.f, 0.f, 0.f);
-        float2 v[F1];
-#pragma unroll
-        for (int r = 0; r < F1; r++) {
-            const int e = q + r * nb;
-            const unsigned pt = __umul24((unsigned)N2, (unsigned)e) + (unsigned)(c0 + c);
-            unsigned f0, f1;
-            if constexpr (SC) {
-                const unsigned byte = raw[pt >> 1], s0 = (pt & 1u) ? 2u : 6u;
-                f0 = (byte >> s0) & 3u;
-                f1 = (byte >> (s0 - 2u)) & 3u;
-            } else {
-                const unsigned t = FR ? *(const uint16_t*)(raw + mk5b_frame_addr(pay + 2LL * pt))
-                                      : *(const uint16_t*)(raw + 2u * pt);
-                f0 = (t >> sh) & 3u;
-                f1 = (t >> (sh + 8u)) & 3u;
-            }
-            float w0 = 1.f, w1 = 1.f;
-            if constexpr (WM != 0) {
-                const float2 z = wg.phi[e];
-                w0 = a0 + z.x * K[0].x + z.y * K[0].y;
-                w1 = a0 + z.x * K[1].x + z.y * K[1].y;
-                if constexpr (WM == 2) {
-                    const float c2 = z.x * z.x - z.y * z.y, s2 = 2.f * z.x * z.y;
-                    w0 += c2 * K[0].z + s2 * K[0].w;
-                    w1 += c2 * K[1].z + s2 * K[1].w;
-                }
-            }
-            v[r] = make_float2(map2bit(f0) * w0, map2bit(f1) * w1);
-        }
-        float2* dst = lp + (size_t)q * F1 * BC + c;
-        DftStore<F1>::template run<false>(v, nullptr, [&](int r, float2 x) { dst[r * BC] = x; });
     }
 }
 
@@ -2715,14 +2650,12 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
         const char* e = getenv("SWSPEC_P1_FUSE1");
         return !(e && atoi(e) == 0);
     }();
-    // With 8 columns, a radix-16 first stage (2^20: 16*32), one column per
-    // thread: 4-channel Mk5B 2^20 +1.6%, single channel unchanged.  (Radix
-    // 25, 8 columns, measured 18-21% slower at 10^6 (25*25: 128 VGPRs with
-    // spills) and unchanged at 320000, and is not used.)
+    // (A one-column-per-thread form for 8 columns, radix 16 or 25, measured
+    // 18-21% slower at 10^6 (25*25: 128 VGPRs with spills); the 8-column
+    // radix-16 plans, 2^20, run two pipelines per workgroup and so the
+    // column fill.  Neither is built.)
     const int r0 = p.sp1.radix[0];
-    const int f1 = !(f1_env && p.sp1.nst >= 2 && p.p1_cx) ? 0
-                   : (rf == 4 && r0 == 10) ? 10
-                   : (rf == 8 && r0 == 16) ? 16 : 0;
+    const int f1 = (f1_env && p.sp1.nst >= 2 && p.p1_cx && rf == 4 && r0 == 10) ? 10 : 0;
     // a fixed column plan with constant strides (SWSPEC_P1_FIXED=0: the generic kernel)
     static const bool fx_env = [] {
         const char* e = getenv("SWSPEC_P1_FIXED");
@@ -2740,7 +2673,6 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
         if (CXV && MP == 32 && fx == 1)                                                               \
             hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4, CXV ? 10 : 0, (CXV && MP == 32) ? 1 : 0>), grid, block, lds, s, a); \
         else if (CXV && f1 == 10) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4, CXV ? 10 : 0>), grid, block, lds, s, a); \
-        else if (CXV && f1 == 16) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 8, CXV ? 16 : 0>), grid, block, lds, s, a); \
         else if (rf == 4) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 4>), grid, block, lds, s, a);    \
         else if (rf == 8) hipLaunchKernelGGL((k_pass1<MP, CXV, false, 8>), grid, block, lds, s, a);    \
         else hipLaunchKernelGGL((k_pass1<MP, CXV, false, 0>), grid, block, lds, s, a);                 \

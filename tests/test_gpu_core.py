@@ -157,8 +157,9 @@ def test_row_fill_windows_vs_oracle(hip, oracle, n, win, framed):
     workgroup's 4 or 8 columns from one wide load; one byte per time sample)
     for every window form: none, one cosine term (Cosine, Hann, Hamming) and
     the cos(2 theta) term (Blackman), 4- and 8-column plans (3.2M: 4, with
-    the radix-10 first column stage fused into the fill; 320000, 10^6, 2^20:
-    8, 2^20 with its radix-16 first stage fused), payload and framed Mk5B.  All four channels of a 4-channel Mk5B
+    the radix-10 first column stage fused into the fill; 10^6: 8; 320000 and
+    2^20, two pipelines per workgroup, take the column fill), payload and
+    framed Mk5B.  All four channels of a 4-channel Mk5B
     stream against the FP64 oracle."""
     from swspec_amd import plan_info
     from swspec_amd._lib import FLAG_MK5B_FRAMED
