@@ -168,8 +168,8 @@ __host__ __device__ constexpr SubPlan fixed_subplan(int L, int T, int padst, int
 }
 // Fixed plans.  Id 1, 3.2M points: columns 1250 = 10*25*5, 4 columns of 64
 // threads; rows 1280 = 5*16*16, 96 threads per row, penultimate stage padded
-// by 5.  (The 2^20 plan as id 2, columns 16*32 and rows 4*16*16, measured no
-// faster than its generic kernels and is not built.)
+// by 5.  (A 2^20 entry would not be taken: both of its passes run two
+// pipelines per workgroup.)
 constexpr int kFixedPlans = 1;
 struct FixedPlan {
     int N1, N2, B, p1_threads;
