@@ -190,7 +190,7 @@ def cpu_baseline(raw_host, threads, ffts_per_thread=NFFT):
     return out
 
 
-def plain_cfg2(torch, swspec_amd, raw, s, reps=2):
+def plain_mk5b_4ch(torch, swspec_amd, raw, s, reps=2):
     """Per-job plain launches: one single-channel core per channel, one after
     the other (the reference's one-run-per-UseFile1Channel pattern)."""
     from swspec_amd import TaskCoreHIP
@@ -218,7 +218,7 @@ def plain_cfg2(torch, swspec_amd, raw, s, reps=2):
     return NCH * SAMPLES / dt / 1e9, dt * 1e3
 
 
-def xpol_cfg3_leg(torch, swspec_amd, nfft=64, reps=5):
+def xpol_2p20_leg(torch, swspec_amd, nfft=64, reps=5):
     """Cross-pol pair at 2^20 points: X and Y 2-bit single-channel streams at
     64 Ms/s, XX, YY and X.Y* with Kahan FP32 accumulators, nfft FFTs per
     source per step, device-resident input; column FFT in FP32 and with the
@@ -263,7 +263,7 @@ def xpol_cfg3_leg(torch, swspec_amd, nfft=64, reps=5):
     return res
 
 
-def host_cfg2_leg(torch, swspec_amd, raw, s, reps=4):
+def host_buffers_leg(torch, swspec_amd, raw, s, reps=4):
     """The drop-in path as TaskDispatcher drives it: the headline 4-channel
     job from a pinned host Buffer through run()/join(), outputs to host
     Buffers, PCIe copies included.  Input copied in 32 MB segments on a copy
@@ -313,7 +313,7 @@ def host_cfg2_leg(torch, swspec_amd, raw, s, reps=4):
     return res
 
 
-def framed_cfg2_leg(torch, swspec_amd, raw, s, reps=4):
+def framed_mk5b_leg(torch, swspec_amd, raw, s, reps=4):
     """The headline job fed as Mk5B frames as stored on disk (16-byte header
     + 10000-byte payload, SWSPEC_FLAG_MK5B_FRAMED): headers skipped on the
     device in the decode and the phase-cal fold, device-resident input."""
@@ -344,7 +344,7 @@ def framed_cfg2_leg(torch, swspec_amd, raw, s, reps=4):
             "ms_per_step": round(dt * 1e3, 3)}
 
 
-def batch_cfg4(torch, replays=3, seed=4):
+def batch_64_streams(torch, replays=3, seed=4):
     """64 streams of mixed FFT sizes: one hipGraph replay of the whole batch
     vs plain per-stream launches (each stream alone, step by step).  Every
     stream reads its whole input (nffts FFTs of random bytes, 111 GB in all)
@@ -413,7 +413,7 @@ def batch_cfg4(torch, replays=3, seed=4):
 SERVE_CAP_CHANNELS = 16   # continuous batching: channels admitted at once
 
 
-def serve_cfg5(torch, seconds, rate=200.0, seed=5):
+def serve_poisson(torch, seconds, rate=200.0, seed=5):
     """One Poisson trace of `seconds` of arrivals at `rate` jobs/s, replayed in
     real time twice: continuous batching (jobs admitted in arrival order while
     at most SERVE_CAP_CHANNELS channels are active, all advancing in the same
@@ -612,24 +612,24 @@ def main():
     if args.only_serve:
         for r in args.serve_rates.split(","):
             log("serving replay at %s jobs/s ..." % r)
-            print(json.dumps({"serving": serve_cfg5(torch, args.serve_seconds, rate=float(r))}), flush=True)
+            print(json.dumps({"serving": serve_poisson(torch, args.serve_seconds, rate=float(r))}), flush=True)
         return
 
     import swspec_amd
     from swspec_amd import TaskCoreHIP, plan_info
 
     if args.only_xpol:
-        print(json.dumps({"xpol_2p20": xpol_cfg3_leg(torch, swspec_amd)}), flush=True)
+        print(json.dumps({"xpol_2p20": xpol_2p20_leg(torch, swspec_amd)}), flush=True)
         return
     if args.only_batch:
-        print(json.dumps({"batch_64_streams": batch_cfg4(torch)}), flush=True)
+        print(json.dumps({"batch_64_streams": batch_64_streams(torch)}), flush=True)
         return
 
     log("generating input on the device ...")
     raw, _ = make_input(torch)
     s = settings(swspec_amd)
     if args.only_host:
-        print(json.dumps({"host_buffers_pcie_inclusive": host_cfg2_leg(torch, swspec_amd, raw, s)}), flush=True)
+        print(json.dumps({"host_buffers_pcie_inclusive": host_buffers_leg(torch, swspec_amd, raw, s)}), flush=True)
         return
     assert s.core_overlapped_ffts == NFFT and s.extract_pcal == 1
     core = TaskCoreHIP(0)
@@ -746,33 +746,33 @@ def main():
     framed = None
     if not args.no_framed:
         log("Mk5B frames as on disk (device-side header skip) ...")
-        framed = framed_cfg2_leg(torch, swspec_amd, raw, s)
+        framed = framed_mk5b_leg(torch, swspec_amd, raw, s)
 
     host = None
     if not args.no_host:
         log("host Buffers through run()/join() (PCIe included) ...")
-        host = host_cfg2_leg(torch, swspec_amd, raw, s)
+        host = host_buffers_leg(torch, swspec_amd, raw, s)
 
     plain_value = plain_ms = None
     if not args.no_plain:
         log("plain per-channel cores ...")
-        plain_value, plain_ms = plain_cfg2(torch, swspec_amd, raw, s)
+        plain_value, plain_ms = plain_mk5b_4ch(torch, swspec_amd, raw, s)
     batch = serve = None
     if not args.no_batch:
         log("64-stream batch ...")
-        batch = batch_cfg4(torch)
+        batch = batch_64_streams(torch)
     if not args.no_serve:
         log("serving replay ...")
-        serve = serve_cfg5(torch, args.serve_seconds)
+        serve = serve_poisson(torch, args.serve_seconds)
         # the same job mix near the device's capacity: continuous batching
         # keeps up where one job at a time falls behind
         log("serving near capacity ...")
-        serve["near_capacity"] = serve_cfg5(torch, args.serve_hi_seconds, rate=args.serve_rate_hi, seed=6)
+        serve["near_capacity"] = serve_poisson(torch, args.serve_hi_seconds, rate=args.serve_rate_hi, seed=6)
 
     xpol = None
     if not args.no_xpol:
         log("cross-pol 2^20 (fp32 / mfma64 column stage) ...")
-        xpol = xpol_cfg3_leg(torch, swspec_amd)
+        xpol = xpol_2p20_leg(torch, swspec_amd)
 
     pcalx = None
     if not args.no_pcalx:

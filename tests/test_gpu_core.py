@@ -74,10 +74,10 @@ def test_spectrum_vs_oracle(hip, oracle, n, win, nf):
 
 
 def test_multichannel_mk5b_and_pcal(hip, oracle):
-    """cfg2 geometry at reduced length: 4-channel Mk5B payload, 3.2M points,
+    """4-channel Mk5B 3.2M-point geometry at reduced length: 4-channel Mk5B payload, 3.2M points,
     Cosine2, phase-cal comb; all four channels in one pass."""
     n, nf, fs = 3200000, 2, 32e6
-    say("cfg2 small", n, nf)
+    say("3.2M 4ch small", n, nf)
     s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=1,
                             WindowType="Cosine2", BandwidthHz=16e6, BitsPerSample=2, SourceChannels=4,
                             SourceFormat="Mk5B", ExtractPCal=True, MaxSourceBufferMB=4096)
@@ -463,7 +463,7 @@ def test_combine_device_matches_host_combine(hip):
     dc.finalize()
 
 
-def test_parseval_full_cfg2_length(hip):
+def test_parseval_full_3p2m_length(hip):
     """Size-independent property at the full 3.2M-point length with many FFTs:
     sum_k c_k P[k] = N * mean_f sum_n (w x)^2 (c = 1 at DC/Nyquist, 2 otherwise)."""
     import torch

@@ -114,17 +114,17 @@ def test_full_length_32000pt_1600ffts(hip, oracle):
 # ------------------------------------------------------------- 3.2M x 100, 4-channel Mk5B, phase-cal
 
 N2, FS2, NF2 = 3200000, 32e6, 100
-_cfg2 = {}
+_mk5b4 = {}
 
 
-def _cfg2_data(hip, oracle):
+def _mk5b4_data(hip, oracle):
     """The 4-channel payload (bench.py's seed), its FP64 oracle spectra and
     phase-cal folds per channel; computed once per module."""
-    if _cfg2:
-        return _cfg2
+    if _mk5b4:
+        return _mk5b4
     import torch
     from swspec_amd import workloads as W
-    say("cfg 3.2M x 100 x 4ch: generating")
+    say("3.2M x 100 x 4ch: generating")
     raw, _ = W.mk5b_payload(torch, 2024, N2 * NF2)
     host = raw.cpu().numpy()
     spectra, folds = [], []
@@ -151,11 +151,11 @@ def _cfg2_data(hip, oracle):
         return acc[0::2] + 1j * acc[1::2]
     with ThreadPoolExecutor(max_workers=4) as ex:
         fold32 = list(ex.map(fold, range(4)))
-    _cfg2.update(raw=raw, host=host, spectra=spectra, folds=folds, fold32=fold32)
-    return _cfg2
+    _mk5b4.update(raw=raw, host=host, spectra=spectra, folds=folds, fold32=fold32)
+    return _mk5b4
 
 
-def _cfg2_settings(hip, **over):
+def _mk5b4_settings(hip, **over):
     k = dict(FFTpoints=N2, FFTIntegrationTimeSec=10.0, FFToverlapFactor=1, WindowType="Cosine2",
              BandwidthHz=16e6, BitsPerSample=2, SourceChannels=4, SourceFormat="Mk5B", ExtractPCal=True,
              PCalOffsetHz=10e3, MaxSourceBufferMB=1024)
@@ -165,8 +165,8 @@ def _cfg2_settings(hip, **over):
 
 def test_full_length_3p2m_4ch_pcal(hip, oracle):
     from swspec_amd._lib import FLAG_NO_KAHAN
-    d = _cfg2_data(hip, oracle)
-    s = _cfg2_settings(hip)
+    d = _mk5b4_data(hip, oracle)
+    s = _mk5b4_settings(hip)
     assert s.core_overlapped_ffts == NF2 and s.extract_pcal == 1 and s.pcal_tonebins == 32
     f32 = _bg(lambda: oracle.domaths(oracle_settings(oracle, s), [d["host"]], 32))
     kah, _, pc, tones = _run(hip, s, [d["raw"]], channels=[0, 1, 2, 3])
@@ -229,8 +229,8 @@ def test_full_length_2p20_xpol_200ffts(hip, oracle):
 
 # ------------------------------------------------------------- dispatcher: spectrum over 25 buffers
 
-def _cfg2_file(tmp_path_factory, d):
-    p = tmp_path_factory.getbasetemp() / "cfg2_payload.raw"
+def _mk5b4_file(tmp_path_factory, d):
+    p = tmp_path_factory.getbasetemp() / "mk5b4_payload.raw"
     if not p.exists():
         d["host"].tofile(str(p))
     return str(p)
@@ -244,12 +244,12 @@ def test_dispatcher_combine_25_buffers_host_and_device(hip, oracle, tmp_path_fac
     oracle's (1/K) x sum over all 100 FFTs and the sum of all folds."""
     import torch
     from swspec_amd import dispatcher, sinks
-    d = _cfg2_data(hip, oracle)
-    s = _cfg2_settings(hip, MaxSourceBufferMB=32, UseFile1Channel=1)
+    d = _mk5b4_data(hip, oracle)
+    s = _mk5b4_settings(hip, MaxSourceBufferMB=32, UseFile1Channel=1)
     assert (s.core_overlapped_ffts, s.max_buffers_per_spectrum, s.max_spectra_per_buffer) == (4, 25, 0)
     assert s.rawbuf_size == 4 * N2
-    path = _cfg2_file(tmp_path_factory, d)
-    prefix = str(tmp_path_factory.getbasetemp() / "py_cfg2")
+    path = _mk5b4_file(tmp_path_factory, d)
+    prefix = str(tmp_path_factory.getbasetemp() / "py_mk5b4")
     nsp, files = dispatcher.run_files(s, [path], prefix)
     assert nsp == 1
     spec = sinks.read_spectra(files[0], N2)
@@ -295,7 +295,7 @@ def test_cpp_adapter_driver_matches_python_dispatcher(hip, oracle, tmp_path_fact
     from oracle import oracle as O
     drv = O.REF_ADAPTER_DRV
     assert os.path.exists(drv), "oracle/_ref/taskcorehip_drv is built by __graft_entry__.build()"
-    d = _cfg2_data(hip, oracle)
+    d = _mk5b4_data(hip, oracle)
     base = tmp_path_factory.getbasetemp()
 
     def both(keys, inputs, name):
@@ -324,7 +324,7 @@ def test_cpp_adapter_driver_matches_python_dispatcher(hip, oracle, tmp_path_fact
     keys = dict(FFTpoints=N2, FFTIntegrationTimeSec=10.0, FFToverlapFactor=1, WindowType="Cosine2",
                 BandwidthHz=16e6, BitsPerSample=2, SourceChannels=4, SourceFormat="Mk5B", ExtractPCal=True,
                 PCalOffsetHz=10e3, MaxSourceBufferMB=32, UseFile1Channel=3)
-    both(keys, [_cfg2_file(tmp_path_factory, d)], "combine")
+    both(keys, [_mk5b4_file(tmp_path_factory, d)], "combine")
     # (b) direct writes: 64000 points (20 rotator periods), overlap 2, 8 spectra per buffer
     n, fs = 64000, 32e6
     keys = dict(FFTpoints=n, FFTIntegrationTimeSec=n * 3 / fs, FFToverlapFactor=2, WindowType="Hann",

@@ -74,7 +74,7 @@ def _hilbert(x):
     return np.fft.ifft(np.fft.fft(x) * h).imag
 
 
-def test_mfma64_xpol_cfg3_shape(hip, oracle):
+def test_mfma64_xpol_2p20_pair(hip, oracle):
     """Cross-pol pair at the 2^20-point length: X, Y Gaussian with complex
     correlation 0.2 at 30 degrees (Y = rho (cos X + sin H{X}) + sqrt(1 - rho^2) Z),
     2-bit, 64 Ms/s, Kahan FP32 accumulators; error of XX, YY and XY* vs FP64

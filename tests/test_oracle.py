@@ -14,7 +14,7 @@ M5 = np.float32(3.3359)
 
 # ---------------------------------------------------------------- settings
 
-def test_settings_cfg1(oracle):
+def test_settings_32000pt_1ch(oracle):
     s = oracle.derive(fft_points=32000, fft_integ_seconds=1.6, fft_overlap_factor=1, window="Hann",
                       bandwidth_hz=16e6, bits_per_sample=2, source_channels=1)
     assert (s.averaged_ffts, s.raw_fullfft_bytes, s.raw_overlap_bytes) == (1600, 8000, 8000)
@@ -23,7 +23,7 @@ def test_settings_cfg1(oracle):
 
 
 @pytest.mark.parametrize("cap,rawbuf,S,B,K", [(-1.0, 12800000, 4, 25, 4), (1024.0, 320000000, 100, 0, 100)])
-def test_settings_cfg2(oracle, cap, rawbuf, S, B, K):
+def test_settings_3p2m_4ch_mk5b(oracle, cap, rawbuf, S, B, K):
     s = oracle.derive(fft_points=3200000, fft_integ_seconds=10.0, fft_overlap_factor=1, window="Cosine2",
                       bandwidth_hz=16e6, bits_per_sample=2, source_channels=4, source_format="Mk5B",
                       extract_pcal=1, max_source_buffer_mb=cap)
@@ -34,7 +34,7 @@ def test_settings_cfg2(oracle, cap, rawbuf, S, B, K):
 
 
 @pytest.mark.parametrize("n,T", [(1 << 20, 327.68), (1000000, 312.5)])
-def test_settings_cfg3(oracle, n, T):
+def test_settings_xpol_pair(oracle, n, T):
     s = oracle.derive(fft_points=n, fft_integ_seconds=T, fft_overlap_factor=1, bandwidth_hz=32e6,
                       bits_per_sample=2, source_channels=1, num_sources=2, do_xpol=1)
     assert s.averaged_ffts == 20000 and s.num_xpols == 1
@@ -222,7 +222,7 @@ def test_cfft64_vs_numpy(oracle):
 
 # ---------------------------------------------------------------- doMaths
 
-def _cfg1_like(oracle, n=32000, T=None, nffts=8, **kw):
+def _one_channel(oracle, n=32000, T=None, nffts=8, **kw):
     fs = 32e6
     T = T if T is not None else n * nffts / fs
     args = dict(fft_points=n, fft_integ_seconds=T, fft_overlap_factor=1, window="Hann", bandwidth_hz=fs / 2,
@@ -232,8 +232,8 @@ def _cfg1_like(oracle, n=32000, T=None, nffts=8, **kw):
 
 
 def test_domaths_tone_bin(oracle):
-    """cfg1 tone at 0.3*fs/2 = 4.8 MHz lands on bin 4800 (df = 1 kHz)."""
-    s = _cfg1_like(oracle, nffts=4)
+    """32000-point tone at 0.3*fs/2 = 4.8 MHz lands on bin 4800 (df = 1 kHz)."""
+    s = _one_channel(oracle, nffts=4)
     raw = synth.raw_1ch(11, 32000 * 4, tones=[(0.3, 4.8e6, 0.0)], fs=32e6)
     n, auto, _, _ = oracle.domaths(s, [raw], 64)
     assert n == 1
@@ -257,7 +257,7 @@ def test_domaths_design_vector(oracle):
 
 
 def test_domaths_fp32_vs_fp64(oracle):
-    s = _cfg1_like(oracle, nffts=16)
+    s = _one_channel(oracle, nffts=16)
     raw = synth.raw_1ch(2, 32000 * 16, tones=[(0.05, 4.8e6, 0.0)], fs=32e6)
     _, a32, _, _ = oracle.domaths(s, [raw], 32)
     _, a64, _, _ = oracle.domaths(s, [raw], 64)
@@ -281,7 +281,7 @@ def test_domaths_matches_numpy_pipeline(oracle):
 
 
 def test_domaths_xpol_equal_inputs(oracle):
-    s = _cfg1_like(oracle, n=4000, nffts=3, num_sources=2, do_xpol=1)
+    s = _one_channel(oracle, n=4000, nffts=3, num_sources=2, do_xpol=1)
     raw = synth.raw_1ch(4, 4000 * 3, fs=32e6)
     n, auto, xpol, _ = oracle.domaths(s, [raw, raw], 64)
     assert n == 1
@@ -293,9 +293,9 @@ def test_domaths_xpol_equal_inputs(oracle):
 def test_domaths_combine_identity(oracle):
     """Sum_b (1/K) Sum_{k in b} |X_k|^2 = (1/K) Sum_all (TaskCoreIPP.cpp:124,311-349)."""
     n = 4000
-    big = _cfg1_like(oracle, n=n, nffts=4, max_source_buffer_mb=1024)
+    big = _one_channel(oracle, n=n, nffts=4, max_source_buffer_mb=1024)
     # cap so that two FFTs fit a buffer: K = 2, two buffers per spectrum
-    small = _cfg1_like(oracle, n=n, nffts=4, max_source_buffer_mb=2 * (n // 4) * 2 / 1048576 * 1.0 + 1e-9)
+    small = _one_channel(oracle, n=n, nffts=4, max_source_buffer_mb=2 * (n // 4) * 2 / 1048576 * 1.0 + 1e-9)
     assert big.core_overlapped_ffts == 4
     raw = synth.raw_1ch(8, n * 4, fs=32e6)
     _, a, _, _ = oracle.domaths(big, [raw], 64)
@@ -307,7 +307,7 @@ def test_domaths_combine_identity(oracle):
 
 def test_domaths_partial_and_overlap(oracle):
     n = 4000
-    s = _cfg1_like(oracle, n=n, nffts=3, max_source_buffer_mb=1024)
+    s = _one_channel(oracle, n=n, nffts=3, max_source_buffer_mb=1024)
     raw = synth.raw_1ch(6, n * 4, fs=32e6)
     m, auto, _, _ = oracle.domaths(s, [raw], 64)
     assert m == 2          # one full spectrum of 3 FFTs + a partial of 1, unscaled
@@ -315,7 +315,7 @@ def test_domaths_partial_and_overlap(oracle):
     w = np.hanning(n)
     P3 = np.abs(np.fft.rfft(x[3 * n:4 * n] * w)) ** 2
     assert np.allclose(auto[0][1], P3, rtol=1e-10, atol=1e-9)
-    so = _cfg1_like(oracle, n=n, nffts=3, fft_overlap_factor=2, max_source_buffer_mb=1024)
+    so = _one_channel(oracle, n=n, nffts=3, fft_overlap_factor=2, max_source_buffer_mb=1024)
     assert so.core_overlapped_ffts == 5
     assert oracle.count_spectra(so, n // 4 * 3) == (1, 5)
 
