@@ -282,6 +282,35 @@ def test_xpol_vs_oracle(hip, oracle):
     core.finalize()
 
 
+def test_xpol_3p2m_constant_stride_kernels(hip, oracle):
+    """Cross-pol pair at 3.2M points (single-channel 2-bit, 32 Ms/s, Hann):
+    the plan is the one with compile-time strides (columns 1250 = 10*25*5,
+    rows 1280 = 5*16*16 with the padded stage), so pass 1 runs the
+    constant-stride fused-fill kernel and pass 2 its two-source constant-
+    stride form; X, Y, XY* vs the FP64 oracle (TaskCoreIPP.cpp:438-498)."""
+    n, nf, fs = 3200000, 2, 32e6
+    say("xpol 3.2M", n, nf)
+    pi = hip.plan_info(n)
+    assert (pi["N1"], pi["N2"], pi["B"], pi["p1_radix0"], pi["p2_pad"]) == (1250, 1280, 4, 10, 5), pi
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=1, WindowType="Hann",
+                            BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=1, DoCrossPolarization=True,
+                            MaxSourceBufferMB=4096, num_sources=2)
+    assert s.num_xpols == 1 and s.core_overlapped_ffts == nf
+    rng = np.random.default_rng(33)
+    a = rng.standard_normal(n * nf)
+    b = 0.3 * a + np.sqrt(1 - 0.09) * rng.standard_normal(n * nf)
+    ra, rb = synth.pack_1ch(synth.quantize(a)), synth.pack_1ch(synth.quantize(b))
+    m, auto, xp, _, core = run_device(s, [ra, rb])
+    os_ = oracle_settings(oracle, s)
+    _, a64, x64, _ = oracle.domaths(os_, [ra, rb], 64)
+    assert m == 1
+    ea, eb, ex = relerr(auto[0], a64[0]), relerr(auto[1], a64[1]), relerr(xp[0], x64[0])
+    say("  auto %.3e %.3e xpol %.3e" % (ea, eb, ex))
+    assert ea <= TOL and eb <= TOL and ex <= TOL
+    assert xp[0][0, 0].imag == 0 and xp[0][0, -1].imag == 0
+    core.finalize()
+
+
 def test_schedule_overlap_partial_multispectrum(hip, oracle):
     """Overlap factor 2, several spectra per buffer, trailing partial spectrum."""
     n, fs = 6400, 32e6     # N % rotatorlen (3200) == 0 keeps phase-cal on
