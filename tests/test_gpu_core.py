@@ -45,8 +45,11 @@ def test_unpack_bitexact(hip, oracle, fmt, nch, incr):
 SIZES = [
     (4096, "Hann", 5), (20000, "Cosine2", 3), (32000, "Hann", 4), (65536, "Hamming", 3),
     (320000, "Blackman", 2), (1 << 20, "Cosine", 2), (1000000, "None", 2), (3200000, "Cosine2", 2),
-    # 3.2M with the first column stage fused into the fill: the other window forms
+    # 3.2M: Blackman with the first column stage fused into the fill; None
+    # (no per-column window table) with the column fill
     (3200000, "Blackman", 2), (3200000, "None", 2),
+    # 10^6 single-channel with a window: the 8-column row fill
+    (1000000, "Cosine", 2),
     # odd row lengths: 2700 = 2 * 30 x 45 (even column count: a self-mirrored
     # middle row with a self-paired bin), 4374 = 2 * 27 x 81, 156250 = 2 * 125 x 625
     (2700, "Hann", 5), (4374, "Cosine2", 4), (156250, "Hamming", 2),
@@ -149,18 +152,21 @@ def test_row_stream_with_pcal_matches_one_stream(hip, oracle):
                                            (3200000, "Blackman", True), (320000, "Hann", True),
                                            (3200000, "None", False), (3200000, "Cosine", True),
                                            (1 << 20, "Hann", True),
+                                           # 8-column row fill (10^6: columns 625 = 25 x 25)
+                                           (1000000, "Hann", False), (1000000, "Blackman", True),
                                            # the fused fill in the kernels with the plan's own strides
                                            # (1250 x 1600, 32 points per thread; 1000 x 1000, 16)
                                            (4000000, "Cosine2", False), (2000000, "Hann", True)])
 def test_row_fill_windows_vs_oracle(hip, oracle, n, win, framed):
     """The row fill of pass 1 (a thread decodes and windows whole rows of the
     workgroup's 4 or 8 columns from one wide load; one byte per time sample)
-    for every window form: none, one cosine term (Cosine, Hann, Hamming) and
-    the cos(2 theta) term (Blackman), 4- and 8-column plans (3.2M: 4, with
-    the radix-10 first column stage fused into the fill; 10^6: 8; 320000 and
-    2^20, two pipelines per workgroup, take the column fill), payload and
-    framed Mk5B.  All four channels of a 4-channel Mk5B
-    stream against the FP64 oracle."""
+    for the window forms it takes: one cosine term (Cosine, Hann, Hamming)
+    and the cos(2 theta) term (Blackman), 4- and 8-column plans (3.2M: 4,
+    with the radix-10 first column stage fused into the fill; 10^6: 8),
+    payload and framed Mk5B; beside them the column fill of the same plans
+    (window None has no per-column window table, so it keeps the column
+    fill; 320000 and 2^20, two pipelines per workgroup, too).  All four
+    channels of a 4-channel Mk5B stream against the FP64 oracle."""
     from swspec_amd import plan_info
     from swspec_amd._lib import FLAG_MK5B_FRAMED
     nf, fs = 2, 32e6
@@ -187,7 +193,8 @@ def test_row_fill_windows_vs_oracle(hip, oracle, n, win, framed):
 
 
 @pytest.mark.parametrize("nch,channels,framed", [(8, list(range(8)), False), (4, [1, 3], False),
-                                                 (4, [0, 1, 2, 3], True), (8, [4, 6], True)])
+                                                 (4, [0, 1, 2, 3], True), (8, [4, 6], True),
+                                                 (8, [0, 2], False), (8, [4, 5, 6, 7], True)])
 def test_channel_subsets_vs_oracle(hip, oracle, nch, channels, framed):
     """Several channels of one stream in one core (jobs sharing the stream's
     bytes in the same launches): all 8 channels of 8-channel Mk5B (bytes 0
@@ -224,6 +231,16 @@ def test_channel_subsets_vs_oracle(hip, oracle, nch, channels, framed):
     ("RawSigned", 8, 0, 16e6, 6400),      # fast fold, 2 bytes per time sample
     ("Mk5B", 8, 1, 32e6, 12800),          # 64 Ms/s: 64 tone bins, 6400-sample rotator
     ("RawSigned", 4, 1, 9e6, 18000),      # 18 tone bins: generic fold kernel
+    # 1600 / 3200 groups of 4 rotator positions: 2 / 4 groups per thread
+    ("RawSigned", 1, 1, 32e6, 64000),
+    ("RawSigned", 1, 1, 64e6, 128000),
+    ("Mk5B", 4, 1, 32e6, 64000),
+    ("Mk5B", 4, 1, 64e6, 128000),
+    ("Mk5B", 8, 1, 64e6, 128000),
+    # 16 channels: 4 bytes per time sample, 1 / 2 / 4 groups per thread
+    ("RawSigned", 16, 1, 16e6, 32000),
+    ("RawSigned", 16, 1, 32e6, 64000),
+    ("RawUnsigned", 16, 0, 64e6, 128000),
 ])
 def test_pcal_formats(hip, oracle, fmt, nch, incr, bw, n):
     say("pcal", fmt, nch, incr, bw, n)
@@ -309,6 +326,45 @@ def test_xpol_3p2m_constant_stride_kernels(hip, oracle):
     assert ea <= TOL and eb <= TOL and ex <= TOL
     assert xp[0][0, 0].imag == 0 and xp[0][0, -1].imag == 0
     core.finalize()
+
+
+@pytest.mark.parametrize("n,knobs", [
+    (65536, {"SWSPEC_COMPOSITE": "0"}),
+    (1000000, {"SWSPEC_COMPOSITE": "0"}),
+    (65536, {"SWSPEC_P2_MAXPTS": "32"}),
+    (1000000, {"SWSPEC_P2_MAXPTS": "32"}),
+    (1000000, {"SWSPEC_P2_MAXPTS": "32", "SWSPEC_COMPOSITE": "0"}),
+    (1000000, {"SWSPEC_P1_MAXPTS": "16"}),
+    (4096, {"SWSPEC_COMPOSITE": "0"}),   # single-pass short kernel, two sources, plain radices
+])
+def test_xpol_plan_variants(hip, oracle, n, knobs):
+    """Cross-pol pairs on the plan variants the autotuner can choose (plain
+    radices only, 32 row points per thread, 16 column points per thread):
+    the two-source row kernels of those plans, X, Y and XY* vs the FP64
+    oracle.  The knobs are read when the core builds its plan (prepare)."""
+    import os
+    nf, fs = 2, 64e6
+    say("xpol plan", n, knobs)
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=1, WindowType="Hann",
+                            BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=1, DoCrossPolarization=True,
+                            MaxSourceBufferMB=4096, num_sources=2)
+    rng = np.random.default_rng(n % 1000 + len(knobs))
+    a = rng.standard_normal(n * nf)
+    b = 0.25 * a + np.sqrt(1 - 0.0625) * rng.standard_normal(n * nf)
+    ra, rb = synth.pack_1ch(synth.quantize(a)), synth.pack_1ch(synth.quantize(b))
+    os.environ.update(knobs)
+    try:
+        m, auto, xp, _, core = run_device(s, [ra, rb])
+    finally:
+        for k in knobs:
+            del os.environ[k]
+    core.finalize()
+    os_ = oracle_settings(oracle, s)
+    _, a64, x64, _ = oracle.domaths(os_, [ra, rb], 64)
+    assert m == 1
+    ea, eb, ex = relerr(auto[0], a64[0]), relerr(auto[1], a64[1]), relerr(xp[0], x64[0])
+    say("  auto %.3e %.3e xpol %.3e" % (ea, eb, ex))
+    assert ea <= TOL and eb <= TOL and ex <= TOL
 
 
 def test_schedule_overlap_partial_multispectrum(hip, oracle):
@@ -709,15 +765,16 @@ def test_unpack_mk5b_framed_bitexact(hip, oracle):
             assert np.array_equal(d_out.cpu().numpy(), ref), (nch, ch)
 
 
-@pytest.mark.parametrize("nch", [4, 8])
-def test_mk5b_framed_run_matches_stripped(hip, oracle, nch):
+@pytest.mark.parametrize("nch,fs", [(4, 32e6), (8, 32e6), (4, 64e6), (8, 64e6), (4, 128e6), (8, 128e6)])
+def test_mk5b_framed_run_matches_stripped(hip, oracle, nch, fs):
     """A core fed the on-disk frames gives the spectra of the core fed the
     stripped payload bit for bit (same decoded samples, same FFT), and both
     match the oracle; phase-cal bit for bit too (the same fold reads each
     payload word -- 4 channels -- or 8-byte group -- 8 channels -- through
-    its frame address)."""
+    its frame address).  32 / 64 / 128 Ms/s: rotators of 3200 / 6400 / 12800
+    samples, 1 / 2 / 4 groups of 4 positions per thread."""
     from swspec_amd._lib import FLAG_MK5B_FRAMED
-    n, nf, fs = 320000, 3, 32e6
+    n, nf = 320000, 3
     say("mk5b framed run", n, nf, nch)
     s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=2, WindowType="Cosine2",
                             BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=nch, SourceFormat="Mk5B",
