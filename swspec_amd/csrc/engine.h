@@ -248,9 +248,6 @@ hipError_t launch_pass1(const Plan& p, const JobDev* jobs, int njobs, int nfft, 
 // finalize sums all slices in order.
 hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const XpolDev* xp, int nfft,
                         int kahan, int smax, hipStream_t s);
-hipError_t launch_finalize(const Plan& p, const float* acc, float* out, double scale, int nslices, hipStream_t s);
-hipError_t launch_finalize_xpol(const Plan& p, const float* acc, float* out, double scale, int nslices,
-                                hipStream_t s);
 hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s);
 // dst[i] += src[i]; src[i] = 0 for i < n
 hipError_t launch_add_clear(float* dst, float* src, size_t n, hipStream_t s);

@@ -1818,40 +1818,6 @@ __global__ void __launch_bounds__(512)
 // scaled in FP64, accumulators cleared for the next spectrum.
 // ---------------------------------------------------------------------------
 
-// Slices are summed in slice order in FP64 (sum - compensation per slice).
-__global__ void k_finalize(float* __restrict__ acc, float* __restrict__ out, long long M, int N1, int N2,
-                           double scale, int S, long long stride)
-{
-    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k <= M;
-         k += (long long)gridDim.x * blockDim.x) {
-        const long long idx = (k == M) ? M : (k % N1) * (long long)N2 + k / N1;
-        double v = 0.0;
-        for (int s = 0; s < S; s++) {
-            const float* a = acc + s * stride + 2 * idx;
-            v += (double)a[0] - (double)a[1];
-        }
-        out[k] = (float)(v * scale);
-    }
-}
-
-__global__ void k_finalize_xpol(float* __restrict__ acc, float* __restrict__ out, long long M, int N1, int N2,
-                                double scale, int S, long long stride)
-{
-    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k <= M;
-         k += (long long)gridDim.x * blockDim.x) {
-        const long long idx = (k == M) ? M : (k % N1) * (long long)N2 + k / N1;
-        double re = 0.0, im = 0.0;
-        for (int s = 0; s < S; s++) {
-            const float* a = acc + s * stride + 4 * idx;
-            re += (double)a[0] - (double)a[2];
-            im += (double)a[1] - (double)a[3];
-        }
-        out[2 * k] = (float)(re * scale);
-        // DC and Nyquist cross products are real (TaskCoreIPP.cpp:482-485)
-        out[2 * k + 1] = (k == 0 || k == M) ? 0.0f : (float)(im * scale);
-    }
-}
-
 // All jobs of a spectrum in one launch, through 32 x 32 tiles: a tile's
 // accumulator entries (k1, k2) are read along k2 (contiguous, 256-byte row
 // segments), summed over slices in slice order in FP64, scaled, and written
@@ -2837,25 +2803,6 @@ hipError_t launch_pass2(const Plan& p, const JobDev* jobs, int njobs, const Xpol
         }
     }
     hipLaunchKernelGGL(k, grid, block, lds, s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_finalize(const Plan& p, const float* acc, float* out, double scale, int nslices, hipStream_t s)
-{
-    const long long n = p.M + 1;
-    const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, s, const_cast<float*>(acc), out, p.M, p.N1,
-                       p.N2, scale, nslices, 2 * (p.M + 1));
-    return hipGetLastError();
-}
-
-hipError_t launch_finalize_xpol(const Plan& p, const float* acc, float* out, double scale, int nslices,
-                                hipStream_t s)
-{
-    const long long n = p.M + 1;
-    const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_finalize_xpol, dim3(blocks), dim3(256), 0, s, const_cast<float*>(acc), out, p.M,
-                       p.N1, p.N2, scale, nslices, 4 * (p.M + 1));
     return hipGetLastError();
 }
 
