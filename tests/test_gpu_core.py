@@ -154,6 +154,11 @@ def test_row_stream_with_pcal_matches_one_stream(hip, oracle):
                                            (1 << 20, "Hann", True),
                                            # 8-column row fill (10^6: columns 625 = 25 x 25)
                                            (1000000, "Hann", False), (1000000, "Blackman", True),
+                                           # row fill without the fused first stage: 8 columns of
+                                           # 384 = 16 x 24 (16 points per thread), 4 columns of
+                                           # 768 = 16 x 48 (16) and of 1200 = 16 x 75 (32)
+                                           (307200, "Hann", False), (1572864, "Cosine2", True),
+                                           (3072000, "Hamming", False),
                                            # the fused fill in the kernels with the plan's own strides
                                            # (1250 x 1600, 32 points per thread; 1000 x 1000, 16)
                                            (4000000, "Cosine2", False), (2000000, "Hann", True)])
