@@ -341,6 +341,10 @@ def test_xpol_3p2m_constant_stride_kernels(hip, oracle):
     (1000000, {"SWSPEC_P2_MAXPTS": "32", "SWSPEC_COMPOSITE": "0"}),
     (1000000, {"SWSPEC_P1_MAXPTS": "16"}),
     (4096, {"SWSPEC_COMPOSITE": "0"}),   # single-pass short kernel, two sources, plain radices
+    (16384, {"SWSPEC_COMPOSITE": "0"}),  # the same with 32 points per thread
+    (1000000, {"SWSPEC_COMPOSITE": "0", "SWSPEC_P1_MAXPTS": "32"}),   # 8-column row fill, plain, 32 points
+    (524288, {"SWSPEC_COMPOSITE": "0"}),   # rows 512: plain first radix 8
+    (294912, {"SWSPEC_COMPOSITE": "0"}),   # rows 384: plain first radix 3
 ])
 def test_xpol_plan_variants(hip, oracle, n, knobs):
     """Cross-pol pairs on the plan variants the autotuner can choose (plain
@@ -370,6 +374,30 @@ def test_xpol_plan_variants(hip, oracle, n, knobs):
     ea, eb, ex = relerr(auto[0], a64[0]), relerr(auto[1], a64[1]), relerr(xp[0], x64[0])
     say("  auto %.3e %.3e xpol %.3e" % (ea, eb, ex))
     assert ea <= TOL and eb <= TOL and ex <= TOL
+
+
+def test_short_plain_radices_single_source(hip, oracle):
+    """Single-pass short FFT with 32 points per thread on plain radices (the
+    autotuner's cx_mask 0 at 16384 points), one source, vs the FP64 oracle."""
+    import os
+    n, nf, fs = 16384, 6, 32e6
+    s = hip.derive_settings(FFTpoints=n, FFTIntegrationTimeSec=n * nf / fs, FFToverlapFactor=1, WindowType="Hamming",
+                            BandwidthHz=fs / 2, BitsPerSample=2, SourceChannels=1, MaxSourceBufferMB=4096)
+    raw = synth.raw_1ch(n + 3, n * nf, tones=[(0.2, 4.8e6, 0.3)], fs=fs)
+    os.environ["SWSPEC_COMPOSITE"] = "0"
+    try:
+        info = hip.plan_info(n)
+        m, auto, _, _, core = run_device(s, [raw])
+    finally:
+        del os.environ["SWSPEC_COMPOSITE"]
+    core.finalize()
+    assert info["single"] == 1 and info["p1_maxpts"] == 32 and info["p1_cx"] == 0, info
+    os_ = oracle_settings(oracle, s)
+    _, a64, _, _ = oracle.domaths(os_, [raw], 64)
+    assert m == 1
+    e = relerr(auto[0], a64[0])
+    say("  short plain 32: err vs fp64 %.3e" % e)
+    assert e <= TOL
 
 
 def test_schedule_overlap_partial_multispectrum(hip, oracle):

@@ -18,9 +18,11 @@ TOL = 1e-5
 
 # (N, window, FFTs): column lengths 512 (2^20), 1280 with a partly filled
 # column tile (3.2M), 128 with one column per workgroup (32000), 64 where the
-# MFMA stage is the whole column FFT (8192, 4096), 320 (320000)
+# MFMA stage is the whole column FFT (8192, 4096), 320 (320000); 640 (819200)
+# and 1024 (2^21) with a composite radix (10, 16) after the MFMA stage
 SIZES = [(1 << 20, "Cosine", 2), (3200000, "Cosine2", 2), (65536, "Hann", 3), (32000, "Hann", 4),
-         (8192, "Hamming", 5), (4096, "Blackman", 5), (320000, "Blackman", 2)]
+         (8192, "Hamming", 5), (4096, "Blackman", 5), (320000, "Blackman", 2),
+         (819200, "Hann", 2), (1 << 21, "Cosine2", 2)]
 
 
 @pytest.mark.parametrize("n,win,nf", SIZES)
